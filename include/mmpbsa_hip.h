@@ -1,0 +1,224 @@
+/*
+ * mmpbsa_hip.h -- C ABI of the MI355X (gfx950) MM-PBSA solvation-energy engine.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes and
+ * returns an int status (MMPBSA_OK == 0).  No C++ exception crosses the
+ * boundary; the message of the last failure on a context is available from
+ * mmpbsa_last_error().  Coordinates are Angstrom, AoS [n_atoms][3] float64,
+ * exactly as the reference crate's `&[[f64; 3]]` arguments.
+ *
+ * Reference interface replaced (rust-simulation-tools 0.2.4, crate rst-mmpbsa):
+ *   crates/mmpbsa/src/gb_energy.rs   GbModel :13, GbParams :24-56, compute_gb_energy :357
+ *   crates/mmpbsa/src/mm_energy.rs   MmEnergy :43-58, compute_mm_energy :168-172
+ *   crates/mmpbsa/src/pb_energy.rs   PbParams :21-66, compute_pb_energy :332-438
+ *   crates/mmpbsa/src/sa_energy.rs   SaParams :12-32, compute_sa_energy :51-75
+ *   crates/mmpbsa/src/binding.rs     FrameEnergy :34-56, compute_binding_energy_single_frame :749
+ *   crates/mmpbsa/src/decomposition.rs  decompose_binding_energy :75, ResidueContribution :29-48
+ *   crates/core/src/amber/prmtop.rs  AmberTopology :34-105 (the topology view below)
+ * Threading: a context is externally synchronised (one host thread at a time);
+ * several contexts may share the device.
+ */
+#ifndef MMPBSA_HIP_H
+#define MMPBSA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MMPBSA_OK 0
+#define MMPBSA_ERR_INVALID_ARG 1   /* null pointer, bad size, bad parameter   */
+#define MMPBSA_ERR_BAD_RESIDUE 2   /* residue index out of range (prmtop.rs:167-175) */
+#define MMPBSA_ERR_DEVICE 3        /* HIP runtime / kernel launch failure      */
+#define MMPBSA_ERR_OOM 4           /* device allocation failed / over budget   */
+#define MMPBSA_ERR_NO_DEVICE 5     /* no gfx950 device visible                 */
+#define MMPBSA_ERR_NOT_READY 6     /* server ticket not finished yet           */
+#define MMPBSA_ERR_TOO_LARGE 7     /* request exceeds a fixed capacity         */
+
+/* ---- topology view: caller-owned, read-only; copied at context creation --
+ * Mirrors AmberTopology (crates/core/src/amber/prmtop.rs:34-105).
+ * All indices are 0-based.  excluded_atoms uses -1 for AMBER's placeholder
+ * entry (the reference's usize::MAX, prmtop.rs:674-683).                   */
+typedef struct mmpbsa_topology_view {
+    int32_t n_atoms;
+    int32_t n_residues;
+    int32_t n_types;
+    const double *charges;        /* [n_atoms] elementary charges (PB)         */
+    const double *charges_amber;  /* [n_atoms] charge * 18.2223 (GB, MM)       */
+    const double *radii;          /* [n_atoms] intrinsic radii, Angstrom       */
+    const double *screen;         /* [n_atoms] HCT screening factors           */
+    const int32_t *atom_type;     /* [n_atoms] 0-based LJ type                 */
+    const int64_t *nb_parm_index; /* [n_types*n_types] 1-based, <1 = no LJ     */
+    const double *lj_acoef;       /* [n_lj]                                    */
+    const double *lj_bcoef;       /* [n_lj]                                    */
+    int32_t n_lj;
+    const int32_t *residue_pointers; /* [n_residues] first atom of residue     */
+    int32_t n_bonds;
+    const int32_t *bonds;         /* [n_bonds][3]: i, j, type                  */
+    const double *bond_k;         /* per bond type */
+    const double *bond_r0;
+    int32_t n_bond_types;
+    int32_t n_angles;
+    const int32_t *angles;        /* [n_angles][4]: i, j, k, type              */
+    const double *angle_k;
+    const double *angle_theta0;
+    int32_t n_angle_types;
+    int32_t n_dihedrals;
+    const int32_t *dihedrals;     /* [n_dihedrals][6]: i, j, k, l, type, ignore_14 */
+    const double *dihedral_pk;
+    const double *dihedral_pn;
+    const double *dihedral_phase;
+    int32_t n_dihedral_types;
+    const int32_t *num_excluded;  /* [n_atoms] NUMBER_EXCLUDED_ATOMS           */
+    const int32_t *excluded_atoms;/* flat EXCLUDED_ATOMS_LIST, -1 = placeholder */
+    int64_t n_excluded;           /* length of excluded_atoms                  */
+    double scee;                  /* SCEE_SCALE_FACTOR (default 1.2)           */
+    double scnb;                  /* SCNB_SCALE_FACTOR (default 2.0)           */
+} mmpbsa_topology_view;
+
+/* ---- parameter blocks (defaults = the reference's Default impls) -------- */
+enum { MMPBSA_GB_HCT = 0, MMPBSA_GB_OBC1 = 1, MMPBSA_GB_OBC2 = 2 }; /* gb_energy.rs:13-21 */
+
+typedef struct mmpbsa_gb_params {   /* gb_energy.rs:24-56 */
+    int32_t model;                  /* default OBC-I */
+    double solute_dielectric;       /* 1.0  */
+    double solvent_dielectric;      /* 78.3 */
+    double salt_concentration;      /* 0.15 */
+    double temperature;             /* 300  */
+    double offset;                  /* 0.09 */
+    double rgbmax;                  /* 25.0 */
+    double cutoff;                  /* 9999 */
+} mmpbsa_gb_params;
+
+typedef struct mmpbsa_pb_params {   /* pb_energy.rs:21-66 */
+    double grid_spacing;            /* 0.5   */
+    double grid_buffer;             /* 10.0  */
+    double solute_dielectric;       /* 1.0   */
+    double solvent_dielectric;      /* 80.0  */
+    double salt_concentration;      /* 0.15  */
+    double temperature;             /* 300   */
+    double probe_radius;            /* 1.4   */
+    double ion_radius;              /* 2.0   */
+    double tolerance;               /* 1e-6  */
+    int64_t max_iterations;         /* 10000 */
+    double fillratio;               /* 4.0   */
+    int64_t fscale;                 /* 8     */
+    /* Extension (no counterpart in PbParams): when fine_dims[0] > 0 the fine
+     * grid uses these dimensions instead of auto_grid's, centred on the
+     * subsystem's bounding-box centre exactly as auto_grid centres
+     * (pb_grid.rs:188-215).  Zero keeps the reference behaviour.           */
+    int32_t fine_dims[3];
+} mmpbsa_pb_params;
+
+typedef struct mmpbsa_sa_params {   /* sa_energy.rs:12-32 */
+    double surface_tension;         /* 0.0072 */
+    double offset;                  /* 0.0    */
+    double probe_radius;            /* 1.4    */
+    int32_t n_sphere_points;        /* 480    */
+} mmpbsa_sa_params;
+
+void mmpbsa_gb_params_default(mmpbsa_gb_params *p);
+void mmpbsa_pb_params_default(mmpbsa_pb_params *p);
+void mmpbsa_sa_params_default(mmpbsa_sa_params *p);
+
+/* ---- result blocks ------------------------------------------------------ */
+typedef struct mmpbsa_mm_terms {    /* MmEnergy, mm_energy.rs:43-58 */
+    double bond, angle, dihedral, vdw, elec, vdw_14, elec_14;
+} mmpbsa_mm_terms;
+
+typedef struct mmpbsa_pb_solve_info {   /* PbSolveResult, pb_solver.rs:143-152 */
+    double energy;          /* 1/2 sum q phi on the fine grid, kcal/mol     */
+    double final_residual;  /* RMS residual of the fine solve               */
+    int32_t iterations;     /* V-cycles (or SOR sweeps) of the fine solve  */
+    int32_t converged;      /* 1 if the fine solve met the tolerance        */
+    int32_t focused;        /* 1 if the coarse focusing solve was accepted  */
+    int32_t coarse_iterations;
+} mmpbsa_pb_solve_info;
+
+typedef struct mmpbsa_pb_result {   /* PbEnergy, pb_energy.rs:68-73 + diagnostics */
+    double total;           /* E_solv - E_ref */
+    mmpbsa_pb_solve_info solv;
+    mmpbsa_pb_solve_info ref;
+    int32_t fine_dims[3];
+    int32_t coarse_dims[3];
+} mmpbsa_pb_result;
+
+typedef struct mmpbsa_subsystem_energy { double mm, polar, sa; } mmpbsa_subsystem_energy; /* binding.rs:34-45 */
+
+typedef struct mmpbsa_frame_terms {    /* FrameEnergy, binding.rs:47-56 */
+    mmpbsa_subsystem_energy complex_, receptor, ligand;
+    double delta_mm, delta_polar, delta_sa, delta_total;
+} mmpbsa_frame_terms;
+
+typedef struct mmpbsa_residue_contrib { /* ResidueContribution, decomposition.rs:29-42 */
+    int32_t residue_index;  /* 0-based index in the caller's topology */
+    double vdw, elec, polar, sa;
+} mmpbsa_residue_contrib;
+
+enum { MMPBSA_METHOD_GB = 0, MMPBSA_METHOD_PB = 1 };  /* SolvationMethod, binding.rs:24-31 */
+
+/* ---- library ------------------------------------------------------------ */
+/* Returns MMPBSA_OK when a gfx950 device is usable; *n_devices may be NULL. */
+int mmpbsa_init(int32_t *n_devices);
+const char *mmpbsa_version(void);
+
+/* ---- context: one topology resident on the device ----------------------- */
+typedef struct mmpbsa_ctx mmpbsa_ctx;
+
+/* Copies the view (host and device); the caller may free it afterwards. */
+int mmpbsa_ctx_create(const mmpbsa_topology_view *top, int32_t device, mmpbsa_ctx **out);
+int mmpbsa_ctx_destroy(mmpbsa_ctx *ctx);
+const char *mmpbsa_last_error(const mmpbsa_ctx *ctx);
+int32_t mmpbsa_ctx_n_atoms(const mmpbsa_ctx *ctx);
+
+/* compute_mm_energy (mm_energy.rs:168) */
+int mmpbsa_mm_energy(mmpbsa_ctx *ctx, const double *xyz, mmpbsa_mm_terms *out);
+/* compute_gb_energy (gb_energy.rs:357); born_radii_out [n_atoms] may be NULL */
+int mmpbsa_gb_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_gb_params *p,
+                     double *born_radii_out, double *total_out);
+/* compute_sa_energy (sa_energy.rs:51); per_atom_sasa_out may be NULL */
+int mmpbsa_sa_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_sa_params *p,
+                     double *per_atom_sasa_out, double *total_sasa_out, double *total_out);
+/* compute_pb_energy (pb_energy.rs:332) */
+int mmpbsa_pb_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_pb_params *p,
+                     mmpbsa_pb_result *out);
+
+/* ---- receptor / ligand split (binding.rs:142-211, subsystem.rs:22-169) -- */
+typedef struct mmpbsa_split mmpbsa_split;
+
+/* Residue lists are 0-based indices into ctx's topology.  Returns
+ * MMPBSA_ERR_BAD_RESIDUE for an index >= n_residues (prmtop.rs:167-175). */
+int mmpbsa_split_create(mmpbsa_ctx *ctx, const int32_t *receptor_res, int32_t n_rec,
+                        const int32_t *ligand_res, int32_t n_lig, mmpbsa_split **out);
+int mmpbsa_split_destroy(mmpbsa_split *s);
+/* which: 0 = complex atoms (in the caller's topology), 1 = receptor atoms,
+ * 2 = ligand atoms (both in the effective complex).  Copies up to cap ints. */
+int mmpbsa_split_atoms(const mmpbsa_split *s, int32_t which, int32_t *out, int32_t cap,
+                       int32_t *n_out);
+
+/* compute_frame_energy (binding.rs:333-415).  xyz holds all atoms of the
+ * caller's topology (the split slices the complex out, binding.rs:213-222). */
+int mmpbsa_frame_energy(mmpbsa_split *s, const double *xyz, int32_t method,
+                        const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
+                        const mmpbsa_sa_params *sa, mmpbsa_frame_terms *out);
+
+/* Many frames of one split.  mode 0 = plain (host-synchronous, one frame at a
+ * time), mode 1 = batched (frames overlapped on device streams, graph-replayed
+ * solver cycles).  xyz_frames is [n_frames][n_atoms][3]. */
+int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_frames,
+                         int32_t method, const mmpbsa_gb_params *gb,
+                         const mmpbsa_pb_params *pb, const mmpbsa_sa_params *sa,
+                         int32_t mode, mmpbsa_frame_terms *out);
+
+/* decompose_binding_energy (decomposition.rs:75).  rec_out/lig_out have one
+ * entry per requested residue, in the order given at split creation. */
+int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params *gb,
+                     const mmpbsa_sa_params *sa, mmpbsa_residue_contrib *rec_out,
+                     mmpbsa_residue_contrib *lig_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MMPBSA_HIP_H */

@@ -1,0 +1,573 @@
+// capi.cpp -- the extern "C" boundary (include/mmpbsa_hip.h): contexts,
+// receptor/ligand splits, frame energies, batched frames, decomposition.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <set>
+#include <thread>
+
+#include "decomp.h"
+#include "engine.h"
+
+using namespace mmpbsa;
+
+struct mmpbsa_ctx {
+    DevTopo top;
+    std::unique_ptr<Worker> w;
+    std::string err;
+    int device = 0;
+};
+
+struct mmpbsa_split {
+    mmpbsa_ctx *ctx = nullptr;
+    bool identity = true;
+    std::vector<int> csel, ra, la;          // complex atoms; rec / lig atoms in the complex
+    std::vector<int> rec_res, lig_res;      // residues as requested (caller numbering)
+    std::vector<int> rec_loc, lig_loc;      // the same residues in the complex numbering
+    DevTopo ctop, rtop, ltop;
+    DArr<int> d_csel, d_ra, d_la;
+    // decomposition: complex residue -> positions in [rec atoms..., lig atoms...]
+    DArr<int> d_seg_pos, d_seg_ptr;
+    int cross_excl = 0;
+    std::vector<std::unique_ptr<Worker>> workers;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(mmpbsa_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    g_err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(mmpbsa_ctx *ctx, F &&f)
+{
+    try {
+        f();
+        return MMPBSA_OK;
+    } catch (const DeviceError &e) {
+        return fail(ctx, e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, MMPBSA_ERR_OOM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(ctx, MMPBSA_ERR_DEVICE, e.what());
+    }
+}
+
+bool g_device_ok = false;
+
+void require_device()
+{
+    if (g_device_ok) return;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        throw DeviceError(MMPBSA_ERR_NO_DEVICE, "no HIP device visible");
+    }
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, 0));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        throw DeviceError(MMPBSA_ERR_NO_DEVICE,
+                          std::string("device is ") + prop.gcnArchName + ", library built for gfx950");
+    g_device_ok = true;
+}
+
+std::vector<double> gather_host(const double *xyz, const std::vector<int> &idx)
+{
+    std::vector<double> out(idx.size() * 3);
+    for (size_t k = 0; k < idx.size(); ++k)
+        for (int c = 0; c < 3; ++c) out[3 * k + c] = xyz[3 * (size_t)idx[k] + c];
+    return out;
+}
+
+struct FrameCoords {
+    DBuf<double> full, c, r, l;
+    const double *dc = nullptr;
+    std::vector<double> hc, hr, hl;
+    const double *hcp = nullptr;
+};
+
+// Upload one frame and slice the complex, receptor and ligand coordinates
+// (binding.rs:213-222, :345-346) on the device.
+void stage_frame(Worker &w, mmpbsa_split *s, const double *xyz, FrameCoords &f)
+{
+    const int n_all = s->ctx->top.h.n;
+    f.full = DBuf<double>(w.pool, 3 * (size_t)n_all);
+    HIP_OK(hipMemcpyAsync(f.full.get(), xyz, sizeof(double) * 3 * n_all, hipMemcpyHostToDevice,
+                          w.stream));
+    if (s->identity) {
+        f.dc = f.full.get();
+        f.hcp = xyz;
+    } else {
+        f.c = DBuf<double>(w.pool, 3 * s->csel.size());
+        launch_gather_xyz(f.full.get(), s->d_csel.p, (int)s->csel.size(), f.c.get(), w.stream);
+        f.dc = f.c.get();
+        f.hc = gather_host(xyz, s->csel);
+        f.hcp = f.hc.data();
+    }
+    f.r = DBuf<double>(w.pool, 3 * s->ra.size() + 3);
+    f.l = DBuf<double>(w.pool, 3 * s->la.size() + 3);
+    launch_gather_xyz(f.dc, s->d_ra.p, (int)s->ra.size(), f.r.get(), w.stream);
+    launch_gather_xyz(f.dc, s->d_la.p, (int)s->la.size(), f.l.get(), w.stream);
+    f.hr = gather_host(f.hcp, s->ra);
+    f.hl = gather_host(f.hcp, s->la);
+}
+
+double polar_of(Worker &w, const DevTopo &t, const double *d, const double *h, int method,
+                const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb)
+{
+    if (method == MMPBSA_METHOD_PB) {
+        mmpbsa_pb_result r;
+        run_pb(w, t, d, h, pb, &r);
+        return r.total;
+    }
+    return run_gb(w, t, d, gb, nullptr, nullptr);
+}
+
+double mm_total(const mmpbsa_mm_terms &m)
+{
+    return m.bond + m.angle + m.dihedral + m.vdw + m.elec + m.vdw_14 + m.elec_14;
+}
+
+// compute_frame_energy (binding.rs:333-415)
+void frame_energy(Worker &w, mmpbsa_split *s, const double *xyz, int method,
+                  const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb,
+                  const mmpbsa_sa_params &sa, mmpbsa_frame_terms *out)
+{
+    FrameCoords f;
+    stage_frame(w, s, xyz, f);
+    const DevTopo *tops[3] = {&s->ctop, &s->rtop, &s->ltop};
+    const double *ds[3] = {f.dc, f.r.get(), f.l.get()};
+    const double *hs[3] = {f.hcp, f.hr.data(), f.hl.data()};
+    mmpbsa_subsystem_energy *e[3] = {&out->complex_, &out->receptor, &out->ligand};
+    for (int k = 0; k < 3; ++k) {
+        mmpbsa_mm_terms mm;
+        run_mm(w, *tops[k], ds[k], &mm);
+        e[k]->mm = mm_total(mm);
+        e[k]->polar = polar_of(w, *tops[k], ds[k], hs[k], method, gb, pb);
+        double tsasa = 0.0, tsa = 0.0;
+        run_sa(w, *tops[k], ds[k], hs[k], sa, nullptr, nullptr, &tsasa, &tsa);
+        e[k]->sa = tsa;
+    }
+    out->delta_mm = out->complex_.mm - out->receptor.mm - out->ligand.mm;
+    out->delta_polar = out->complex_.polar - out->receptor.polar - out->ligand.polar;
+    out->delta_sa = out->complex_.sa - out->receptor.sa - out->ligand.sa;
+    out->delta_total = out->delta_mm + out->delta_polar + out->delta_sa;
+}
+
+void check_xyz(const double *xyz) {
+    if (!xyz) throw DeviceError(MMPBSA_ERR_INVALID_ARG, "null coordinates");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mmpbsa_version(void) { return "mmpbsa-hip 0.1.0 (gfx950)"; }
+
+int mmpbsa_init(int32_t *n_devices)
+{
+    return guarded(nullptr, [&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) {
+            (void)hipGetLastError();
+            n = 0;
+        }
+        if (n_devices) *n_devices = n;
+        require_device();
+    });
+}
+
+void mmpbsa_gb_params_default(mmpbsa_gb_params *p)
+{
+    if (!p) return;
+    p->model = MMPBSA_GB_OBC1;
+    p->solute_dielectric = 1.0;
+    p->solvent_dielectric = 78.3;
+    p->salt_concentration = 0.15;
+    p->temperature = 300.0;
+    p->offset = 0.09;
+    p->rgbmax = 25.0;
+    p->cutoff = 9999.0;
+}
+
+void mmpbsa_pb_params_default(mmpbsa_pb_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->grid_spacing = 0.5;
+    p->grid_buffer = 10.0;
+    p->solute_dielectric = 1.0;
+    p->solvent_dielectric = 80.0;
+    p->salt_concentration = 0.15;
+    p->temperature = 300.0;
+    p->probe_radius = 1.4;
+    p->ion_radius = 2.0;
+    p->tolerance = 1e-6;
+    p->max_iterations = 10000;
+    p->fillratio = 4.0;
+    p->fscale = 8;
+}
+
+void mmpbsa_sa_params_default(mmpbsa_sa_params *p)
+{
+    if (!p) return;
+    p->surface_tension = 0.0072;
+    p->offset = 0.0;
+    p->probe_radius = 1.4;
+    p->n_sphere_points = 480;
+}
+
+int mmpbsa_ctx_create(const mmpbsa_topology_view *top, int32_t device, mmpbsa_ctx **out)
+{
+    if (!top || !out) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<mmpbsa_ctx> c(new mmpbsa_ctx());
+    const int rc = guarded(nullptr, [&] {
+        require_device();
+        HIP_OK(hipSetDevice(device));
+        c->device = device;
+        c->top.build(HostTopo::from_view(*top));
+        c->w.reset(new Worker());
+    });
+    if (rc == MMPBSA_OK) *out = c.release();
+    return rc;
+}
+
+int mmpbsa_ctx_destroy(mmpbsa_ctx *ctx)
+{
+    delete ctx;
+    return MMPBSA_OK;
+}
+
+const char *mmpbsa_last_error(const mmpbsa_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_err.c_str();
+}
+
+int32_t mmpbsa_ctx_n_atoms(const mmpbsa_ctx *ctx) { return ctx ? ctx->top.h.n : -1; }
+
+int mmpbsa_mm_energy(mmpbsa_ctx *ctx, const double *xyz, mmpbsa_mm_terms *out)
+{
+    if (!ctx || !out) return fail(ctx, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        Worker &w = *ctx->w;
+        DBuf<double> d(w.pool, 3 * (size_t)ctx->top.h.n + 3);
+        HIP_OK(hipMemcpyAsync(d.get(), xyz, sizeof(double) * 3 * ctx->top.h.n,
+                              hipMemcpyHostToDevice, w.stream));
+        run_mm(w, ctx->top, d.get(), out);
+    });
+}
+
+int mmpbsa_gb_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_gb_params *p,
+                     double *born_radii_out, double *total_out)
+{
+    if (!ctx || !p || !total_out) return fail(ctx, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        Worker &w = *ctx->w;
+        const int n = ctx->top.h.n;
+        DBuf<double> d(w.pool, 3 * (size_t)n + 3);
+        HIP_OK(hipMemcpyAsync(d.get(), xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, w.stream));
+        std::vector<double> born;
+        *total_out = run_gb(w, ctx->top, d.get(), *p, nullptr, born_radii_out ? &born : nullptr);
+        if (born_radii_out && n) std::memcpy(born_radii_out, born.data(), sizeof(double) * n);
+    });
+}
+
+int mmpbsa_sa_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_sa_params *p,
+                     double *per_atom_sasa_out, double *total_sasa_out, double *total_out)
+{
+    if (!ctx || !p) return fail(ctx, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        Worker &w = *ctx->w;
+        const int n = ctx->top.h.n;
+        DBuf<double> d(w.pool, 3 * (size_t)n + 3);
+        HIP_OK(hipMemcpyAsync(d.get(), xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, w.stream));
+        std::vector<double> per;
+        double ts = 0.0, t = 0.0;
+        run_sa(w, ctx->top, d.get(), xyz, *p, nullptr, per_atom_sasa_out ? &per : nullptr, &ts, &t);
+        if (per_atom_sasa_out && n) std::memcpy(per_atom_sasa_out, per.data(), sizeof(double) * n);
+        if (total_sasa_out) *total_sasa_out = ts;
+        if (total_out) *total_out = t;
+    });
+}
+
+int mmpbsa_pb_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_pb_params *p,
+                     mmpbsa_pb_result *out)
+{
+    if (!ctx || !p || !out) return fail(ctx, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        Worker &w = *ctx->w;
+        const int n = ctx->top.h.n;
+        DBuf<double> d(w.pool, 3 * (size_t)n + 3);
+        HIP_OK(hipMemcpyAsync(d.get(), xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, w.stream));
+        run_pb(w, ctx->top, d.get(), xyz, *p, out);
+    });
+}
+
+// build_complex_subsystem (binding.rs:142-211) + extract_subtopology x2
+int mmpbsa_split_create(mmpbsa_ctx *ctx, const int32_t *receptor_res, int32_t n_rec,
+                        const int32_t *ligand_res, int32_t n_lig, mmpbsa_split **out)
+{
+    if (!ctx || !out || n_rec < 0 || n_lig < 0 || (n_rec && !receptor_res) || (n_lig && !ligand_res))
+        return fail(ctx, MMPBSA_ERR_INVALID_ARG, "invalid argument");
+    *out = nullptr;
+    std::unique_ptr<mmpbsa_split> s(new mmpbsa_split());
+    const int rc = guarded(ctx, [&] {
+        const HostTopo &T = ctx->top.h;
+        s->ctx = ctx;
+        s->rec_res.assign(receptor_res, receptor_res + n_rec);
+        s->lig_res.assign(ligand_res, ligand_res + n_lig);
+        std::set<int> cres(s->rec_res.begin(), s->rec_res.end());
+        cres.insert(s->lig_res.begin(), s->lig_res.end());
+        const std::vector<int> cresv(cres.begin(), cres.end());
+        const std::vector<int> csel = T.build_selection(cresv);
+        HostTopo ctop;
+        if ((int)csel.size() == T.n) {
+            s->identity = true;
+            ctop = T;
+            s->rec_loc = s->rec_res;
+            s->lig_loc = s->lig_res;
+        } else {
+            s->identity = false;
+            s->csel = csel;
+            ctop = T.extract(csel);
+            for (int r : s->rec_res)
+                s->rec_loc.push_back((int)(std::lower_bound(cresv.begin(), cresv.end(), r) - cresv.begin()));
+            for (int r : s->lig_res)
+                s->lig_loc.push_back((int)(std::lower_bound(cresv.begin(), cresv.end(), r) - cresv.begin()));
+        }
+        s->ra = ctop.build_selection(s->rec_loc);
+        s->la = ctop.build_selection(s->lig_loc);
+        s->ctop.build(ctop);
+        s->rtop.build(ctop.extract(s->ra));
+        s->ltop.build(ctop.extract(s->la));
+        s->d_csel.upload(s->csel);
+        s->d_ra.upload(s->ra);
+        s->d_la.upload(s->la);
+        // residue -> positions in [ra..., la...] for the decomposition sums
+        const std::vector<int> ares = ctop.atom_residue_indices();
+        std::vector<std::vector<int>> pos(ctop.n_res);
+        for (size_t k = 0; k < s->ra.size(); ++k) pos[ares[s->ra[k]]].push_back((int)k);
+        for (size_t k = 0; k < s->la.size(); ++k)
+            pos[ares[s->la[k]]].push_back((int)(s->ra.size() + k));
+        std::vector<int> sp, sptr(1, 0);
+        for (auto &v : pos) {
+            sp.insert(sp.end(), v.begin(), v.end());
+            sptr.push_back((int)sp.size());
+        }
+        s->d_seg_pos.upload(sp);
+        s->d_seg_ptr.upload(sptr);
+        // any excluded / 1-4 pair between a receptor and a ligand atom?
+        std::vector<char> is_r(ctop.n, 0), is_l(ctop.n, 0);
+        for (int a : s->ra) is_r[a] = 1;
+        for (int a : s->la) is_l[a] = 1;
+        auto scan = [&](const std::vector<int> &ptr, const std::vector<int> &col) {
+            for (int i = 0; i < ctop.n; ++i)
+                for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+                    const int j = col[k];
+                    if ((is_r[i] && is_l[j]) || (is_l[i] && is_r[j])) s->cross_excl = 1;
+                }
+        };
+        scan(s->ctop.h_ex_ptr, s->ctop.h_ex_col);
+        scan(s->ctop.h_p14_ptr, s->ctop.h_p14_col);
+    });
+    if (rc == MMPBSA_OK) *out = s.release();
+    return rc;
+}
+
+int mmpbsa_split_destroy(mmpbsa_split *s)
+{
+    delete s;
+    return MMPBSA_OK;
+}
+
+int mmpbsa_split_atoms(const mmpbsa_split *s, int32_t which, int32_t *out, int32_t cap,
+                       int32_t *n_out)
+{
+    if (!s || which < 0 || which > 2) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "invalid argument");
+    std::vector<int> v;
+    if (which == 0) {
+        if (s->identity) {
+            v.resize(s->ctx->top.h.n);
+            for (int i = 0; i < (int)v.size(); ++i) v[i] = i;
+        } else {
+            v = s->csel;
+        }
+    } else {
+        v = which == 1 ? s->ra : s->la;
+    }
+    if (n_out) *n_out = (int32_t)v.size();
+    if (out) std::memcpy(out, v.data(), sizeof(int32_t) * std::min<size_t>(v.size(), cap > 0 ? cap : 0));
+    return MMPBSA_OK;
+}
+
+int mmpbsa_frame_energy(mmpbsa_split *s, const double *xyz, int32_t method,
+                        const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
+                        const mmpbsa_sa_params *sa, mmpbsa_frame_terms *out)
+{
+    if (!s || !out) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    mmpbsa_ctx *ctx = s->ctx;
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        mmpbsa_gb_params g;
+        mmpbsa_pb_params p;
+        mmpbsa_sa_params a;
+        mmpbsa_gb_params_default(&g);
+        mmpbsa_pb_params_default(&p);
+        mmpbsa_sa_params_default(&a);
+        if (gb) g = *gb;
+        if (pb) p = *pb;
+        if (sa) a = *sa;
+        frame_energy(*ctx->w, s, xyz, method, g, p, a, out);
+    });
+}
+
+int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_frames,
+                         int32_t method, const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
+                         const mmpbsa_sa_params *sa, int32_t mode, mmpbsa_frame_terms *out)
+{
+    if (!s || !out || n_frames < 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "invalid argument");
+    mmpbsa_ctx *ctx = s->ctx;
+    return guarded(ctx, [&] {
+        if (n_frames == 0) return;
+        check_xyz(xyz_frames);
+        mmpbsa_gb_params g;
+        mmpbsa_pb_params p;
+        mmpbsa_sa_params a;
+        mmpbsa_gb_params_default(&g);
+        mmpbsa_pb_params_default(&p);
+        mmpbsa_sa_params_default(&a);
+        if (gb) g = *gb;
+        if (pb) p = *pb;
+        if (sa) a = *sa;
+        const size_t stride = 3 * (size_t)ctx->top.h.n;
+        if (mode == 0) {
+            for (int f = 0; f < n_frames; ++f)
+                frame_energy(*ctx->w, s, xyz_frames + stride * f, method, g, p, a, out + f);
+            return;
+        }
+        // batched: frames spread over concurrent device streams
+        const int nw = std::max(1, std::min(n_frames, 4));
+        while ((int)s->workers.size() < nw) s->workers.emplace_back(new Worker());
+        std::atomic<int> next(0);
+        std::vector<std::string> errs(nw);
+        std::vector<int> codes(nw, 0);
+        std::vector<std::thread> th;
+        for (int k = 0; k < nw; ++k) {
+            th.emplace_back([&, k] {
+                try {
+                    HIP_OK(hipSetDevice(ctx->device));
+                    for (;;) {
+                        const int f = next.fetch_add(1);
+                        if (f >= n_frames) break;
+                        frame_energy(*s->workers[k], s, xyz_frames + stride * f, method, g, p, a,
+                                     out + f);
+                    }
+                } catch (const DeviceError &e) {
+                    codes[k] = e.code;
+                    errs[k] = e.what();
+                } catch (const std::exception &e) {
+                    codes[k] = MMPBSA_ERR_DEVICE;
+                    errs[k] = e.what();
+                }
+            });
+        }
+        for (auto &t : th) t.join();
+        for (int k = 0; k < nw; ++k)
+            if (codes[k]) throw DeviceError(codes[k], errs[k]);
+    });
+}
+
+// decompose_binding_energy_impl (decomposition.rs:210-410)
+int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params *gbp,
+                     const mmpbsa_sa_params *sap, mmpbsa_residue_contrib *rec_out,
+                     mmpbsa_residue_contrib *lig_out)
+{
+    if (!s || (!rec_out && !s->rec_res.empty()) || (!lig_out && !s->lig_res.empty()))
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    mmpbsa_ctx *ctx = s->ctx;
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        mmpbsa_gb_params gb;
+        mmpbsa_sa_params sa;
+        mmpbsa_gb_params_default(&gb);
+        mmpbsa_sa_params_default(&sa);
+        if (gbp) gb = *gbp;
+        if (sap) sa = *sap;
+        Worker &w = *ctx->w;
+        FrameCoords f;
+        stage_frame(w, s, xyz, f);
+        const int nr = (int)s->ra.size(), nl = (int)s->la.size(), na = nr + nl;
+        const PairTopo cv = s->ctop.view();
+        // per-atom vdW / elec halves (receptor block, then ligand block)
+        DBuf<double> vdw(w.pool, na + 1), elec(w.pool, na + 1);
+        {
+            const size_t wr = (size_t)decomp_chunks(nl) * std::max(nr, 1) * 4;
+            const size_t wl = (size_t)decomp_chunks(nr) * std::max(nl, 1) * 4;
+            DBuf<double> work(w.pool, std::max(wr, wl));
+            launch_decomp_pairs(f.dc, cv, s->d_ra.p, nr, s->d_la.p, nl, 1, s->cross_excl,
+                                work.get(), vdw.get(), elec.get(), w.stream);
+            launch_decomp_pairs(f.dc, cv, s->d_la.p, nl, s->d_ra.p, nr, 0, s->cross_excl,
+                                work.get(), vdw.get() + nr, elec.get() + nr, w.stream);
+        }
+        // GB shares
+        DBuf<double> Rc(w.pool, s->ctop.h.n + 1), Rr(w.pool, nr + 1), Rl(w.pool, nl + 1);
+        const double ec = run_gb(w, s->ctop, f.dc, gb, Rc.get(), nullptr);
+        const double er = run_gb(w, s->rtop, f.r.get(), gb, Rr.get(), nullptr);
+        const double el = run_gb(w, s->ltop, f.l.get(), gb, Rl.get(), nullptr);
+        const double fac = -0.5 * (1.0 / gb.solute_dielectric - 1.0 / gb.solvent_dielectric);
+        DBuf<double> wts(w.pool, na + 1), scal(w.pool, 2), pol(w.pool, na + 1);
+        launch_gb_weight(s->d_ra.p, nr, s->ctop.qa.p, Rc.get(), Rr.get(), fac, wts.get(), w.stream);
+        launch_gb_weight(s->d_la.p, nl, s->ctop.qa.p, Rc.get(), Rl.get(), fac, wts.get() + nr,
+                         w.stream);
+        launch_fill_f64(scal.get(), ec - er - el, 1, w.stream);
+        launch_sum_partials(wts.get(), na, 1, 1, scal.get() + 1, w.stream);
+        launch_gb_share(wts.get(), na, scal.get(), pol.get(), w.stream);
+        // SA shares
+        DBuf<double> Sc(w.pool, s->ctop.h.n + 1), Sr(w.pool, nr + 1), Sl(w.pool, nl + 1);
+        double ts, tt;
+        run_sa(w, s->ctop, f.dc, f.hcp, sa, Sc.get(), nullptr, &ts, &tt);
+        run_sa(w, s->rtop, f.r.get(), f.hr.data(), sa, Sr.get(), nullptr, &ts, &tt);
+        run_sa(w, s->ltop, f.l.get(), f.hl.data(), sa, Sl.get(), nullptr, &ts, &tt);
+        DBuf<double> sav(w.pool, na + 1);
+        launch_sa_share(s->d_ra.p, nr, Sc.get(), Sr.get(), sa.surface_tension, sav.get(), w.stream);
+        launch_sa_share(s->d_la.p, nl, Sc.get(), Sl.get(), sa.surface_tension, sav.get() + nr,
+                        w.stream);
+        // per-residue sums (complex numbering)
+        const int ncr = s->ctop.h.n_res;
+        DBuf<double> res(w.pool, 4 * (size_t)std::max(ncr, 1));
+        launch_segment_sum(vdw.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get(), w.stream);
+        launch_segment_sum(elec.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + ncr, w.stream);
+        launch_segment_sum(pol.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + 2 * ncr, w.stream);
+        launch_segment_sum(sav.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + 3 * ncr, w.stream);
+        std::vector<double> h(4 * (size_t)std::max(ncr, 1));
+        HIP_OK(hipMemcpyAsync(h.data(), res.get(), sizeof(double) * 4 * ncr, hipMemcpyDeviceToHost,
+                              w.stream));
+        w.sync();
+        auto fill = [&](const std::vector<int> &orig, const std::vector<int> &loc,
+                        mmpbsa_residue_contrib *o) {
+            for (size_t k = 0; k < orig.size(); ++k) {
+                const int r = loc[k];
+                o[k].residue_index = orig[k];
+                o[k].vdw = h[r];
+                o[k].elec = h[ncr + r];
+                o[k].polar = h[2 * ncr + r];
+                o[k].sa = h[3 * ncr + r];
+            }
+        };
+        fill(s->rec_res, s->rec_loc, rec_out);
+        fill(s->lig_res, s->lig_loc, lig_out);
+    });
+}
+
+}  // extern "C"

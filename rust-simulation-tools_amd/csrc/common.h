@@ -1,0 +1,77 @@
+// common.h -- shared device helpers for the MM-PBSA engine (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace mmpbsa {
+
+struct DeviceError : std::runtime_error {
+    int code;
+    DeviceError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_OK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::mmpbsa::DeviceError(e_ == hipErrorOutOfMemory ? 4 : 3,                \
+                                        std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_OK()                                                                       \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess)                                                             \
+            throw ::mmpbsa::DeviceError(3, std::string("kernel launch: ") +               \
+                                               hipGetErrorString(e_));                    \
+    } while (0)
+
+constexpr double kCoulomb = 332.0522;
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kWave = 64;
+
+// Two-sum based double-double accumulation (error-free transform; no FMA
+// contraction is allowed in this library, see the build flags).
+struct dd {
+    double s, c;
+};
+__host__ __device__ inline void dd_add(dd &a, double v)
+{
+    const double t = a.s + v;
+    const double bp = t - a.s;
+    const double err = (a.s - (t - bp)) + (v - bp);
+    a.s = t;
+    a.c += err;
+}
+__host__ __device__ inline void dd_add(dd &a, const dd &b)
+{
+    dd_add(a, b.s);
+    a.c += b.c;
+}
+__host__ __device__ inline double dd_val(const dd &a) { return a.s + a.c; }
+
+template <typename T>
+__device__ inline T wave_sum(T v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ inline dd wave_sum_dd(dd v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        dd o;
+        o.s = __shfl_xor(v.s, off, 64);
+        o.c = __shfl_xor(v.c, off, 64);
+        dd_add(v, o);
+    }
+    return v;
+}
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+}  // namespace mmpbsa

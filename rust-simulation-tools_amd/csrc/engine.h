@@ -1,0 +1,154 @@
+// engine.h -- host runtime of the MM-PBSA engine: topology, device
+// residency, memory pool, per-term drivers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mmpbsa_hip.h"
+#include "kernels.h"
+#include "mg.h"
+
+namespace mmpbsa {
+
+// ---- host topology: AmberTopology fields the path reads (prmtop.rs:34-105)
+struct HostTopo {
+    int n = 0, n_res = 0, n_types = 0;
+    std::vector<double> q, qa, radii, screen;
+    std::vector<int> type;
+    std::vector<int64_t> nbidx;
+    std::vector<double> ljA, ljB;
+    std::vector<int> res_ptr;
+    std::vector<int> bonds, angles, dihs;  // 3 / 4 / 6 ints per term
+    std::vector<double> bond_k, bond_r0, angle_k, angle_t0, dih_pk, dih_pn, dih_ph;
+    std::vector<int> nex, exl;
+    double scee = 1.2, scnb = 2.0;
+
+    static HostTopo from_view(const mmpbsa_topology_view &v);
+    std::vector<int> atom_residue_indices() const;                    // prmtop.rs:120-132
+    std::vector<int> build_selection(const std::vector<int> &res) const;  // prmtop.rs:165-204
+    HostTopo extract(const std::vector<int> &atoms) const;            // subsystem.rs:22-169
+};
+
+// ---- device memory pool (per worker; reuse follows stream order) ----------
+class Pool {
+public:
+    ~Pool();
+    void *get(size_t bytes);
+    void put(void *p);
+    void trim();
+    size_t cached() const { return cached_; }
+
+private:
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> live_;
+    size_t cached_ = 0;
+};
+
+template <typename T>
+struct DBuf {
+    Pool *pool = nullptr;
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(Pool &pl, size_t count) : pool(&pl), n(count)
+    {
+        p = static_cast<T *>(pl.get(sizeof(T) * (count ? count : 1)));
+    }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : pool(o.pool), p(o.p), n(o.n) { o.p = nullptr; }
+    DBuf &operator=(DBuf &&o) noexcept
+    {
+        release();
+        pool = o.pool;
+        p = o.p;
+        n = o.n;
+        o.p = nullptr;
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p && pool) pool->put(p);
+        p = nullptr;
+    }
+    T *get() const { return p; }
+};
+
+// Owned device array (not pooled): topology residency.
+template <typename T>
+struct DArr {
+    T *p = nullptr;
+    size_t n = 0;
+    DArr() = default;
+    DArr(const DArr &) = delete;
+    DArr &operator=(const DArr &) = delete;
+    ~DArr()
+    {
+        if (p) (void)hipFree(p);
+    }
+    void upload(const std::vector<T> &v)
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = v.size();
+        HIP_OK(hipMalloc(&p, sizeof(T) * (n ? n : 1)));
+        if (n) HIP_OK(hipMemcpy(p, v.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+    }
+};
+
+// ---- one topology resident on the device ----------------------------------
+struct DevTopo {
+    HostTopo h;
+    double max_radius = 0.0;
+    int cross_dummy = 0;
+    DArr<double> q, qa, radii, screen, ljA, ljB;
+    DArr<int> type, ex_ptr, ex_col, p14_ptr, p14_col;
+    DArr<uint8_t> tile_flag;
+    DArr<int> bonds, angles, dihs;
+    DArr<double> bond_k, bond_r0, angle_k, angle_t0, dih_pk, dih_pn, dih_ph;
+    std::vector<int> h_ex_ptr, h_ex_col, h_p14_ptr, h_p14_col;
+    int nb = 1;
+    void build(const HostTopo &t);
+    PairTopo view() const;
+};
+
+// A worker: one stream, its pool and small pinned staging areas.
+struct Worker {
+    hipStream_t stream = nullptr;
+    Pool pool;
+    double *pin = nullptr;   // pinned host doubles
+    int *pin_i = nullptr;    // pinned host ints
+    Worker();
+    ~Worker();
+    void sync();
+};
+
+// Results of one PB solve on one grid (solve_lpbe_multigrid semantics).
+struct SolveInfo {
+    int iterations = 0;
+    int converged = 0;
+    double final_residual = 0.0;
+};
+
+// ---- drivers (device coordinates already resident on w.stream) ------------
+void run_mm(Worker &w, const DevTopo &t, const double *d_xyz, mmpbsa_mm_terms *out);
+double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_params &p,
+              double *d_born /* nullable: pooled scratch used */, std::vector<double> *born_out);
+void run_sa(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xyz,
+            const mmpbsa_sa_params &p, double *d_per_atom_out, std::vector<double> *per_atom,
+            double *total_sasa, double *total);
+void run_pb(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xyz,
+            const mmpbsa_pb_params &p, mmpbsa_pb_result *out);
+
+// FP64 kappa (gb_energy.rs:126-139)
+double compute_kappa(double salt, double eps, double temperature);
+
+}  // namespace mmpbsa

@@ -1,0 +1,682 @@
+// grid.hip -- Poisson-Boltzmann grid assembly on gfx950: cell lists, charge
+// spreading, SAS / ion-exclusion maps, SES classification, face dielectrics,
+// boundary interpolation and the energy gather.
+//
+// Geometry tests are FP64 with no contraction, in the reference's operation
+// order, so boolean maps and face values match the CPU bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "kernels.h"
+
+namespace mmpbsa {
+
+__device__ __forceinline__ int64_t gi(const GridDesc &g, int x, int y, int z)
+{
+    return (int64_t)x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+}
+
+__device__ __forceinline__ void node_of(const GridDesc &g, int64_t idx, int &x, int &y, int &z)
+{
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    z = (int)(idx / nxy);
+    const int64_t r = idx - (int64_t)z * nxy;
+    y = (int)(r / g.nx);
+    x = (int)(r - (int64_t)y * g.nx);
+}
+
+// ---- cell list -------------------------------------------------------------
+__global__ void cell_count_kernel(const double *__restrict__ xyz, int n, double ox, double oy,
+                                  double oz, double inv, int dx, int dy, int dz,
+                                  int *__restrict__ cell_of, int *__restrict__ count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int cx = (int)floor((xyz[3 * i] - ox) * inv);
+    int cy = (int)floor((xyz[3 * i + 1] - oy) * inv);
+    int cz = (int)floor((xyz[3 * i + 2] - oz) * inv);
+    cx = min(max(cx, 0), dx - 1);
+    cy = min(max(cy, 0), dy - 1);
+    cz = min(max(cz, 0), dz - 1);
+    const int c = cx + dx * (cy + dy * cz);
+    cell_of[i] = c;
+    atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of count[0..nc) into start[0..nc]; one block of 1024.
+__global__ __launch_bounds__(1024) void scan_kernel(const int *__restrict__ count, int nc,
+                                                    int *__restrict__ start)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (nc + 1023) / 1024;
+    const int a = t * per, b = min(nc, a + per);
+    int s = 0;
+    for (int k = a; k < b; ++k) s += count[k];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? part[t - 1] : 0;
+    for (int k = a; k < b; ++k) {
+        start[k] = run;
+        run += count[k];
+    }
+    if (t == 1023) start[nc] = part[1023];
+}
+
+__global__ void cell_fill_kernel(const int *__restrict__ cell_of, int n,
+                                 const int *__restrict__ start, int *__restrict__ fill,
+                                 int *__restrict__ atoms)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cell_of[i];
+    const int p = atomicAdd(&fill[c], 1);
+    atoms[start[c] + p] = i;
+}
+
+void launch_cell_build(const double *xyz, int n, double ox, double oy, double oz, double inv,
+                       int dx, int dy, int dz, int *cell_of, int *start, int *fill, int *atoms,
+                       hipStream_t s)
+{
+    const int64_t nc = (int64_t)dx * dy * dz;
+    launch_fill_i32(fill, 0, nc, s);
+    if (n > 0) {
+        hipLaunchKernelGGL(cell_count_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, xyz, n, ox,
+                           oy, oz, inv, dx, dy, dz, cell_of, fill);
+        LAUNCH_OK();
+    }
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, fill, (int)nc, start);
+    LAUNCH_OK();
+    launch_fill_i32(fill, 0, nc, s);
+    if (n > 0) {
+        hipLaunchKernelGGL(cell_fill_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, cell_of, n,
+                           start, fill, atoms);
+        LAUNCH_OK();
+    }
+}
+
+// ---- charge spreading, map_charges (pb_grid.rs:220-292) ---------------------
+// Contributions are rounded to 2^-40 fixed point and summed with integer
+// atomics: the map is independent of atom scheduling.
+constexpr double kFix = 1099511627776.0;  // 2^40
+
+__global__ void spread_kernel(GridDesc g, int n, const double *__restrict__ xyz,
+                              const double *__restrict__ q, double vol,
+                              unsigned long long *__restrict__ acc)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const double fx = (xyz[3 * a] - g.ox) / g.h;
+    const double fy = (xyz[3 * a + 1] - g.oy) / g.h;
+    const double fz = (xyz[3 * a + 2] - g.oz) / g.h;
+    const double flx = floor(fx), fly = floor(fy), flz = floor(fz);
+    if (flx < 0.0 || flx >= (double)(g.nx - 1) || fly < 0.0 || fly >= (double)(g.ny - 1) ||
+        flz < 0.0 || flz >= (double)(g.nz - 1))
+        return;
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    const double dx = fx - (double)ix, dy = fy - (double)iy, dz = fz - (double)iz;
+    const double qd = q[a] / vol;
+    const double wx[2] = {1.0 - dx, dx}, wy[2] = {1.0 - dy, dy}, wz[2] = {1.0 - dz, dz};
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const double w = qd * wx[i] * wy[j] * wz[k];
+                const long long f = llrint(w * kFix);
+                atomicAdd(&acc[gi(g, ix + i, iy + j, iz + k)], (unsigned long long)f);
+            }
+}
+
+__global__ void fixed_to_f64_kernel(const unsigned long long *__restrict__ acc, int64_t n,
+                                    double *__restrict__ rho)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rho[i] = (double)(long long)acc[i] * (1.0 / kFix);
+}
+
+void launch_spread(const GridDesc &g, int n, const double *xyz, const double *q,
+                   unsigned long long *acc, double *rho, hipStream_t s)
+{
+    const int64_t N = g.size();
+    launch_fill_u64(acc, 0ull, N, s);
+    if (n > 0) {
+        hipLaunchKernelGGL(spread_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, g, n, xyz, q,
+                           g.h * g.h * g.h, acc);
+        LAUNCH_OK();
+    }
+    hipLaunchKernelGGL(fixed_to_f64_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, acc, N, rho);
+    LAUNCH_OK();
+}
+
+// ---- inside tests (AtomSpatialHash::point_inside, pb_grid.rs:400-442) -------
+__global__ void inside_kernel(GridDesc g, const double *__restrict__ xyz,
+                              const double *__restrict__ radii, double probe, double reach,
+                              CellList cl, uint8_t *__restrict__ inside)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.size()) return;
+    int x, y, z;
+    node_of(g, idx, x, y, z);
+    const double px = g.ox + (double)x * g.h, py = g.oy + (double)y * g.h,
+                 pz = g.oz + (double)z * g.h;
+    const int cx0 = max(0, (int)floor((px - reach - cl.ox) * cl.inv));
+    const int cx1 = min(cl.dx - 1, (int)floor((px + reach - cl.ox) * cl.inv));
+    const int cy0 = max(0, (int)floor((py - reach - cl.oy) * cl.inv));
+    const int cy1 = min(cl.dy - 1, (int)floor((py + reach - cl.oy) * cl.inv));
+    const int cz0 = max(0, (int)floor((pz - reach - cl.oz) * cl.inv));
+    const int cz1 = min(cl.dz - 1, (int)floor((pz + reach - cl.oz) * cl.inv));
+    uint8_t in = 0;
+    for (int cz = cz0; cz <= cz1 && !in; ++cz)
+        for (int cy = cy0; cy <= cy1 && !in; ++cy)
+            for (int cx = cx0; cx <= cx1 && !in; ++cx) {
+                const int c = cx + cl.dx * (cy + cl.dy * cz);
+                for (int k = cl.start[c]; k < cl.start[c + 1]; ++k) {
+                    const int a = cl.atoms[k];
+                    const double dx = px - xyz[3 * a], dy = py - xyz[3 * a + 1],
+                                 dz = pz - xyz[3 * a + 2];
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const double th = radii[a] + probe;
+                    if (d2 < th * th) {
+                        in = 1;
+                        break;
+                    }
+                }
+            }
+    inside[idx] = in;
+}
+
+void launch_inside(const GridDesc &g, const double *xyz, const double *radii, double probe,
+                   double max_r, const CellList &cl, uint8_t *inside, hipStream_t s)
+{
+    const int64_t N = g.size();
+    hipLaunchKernelGGL(inside_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, xyz, radii,
+                       probe, max_r + probe, cl, inside);
+    LAUNCH_OK();
+}
+
+// ---- SES: flood fill of the exterior + bounded squared distance -----------
+// compute_inside_ses, pb_grid.rs:576-708.  Connected exterior = outside-SAS
+// nodes reachable from the box faces through outside-SAS nodes.  Line sweeps
+// propagate along maximal runs; rounds repeat until nothing changes.
+__global__ void ext_init_kernel(GridDesc g, const uint8_t *__restrict__ sas, uint8_t *__restrict__ ext)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.size()) return;
+    int x, y, z;
+    node_of(g, idx, x, y, z);
+    const bool bnd = x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1;
+    ext[idx] = (bnd && !sas[idx]) ? 1 : 0;
+}
+
+// One thread per line along `axis`; forward then backward propagation.
+__global__ void flood_line_kernel(GridDesc g, int axis, const uint8_t *__restrict__ sas,
+                                  uint8_t *__restrict__ ext, int *__restrict__ changed)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t base, stride, len;
+    if (axis == 0) {
+        if (t >= (int64_t)g.ny * g.nz) return;
+        base = t * g.nx;
+        stride = 1;
+        len = g.nx;
+    } else if (axis == 1) {
+        if (t >= (int64_t)g.nx * g.nz) return;
+        const int64_t x = t % g.nx, z = t / g.nx;
+        base = x + (int64_t)g.nx * g.ny * z;
+        stride = g.nx;
+        len = g.ny;
+    } else {
+        if (t >= (int64_t)g.nx * g.ny) return;
+        base = t;
+        stride = (int64_t)g.nx * g.ny;
+        len = g.nz;
+    }
+    int ch = 0;
+    uint8_t prev = 0;
+    for (int64_t k = 0; k < len; ++k) {
+        const int64_t i = base + k * stride;
+        const uint8_t s = sas[i];
+        uint8_t e = ext[i];
+        if (!s && !e && prev) {
+            e = 1;
+            ext[i] = 1;
+            ch = 1;
+        }
+        prev = s ? 0 : e;
+    }
+    prev = 0;
+    for (int64_t k = len - 1; k >= 0; --k) {
+        const int64_t i = base + k * stride;
+        const uint8_t s = sas[i];
+        uint8_t e = ext[i];
+        if (!s && !e && prev) {
+            e = 1;
+            ext[i] = 1;
+            ch = 1;
+        }
+        prev = s ? 0 : e;
+    }
+    if (ch) *changed = 1;
+}
+
+// Bounded separable squared EDT (values capped at T+1) and the SES test
+// inside <=> min squared index distance to the exterior > T, T = floor((p/h)^2).
+__global__ void dil_x_kernel(GridDesc g, int R, int T, const uint8_t *__restrict__ ext,
+                             uint8_t *__restrict__ out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.size()) return;
+    int x, y, z;
+    node_of(g, idx, x, y, z);
+    int best = T + 1;
+    for (int d = -R; d <= R; ++d) {
+        const int xx = x + d;
+        if (xx < 0 || xx >= g.nx) continue;
+        if (ext[idx + d]) best = min(best, d * d);
+    }
+    out[idx] = (uint8_t)best;
+}
+
+__global__ void dil_yz_kernel(GridDesc g, int R, int T, int axis, const uint8_t *__restrict__ in,
+                              uint8_t *__restrict__ out, int final_test)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.size()) return;
+    int x, y, z;
+    node_of(g, idx, x, y, z);
+    const int pos = axis == 1 ? y : z;
+    const int len = axis == 1 ? g.ny : g.nz;
+    const int64_t st = axis == 1 ? (int64_t)g.nx : (int64_t)g.nx * g.ny;
+    int best = T + 1;
+    for (int d = -R; d <= R; ++d) {
+        const int p = pos + d;
+        if (p < 0 || p >= len) continue;
+        best = min(best, (int)in[idx + d * st] + d * d);
+    }
+    best = min(best, T + 1);
+    out[idx] = final_test ? (uint8_t)(best > T ? 1 : 0) : (uint8_t)best;
+}
+
+void launch_ses(const GridDesc &g, const uint8_t *sas, uint8_t *ext, uint8_t *work_a,
+                uint8_t *work_b, int thresh, uint8_t *inside_out, int *flag_dev, int *flag_host,
+                hipStream_t s)
+{
+    const int64_t N = g.size();
+    hipLaunchKernelGGL(ext_init_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, sas, ext);
+    LAUNCH_OK();
+    const int64_t lines[3] = {(int64_t)g.ny * g.nz, (int64_t)g.nx * g.nz, (int64_t)g.nx * g.ny};
+    for (int round = 0; round < 100000; ++round) {
+        launch_fill_i32(flag_dev, 0, 1, s);
+        for (int ax = 0; ax < 3; ++ax) {
+            hipLaunchKernelGGL(flood_line_kernel, dim3(ceil_div(lines[ax], 128)), dim3(128), 0, s,
+                               g, ax, sas, ext, flag_dev);
+            LAUNCH_OK();
+        }
+        HIP_OK(hipMemcpyAsync(flag_host, flag_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (!*flag_host) break;
+    }
+    const int R = (int)std::floor(std::sqrt((double)thresh));
+    hipLaunchKernelGGL(dil_x_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, R, thresh, ext,
+                       work_a);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(dil_yz_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, R, thresh, 1,
+                       work_a, work_b, 0);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(dil_yz_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, R, thresh, 2,
+                       work_b, inside_out, 1);
+    LAUNCH_OK();
+}
+
+// ---- face dielectrics, assign_dielectrics (pb_grid.rs:827-940) -------------
+// Mixed edges take the VDW-sphere fraction of fraction_inside_along_axis
+// (pb_grid.rs:717-807) from atoms in nearby cells (the same interval set as the
+// reference's all-atom scan), sorted and merged in FP64.
+constexpr int kMaxIv = 48;
+
+__device__ double edge_fraction(double lo, double hi, double p1, double p2, int axis,
+                                const double *__restrict__ xyz, const double *__restrict__ radii,
+                                double max_r, const CellList &cl, int *overflow)
+{
+    const double len = hi - lo;
+    if (len <= 0.0) return 0.0;
+    const int a1 = axis == 0 ? 1 : 0;
+    const int a2 = axis == 2 ? 1 : 2;
+    double mid[3];
+    mid[axis] = 0.5 * (lo + hi);
+    mid[a1] = p1;
+    mid[a2] = p2;
+    const double w = max_r + 0.5 * len + 1e-9;
+    const int cx0 = max(0, (int)floor((mid[0] - w - cl.ox) * cl.inv));
+    const int cx1 = min(cl.dx - 1, (int)floor((mid[0] + w - cl.ox) * cl.inv));
+    const int cy0 = max(0, (int)floor((mid[1] - w - cl.oy) * cl.inv));
+    const int cy1 = min(cl.dy - 1, (int)floor((mid[1] + w - cl.oy) * cl.inv));
+    const int cz0 = max(0, (int)floor((mid[2] - w - cl.oz) * cl.inv));
+    const int cz1 = min(cl.dz - 1, (int)floor((mid[2] + w - cl.oz) * cl.inv));
+    double ia[kMaxIv], ib[kMaxIv];
+    int m = 0;
+    for (int cz = cz0; cz <= cz1; ++cz)
+        for (int cy = cy0; cy <= cy1; ++cy)
+            for (int cx = cx0; cx <= cx1; ++cx) {
+                const int c = cx + cl.dx * (cy + cl.dy * cz);
+                for (int k = cl.start[c]; k < cl.start[c + 1]; ++k) {
+                    const int a = cl.atoms[k];
+                    const double R = radii[a] + 0.0;
+                    const double d1 = p1 - xyz[3 * a + a1];
+                    const double d2 = p2 - xyz[3 * a + a2];
+                    const double rs = R * R - d1 * d1 - d2 * d2;
+                    if (rs <= 0.0) continue;
+                    const double rho = sqrt(rs);
+                    const double ca = xyz[3 * a + axis];
+                    const double ilo = fmax(ca - rho, lo);
+                    const double ihi = fmin(ca + rho, hi);
+                    if (ilo < ihi) {
+                        if (m < kMaxIv) {
+                            // insertion by start (ties in any order merge identically)
+                            int p = m++;
+                            while (p > 0 && ia[p - 1] > ilo) {
+                                ia[p] = ia[p - 1];
+                                ib[p] = ib[p - 1];
+                                --p;
+                            }
+                            ia[p] = ilo;
+                            ib[p] = ihi;
+                        } else {
+                            atomicExch(overflow, 1);
+                        }
+                    }
+                }
+            }
+    if (m == 0) return 0.0;
+    double tot = 0.0, cl_ = ia[0], ch = ib[0];
+    for (int k = 1; k < m; ++k) {
+        if (ia[k] <= ch) {
+            ch = fmax(ch, ib[k]);
+        } else {
+            tot += ch - cl_;
+            cl_ = ia[k];
+            ch = ib[k];
+        }
+    }
+    tot += ch - cl_;
+    double f = tot / len;
+    f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+    return f;
+}
+
+__global__ void faces_kernel(GridDesc g, const uint8_t *__restrict__ inside,
+                             const double *__restrict__ xyz, const double *__restrict__ radii,
+                             double max_r, CellList cl, double ein, double eout,
+                             double *__restrict__ ex, double *__restrict__ ey,
+                             double *__restrict__ ez, float *__restrict__ fx,
+                             float *__restrict__ fy, float *__restrict__ fz, int *overflow)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.size()) return;
+    int x, y, z;
+    node_of(g, idx, x, y, z);
+    const uint8_t a = inside[idx];
+    const double pt[3] = {g.ox + (double)x * g.h, g.oy + (double)y * g.h, g.oz + (double)z * g.h};
+    const int pos[3] = {x, y, z};
+    const int lim[3] = {g.nx, g.ny, g.nz};
+    const int64_t step[3] = {1, (int64_t)g.nx, (int64_t)g.nx * g.ny};
+    double *dst[3] = {ex, ey, ez};
+    float *dstf[3] = {fx, fy, fz};
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        double e = eout;
+        if (pos[axis] + 1 < lim[axis]) {
+            const uint8_t b = inside[idx + step[axis]];
+            if (a == b) {
+                e = a ? ein : eout;
+            } else {
+                const double lo = pt[axis], hi = pt[axis] + g.h;
+                const double p1 = axis == 0 ? pt[1] : pt[0];
+                const double p2 = axis == 2 ? pt[1] : pt[2];
+                const double f = edge_fraction(lo, hi, p1, p2, axis, xyz, radii, max_r, cl, overflow);
+                if (f <= 0.0)
+                    e = eout;
+                else if (f >= 1.0)
+                    e = ein;
+                else
+                    e = 1.0 / (f / ein + (1.0 - f) / eout);
+            }
+        }
+        dst[axis][idx] = e;
+        dstf[axis][idx] = (float)e;
+    }
+}
+
+void launch_faces(const GridDesc &g, const uint8_t *inside, const double *xyz,
+                  const double *radii, double max_r, const CellList &cl, double eps_in,
+                  double eps_out, double *ex, double *ey, double *ez, float *fx, float *fy,
+                  float *fz, int *overflow, hipStream_t s)
+{
+    const int64_t N = g.size();
+    hipLaunchKernelGGL(faces_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, s, g, inside, xyz,
+                       radii, max_r, cl, eps_in, eps_out, ex, ey, ez, fx, fy, fz, overflow);
+    LAUNCH_OK();
+}
+
+// assign_kappa, pb_grid.rs:951-985
+__global__ void kappa_map_kernel(int64_t n, const uint8_t *__restrict__ ion_inside, double kb2,
+                                 double *__restrict__ k2, float *__restrict__ k2f)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = ion_inside[i] ? 0.0 : kb2;
+    k2[i] = v;
+    k2f[i] = (float)v;
+}
+
+void launch_kappa_map(const GridDesc &g, const uint8_t *ion_inside, double kbar2, double *k2,
+                      float *k2f, hipStream_t s)
+{
+    const int64_t N = g.size();
+    hipLaunchKernelGGL(kappa_map_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, N, ion_inside,
+                       kbar2, k2, k2f);
+    LAUNCH_OK();
+}
+
+// ---- trilinear interpolation, PbGrid::interpolate_with_data (pb_grid.rs:140-182)
+__device__ double interp3(const GridDesc &g, const double *__restrict__ d, double px, double py,
+                          double pz)
+{
+    const double fx = (px - g.ox) / g.h, fy = (py - g.oy) / g.h, fz = (pz - g.oz) / g.h;
+    const double flx = floor(fx), fly = floor(fy), flz = floor(fz);
+    if (flx < 0.0 || flx >= (double)(g.nx - 1) || fly < 0.0 || fly >= (double)(g.ny - 1) ||
+        flz < 0.0 || flz >= (double)(g.nz - 1))
+        return 0.0;
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    const double dx = fx - (double)ix, dy = fy - (double)iy, dz = fz - (double)iz;
+    const double c000 = d[gi(g, ix, iy, iz)], c100 = d[gi(g, ix + 1, iy, iz)];
+    const double c010 = d[gi(g, ix, iy + 1, iz)], c110 = d[gi(g, ix + 1, iy + 1, iz)];
+    const double c001 = d[gi(g, ix, iy, iz + 1)], c101 = d[gi(g, ix + 1, iy, iz + 1)];
+    const double c011 = d[gi(g, ix, iy + 1, iz + 1)], c111 = d[gi(g, ix + 1, iy + 1, iz + 1)];
+    const double c00 = c000 * (1.0 - dx) + c100 * dx;
+    const double c10 = c010 * (1.0 - dx) + c110 * dx;
+    const double c01 = c001 * (1.0 - dx) + c101 * dx;
+    const double c11 = c011 * (1.0 - dx) + c111 * dx;
+    const double c0 = c00 * (1.0 - dy) + c10 * dy;
+    const double c1 = c01 * (1.0 - dy) + c11 * dy;
+    return c0 * (1.0 - dz) + c1 * dz;
+}
+
+// interpolated_boundary, pb_solver.rs:100-121
+__global__ void interp_boundary_kernel(GridDesc f, GridDesc c, const double *__restrict__ cphi,
+                                       double *__restrict__ phi)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= f.size()) return;
+    int x, y, z;
+    node_of(f, idx, x, y, z);
+    const bool bnd = x == 0 || x == f.nx - 1 || y == 0 || y == f.ny - 1 || z == 0 || z == f.nz - 1;
+    if (!bnd) return;
+    phi[idx] = interp3(c, cphi, f.ox + (double)x * f.h, f.oy + (double)y * f.h,
+                       f.oz + (double)z * f.h);
+}
+
+void launch_interp_boundary(const GridDesc &fine, const GridDesc &coarse, const double *cphi,
+                            double *phi, hipStream_t s)
+{
+    const int64_t N = fine.size();
+    hipLaunchKernelGGL(interp_boundary_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, fine,
+                       coarse, cphi, phi);
+    LAUNCH_OK();
+}
+
+// compute_elec_energy, pb_solver.rs:1174-1186 (block partials of q * phi(r))
+__global__ __launch_bounds__(256) void gather_kernel(GridDesc g, const double *__restrict__ phi,
+                                                     int n, const double *__restrict__ xyz,
+                                                     const double *__restrict__ q,
+                                                     double *__restrict__ partials)
+{
+    __shared__ double red[4];
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (a < n) v = q[a] * interp3(g, phi, xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]);
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+void launch_energy_gather(const GridDesc &g, const double *phi, int n, const double *xyz,
+                          const double *q, double *partials, int *n_blocks, hipStream_t s)
+{
+    *n_blocks = std::max(1, ceil_div(n, 256));
+    hipLaunchKernelGGL(gather_kernel, dim3(*n_blocks), dim3(256), 0, s, g, phi, n, xyz, q,
+                       partials);
+    LAUNCH_OK();
+}
+
+// max |v| ignoring NaN (as f64::max does), pb_energy.rs:188-194
+__global__ __launch_bounds__(256) void maxabs_kernel(const double *__restrict__ v, int64_t n,
+                                                     double *__restrict__ partials)
+{
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        m = fmax(m, fabs(v[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void max_partials_kernel(const double *__restrict__ p, int nb, double *out)
+{
+    if (threadIdx.x != 0) return;
+    double m = 0.0;
+    for (int k = 0; k < nb; ++k) m = fmax(m, p[k]);
+    *out = m;
+}
+
+void launch_maxabs(const double *v, int64_t n, double *partials, int *n_blocks, hipStream_t s)
+{
+    *n_blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(n, 256)));
+    hipLaunchKernelGGL(maxabs_kernel, dim3(*n_blocks), dim3(256), 0, s, v, n, partials);
+    LAUNCH_OK();
+}
+
+void launch_max_partials(const double *partials, int n_blocks, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(max_partials_kernel, dim3(1), dim3(64), 0, s, partials, n_blocks, out);
+    LAUNCH_OK();
+}
+
+// ---- small helpers ----------------------------------------------------------
+template <typename T>
+__global__ void fill_kernel(T *p, T v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_kernel<uint8_t>, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, v, n);
+    LAUNCH_OK();
+}
+void launch_fill_f64(double *p, double v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, v, n);
+    LAUNCH_OK();
+}
+void launch_fill_i32(int *p, int v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_kernel<int>, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, v, n);
+    LAUNCH_OK();
+}
+void launch_fill_u64(unsigned long long *p, unsigned long long v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(ceil_div(n, 256)), dim3(256), 0, s, p,
+                       v, n);
+    LAUNCH_OK();
+}
+void launch_fill_f32(float *p, float v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_kernel<float>, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, v, n);
+    LAUNCH_OK();
+}
+
+__global__ void gather_xyz_kernel(const double *__restrict__ src, const int *__restrict__ idx,
+                                  int n, double *__restrict__ dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int a = idx[k];
+    dst[3 * k] = src[3 * a];
+    dst[3 * k + 1] = src[3 * a + 1];
+    dst[3 * k + 2] = src[3 * a + 2];
+}
+
+void launch_gather_xyz(const double *src, const int *idx, int n, double *dst, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(gather_xyz_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, src, idx, n, dst);
+    LAUNCH_OK();
+}
+
+// Per-segment sums in segment order (one lane per segment; segments are
+// residues of at most a few hundred atoms).
+__global__ void segment_sum_kernel(const double *__restrict__ v, const int *__restrict__ seg_atoms,
+                                   const int *__restrict__ seg_ptr, int n_seg,
+                                   double *__restrict__ out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_seg) return;
+    double acc = 0.0;
+    for (int k = seg_ptr[s]; k < seg_ptr[s + 1]; ++k) acc += v[seg_atoms[k]];
+    out[s] = acc;
+}
+
+void launch_segment_sum(const double *v, const int *seg_atoms, const int *seg_ptr, int n_seg,
+                        double *out, hipStream_t s)
+{
+    if (n_seg <= 0) return;
+    hipLaunchKernelGGL(segment_sum_kernel, dim3(ceil_div(n_seg, 256)), dim3(256), 0, s, v,
+                       seg_atoms, seg_ptr, n_seg, out);
+    LAUNCH_OK();
+}
+
+}  // namespace mmpbsa

@@ -1,0 +1,122 @@
+// kernels.h -- host-side launchers of the MM-PBSA HIP kernels (gfx950).
+// Every launcher enqueues on `s` and never synchronises.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.h"
+
+namespace mmpbsa {
+
+// ---- atom-pair kernels (pairs.hip) ------------------------------------------
+struct PairTopo {          // device views of one subsystem
+    int n;
+    int n_types;
+    const double *q_amber;   // [n]
+    const double *radii;     // [n]
+    const double *screen;    // [n]
+    const int *type;         // [n]
+    const double *ljA;       // [n_types^2] (0 where nb_parm_index < 1)
+    const double *ljB;
+    const int *ex_ptr;       // [n+1] CSR of excluded partners j > i
+    const int *ex_col;
+    const int *p14_ptr;      // [n+1] CSR of 1-4 partners j > i
+    const int *p14_col;
+    const uint8_t *tile_flag;  // [nb*nb] tile (ib, jb) holds an excluded / 1-4 pair
+    int nb;                    // tiles per side (256 atoms per tile)
+    const int *bonds; int n_bonds; const double *bond_k; const double *bond_r0;
+    const int *angles; int n_angles; const double *angle_k; const double *angle_t0;
+    const int *dihs; int n_dihs; const double *dih_pk; const double *dih_pn; const double *dih_ph;
+    double scee, scnb;
+};
+
+constexpr int kTile = 256;
+
+// Born radii: psi partials then the OBC/HCT map.  work: [chunks * n] doubles.
+int gb_chunks(int n);
+void launch_gb_radii(const double *xyz, const PairTopo &t, int model, double offset,
+                     double rgbmax, double cutoff, double *psi_work, double *born_out,
+                     hipStream_t s);
+// GB energy: writes partials; final sum by launch_sum_partials.
+int gb_energy_blocks(int n);
+void launch_gb_energy(const double *xyz, const PairTopo &t, const double *born, double kappa,
+                      double eps_in, double eps_out, double cutoff, double *partials,
+                      hipStream_t s);
+// MM: 8 doubles (4 dd terms) per tile block, plus bonded partials.
+int mm_blocks(int n);
+void launch_mm_pairs(const double *xyz, const PairTopo &t, double *partials, hipStream_t s);
+int bonded_blocks(const PairTopo &t);
+void launch_bonded(const double *xyz, const PairTopo &t, double *partials, hipStream_t s);
+// Deterministic reductions of block partials.
+void launch_sum_partials(const double *partials, int n_blocks, int stride, int n_terms,
+                         double *out, hipStream_t s);
+void launch_sum_partials_dd(const double *partials, int n_blocks, int n_terms, double *out,
+                            hipStream_t s);
+
+// Debye-Hueckel boundary on the boundary nodes of grid g (FP32 terms, FP64 sums).
+struct GridDesc {
+    int nx, ny, nz;
+    double ox, oy, oz;
+    double h;
+    __host__ __device__ int64_t size() const { return (int64_t)nx * ny * nz; }
+};
+int64_t boundary_count(const GridDesc &g);
+void launch_dh_boundary(const GridDesc &g, int n, const double *xyz, const double *q,
+                        double kappa, double eps_out, double *phi, hipStream_t s);
+
+// Decomposition atom sums: for each atom a in A, sum over b in B of the
+// half pair vdW / elec (dd) into work, then reduce into out_vdw / out_elec.
+int decomp_chunks(int nB);
+void launch_decomp_pairs(const double *xyz, const PairTopo &t, const int *A, int nA,
+                         const int *B, int nB, int a_is_rec, int cross_excl, double *work,
+                         double *out_vdw, double *out_elec, hipStream_t s);
+
+// SASA per atom (Shrake-Rupley) using a cell list.
+struct CellList {
+    double ox, oy, oz, inv;
+    int dx, dy, dz;
+    const int *start;   // [ncell + 1]
+    const int *atoms;   // [n]
+};
+void launch_sasa(const double *xyz, const double *radii, int n, double probe,
+                 const double *pts, int n_pts, const CellList &cl, double max_ext,
+                 double *per_atom, int *overflow, hipStream_t s);
+
+// ---- grid kernels (grid.hip) ------------------------------------------------
+void launch_cell_build(const double *xyz, int n, double ox, double oy, double oz, double inv,
+                       int dx, int dy, int dz, int *cell_of, int *start, int *fill, int *atoms,
+                       hipStream_t s);
+void launch_spread(const GridDesc &g, int n, const double *xyz, const double *q,
+                   unsigned long long *acc, double *rho, hipStream_t s);
+// inside[i] = exists atom a with |p_i - x_a|^2 < (r_a + probe)^2
+void launch_inside(const GridDesc &g, const double *xyz, const double *radii, double probe,
+                   double max_r, const CellList &cl, uint8_t *inside, hipStream_t s);
+// SES classification from the SAS map (flood fill + bounded distance transform).
+void launch_ses(const GridDesc &g, const uint8_t *sas, uint8_t *ext, uint8_t *work_a,
+                uint8_t *work_b, int thresh, uint8_t *inside_out, int *flag_dev,
+                int *flag_host, hipStream_t s);
+// Face dielectrics (FP64 + FP32 copies) from the SES map.
+void launch_faces(const GridDesc &g, const uint8_t *inside, const double *xyz,
+                  const double *radii, double max_r, const CellList &cl, double eps_in,
+                  double eps_out, double *ex, double *ey, double *ez, float *fx, float *fy,
+                  float *fz, int *overflow, hipStream_t s);
+void launch_kappa_map(const GridDesc &g, const uint8_t *ion_inside, double kbar2, double *k2,
+                      float *k2f, hipStream_t s);
+void launch_interp_boundary(const GridDesc &fine, const GridDesc &coarse, const double *cphi,
+                            double *phi, hipStream_t s);
+void launch_energy_gather(const GridDesc &g, const double *phi, int n, const double *xyz,
+                          const double *q, double *partials, int *n_blocks, hipStream_t s);
+void launch_maxabs(const double *v, int64_t n, double *partials, int *n_blocks, hipStream_t s);
+void launch_max_partials(const double *partials, int n_blocks, double *out, hipStream_t s);
+void launch_fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);
+void launch_fill_f64(double *p, double v, int64_t n, hipStream_t s);
+void launch_fill_f32(float *p, float v, int64_t n, hipStream_t s);
+void launch_fill_i32(int *p, int v, int64_t n, hipStream_t s);
+void launch_fill_u64(unsigned long long *p, unsigned long long v, int64_t n, hipStream_t s);
+void launch_gather_xyz(const double *src, const int *idx, int n, double *dst, hipStream_t s);
+void launch_segment_sum(const double *v, const int *seg_atoms, const int *seg_ptr, int n_seg,
+                        double *out, hipStream_t s);
+
+}  // namespace mmpbsa

@@ -1,0 +1,400 @@
+"""MI355X (gfx950) MM-PBSA solvation energies behind the reference's Python API.
+
+Names, keyword arguments, defaults, units and result getters follow the PyO3
+module of rust-simulation-tools (crates/bindings/src/lib.rs:1450-2280):
+coordinates are float64 (N, 3) arrays in nm and are scaled to Angstrom
+(lib.rs:83-88); string errors raise ValueError (lib.rs:2136, :2158).
+Every energy is computed by libmmpbsa_hip.so on the GPU; importing works
+without a GPU, calling a function without one raises RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi
+from ._abi import (FrameTermsC, GbParamsC, MmTermsC, PbParamsC, PbResultC, ResidueContribC,
+                   SaParamsC, load_library)
+from .topology import AmberTopology, extract_coords
+from .topology import extract_subtopology as _extract_subtopology
+
+__all__ = [
+    "AmberTopology", "GbModel", "GbParams", "PbParams", "SaParams", "MmEnergy", "GbEnergy",
+    "SaEnergy", "PbEnergy", "FrameEnergy", "BindingResult", "ResidueContribution",
+    "DecompositionResult", "compute_mm_energy", "compute_gb_energy", "compute_sa_energy",
+    "compute_pb_energy", "compute_binding_energy_single_frame", "compute_binding_energy_frames",
+    "decompose_binding_energy", "extract_subtopology",
+]
+
+
+class GbModel:
+    """lib.rs:1454-1470 (AMBER igb 1 / 2 / 5)."""
+    Hct = 0
+    ObcI = 1
+    ObcII = 2
+
+
+def _opt(p, **kw):
+    for k, v in kw.items():
+        if v is not None:
+            setattr(p, k, v)
+    return p
+
+
+class GbParams:
+    """lib.rs:1472-1540; defaults gb_energy.rs:43-56."""
+
+    def __init__(self, model=None, solute_dielectric=None, solvent_dielectric=None,
+                 salt_concentration=None, temperature=None, offset=None, rgbmax=None,
+                 cutoff=None):
+        self._c = GbParamsC()
+        load_library().mmpbsa_gb_params_default(C.byref(self._c))
+        _opt(self._c, model=None if model is None else int(model),
+             solute_dielectric=solute_dielectric, solvent_dielectric=solvent_dielectric,
+             salt_concentration=salt_concentration, temperature=temperature, offset=offset,
+             rgbmax=rgbmax, cutoff=cutoff)
+
+    solute_dielectric = property(lambda s: s._c.solute_dielectric)
+    solvent_dielectric = property(lambda s: s._c.solvent_dielectric)
+    salt_concentration = property(lambda s: s._c.salt_concentration)
+    temperature = property(lambda s: s._c.temperature)
+    cutoff = property(lambda s: s._c.cutoff)
+
+
+class PbParams:
+    """lib.rs:1542-1626; defaults pb_energy.rs:49-66.  ``fine_dims`` (no
+    counterpart in the reference) fixes the fine grid dimensions, centred as
+    auto_grid centres."""
+
+    def __init__(self, grid_spacing=None, grid_buffer=None, solute_dielectric=None,
+                 solvent_dielectric=None, salt_concentration=None, temperature=None,
+                 probe_radius=None, ion_radius=None, tolerance=None, max_iterations=None,
+                 fillratio=None, fscale=None, fine_dims=None):
+        self._c = PbParamsC()
+        load_library().mmpbsa_pb_params_default(C.byref(self._c))
+        _opt(self._c, grid_spacing=grid_spacing, grid_buffer=grid_buffer,
+             solute_dielectric=solute_dielectric, solvent_dielectric=solvent_dielectric,
+             salt_concentration=salt_concentration, temperature=temperature,
+             probe_radius=probe_radius, ion_radius=ion_radius, tolerance=tolerance,
+             max_iterations=None if max_iterations is None else int(max_iterations),
+             fillratio=fillratio, fscale=None if fscale is None else int(fscale))
+        if fine_dims is not None:
+            for i in range(3):
+                self._c.fine_dims[i] = int(fine_dims[i])
+
+    grid_spacing = property(lambda s: s._c.grid_spacing)
+    solute_dielectric = property(lambda s: s._c.solute_dielectric)
+    solvent_dielectric = property(lambda s: s._c.solvent_dielectric)
+    salt_concentration = property(lambda s: s._c.salt_concentration)
+    temperature = property(lambda s: s._c.temperature)
+
+
+class SaParams:
+    """lib.rs:1628-1676; defaults sa_energy.rs:23-32."""
+
+    def __init__(self, surface_tension=None, offset=None, probe_radius=None,
+                 n_sphere_points=None):
+        self._c = SaParamsC()
+        load_library().mmpbsa_sa_params_default(C.byref(self._c))
+        _opt(self._c, surface_tension=surface_tension, offset=offset, probe_radius=probe_radius,
+             n_sphere_points=None if n_sphere_points is None else int(n_sphere_points))
+
+    surface_tension = property(lambda s: s._c.surface_tension)
+    offset = property(lambda s: s._c.offset)
+    probe_radius = property(lambda s: s._c.probe_radius)
+    n_sphere_points = property(lambda s: s._c.n_sphere_points)
+
+
+# ---- results ----------------------------------------------------------------
+class MmEnergy:
+    def __init__(self, t: MmTermsC):
+        for k, _ in MmTermsC._fields_:
+            setattr(self, k, getattr(t, k))
+
+    @property
+    def total(self):
+        return (self.bond + self.angle + self.dihedral + self.vdw + self.elec + self.vdw_14
+                + self.elec_14)
+
+
+class GbEnergy:
+    def __init__(self, total, radii):
+        self.total = total
+        self._r = radii
+
+    def born_radii(self):
+        return self._r.copy()
+
+
+class SaEnergy:
+    def __init__(self, total, total_sasa, per_atom):
+        self.total = total
+        self.total_sasa = total_sasa
+        self._p = per_atom
+
+    def per_atom_sasa(self):
+        return self._p.copy()
+
+
+class PbEnergy:
+    def __init__(self, r: PbResultC):
+        self.total = r.total
+        self.e_solv = r.solv.energy
+        self.e_ref = r.ref.energy
+        self.converged = bool(r.solv.converged and r.ref.converged)
+        self.iterations = (r.solv.iterations, r.ref.iterations)
+        self.final_residual = (r.solv.final_residual, r.ref.final_residual)
+        self.fine_dims = tuple(r.fine_dims)
+        self.coarse_dims = tuple(r.coarse_dims)
+
+
+class FrameEnergy:
+    """FrameEnergy getters (lib.rs:1760-1850), *_gb aliases of *_polar."""
+
+    def __init__(self, t: FrameTermsC):
+        for part, src in (("complex", t.complex_), ("receptor", t.receptor), ("ligand", t.ligand)):
+            setattr(self, part + "_mm", src.mm)
+            setattr(self, part + "_polar", src.polar)
+            setattr(self, part + "_gb", src.polar)
+            setattr(self, part + "_sa", src.sa)
+            setattr(self, part + "_total", src.mm + src.polar + src.sa)
+        self.delta_mm = t.delta_mm
+        self.delta_polar = t.delta_polar
+        self.delta_gb = t.delta_polar
+        self.delta_sa = t.delta_sa
+        self.delta_total = t.delta_total
+
+
+class BindingResult:
+    """Statistics of binding.rs:704-743 over in-memory frames."""
+
+    def __init__(self, frames):
+        self._frames = frames
+        n = len(frames)
+
+        def mean(key):
+            return sum(getattr(f, key) for f in frames) / n
+
+        def std(key, m):
+            if n <= 1:
+                return 0.0
+            return math.sqrt(sum((getattr(f, key) - m) ** 2 for f in frames) / (n - 1))
+
+        self.mean_delta_mm = mean("delta_mm")
+        self.mean_delta_polar = mean("delta_polar")
+        self.mean_delta_gb = self.mean_delta_polar
+        self.mean_delta_sa = mean("delta_sa")
+        self.mean_delta_total = mean("delta_total")
+        self.std_delta_total = std("delta_total", self.mean_delta_total)
+        self.std_delta_mm = std("delta_mm", self.mean_delta_mm)
+        self.std_delta_polar = std("delta_polar", self.mean_delta_polar)
+        self.std_delta_gb = self.std_delta_polar
+        self.std_delta_sa = std("delta_sa", self.mean_delta_sa)
+        self.sem_delta_total = self.std_delta_total / math.sqrt(n)
+
+    def frames(self):
+        return list(self._frames)
+
+
+class ResidueContribution:
+    def __init__(self, c: ResidueContribC, label: str):
+        self.residue_index = int(c.residue_index)
+        self.residue_label = label
+        self.vdw = c.vdw
+        self.elec = c.elec
+        self.polar = c.polar
+        self.gb = c.polar
+        self.sa = c.sa
+
+    @property
+    def total(self):
+        return self.vdw + self.elec + self.polar + self.sa
+
+
+class DecompositionResult:
+    def __init__(self, rec, lig):
+        self._r = rec
+        self._l = lig
+
+    def receptor_residues(self):
+        return list(self._r)
+
+    def ligand_residues(self):
+        return list(self._l)
+
+
+# ---- device contexts (cached per topology) ---------------------------------
+def _check(rc, lib, ctx=None):
+    if rc == _abi.MMPBSA_OK:
+        return
+    msg = lib.mmpbsa_last_error(ctx)
+    msg = msg.decode() if msg else _abi.ERRORS.get(rc, "error")
+    if rc in (1, 2):
+        raise ValueError(msg)
+    raise RuntimeError(f"mmpbsa-hip error {rc}: {msg}")
+
+
+class _Ctx:
+    def __init__(self, top: AmberTopology):
+        self.lib = load_library()
+        self.view = top.view()
+        self.top = top
+        self.h = C.c_void_p()
+        _check(self.lib.mmpbsa_ctx_create(C.byref(self.view), 0, C.byref(self.h)), self.lib)
+        self.splits = {}
+
+    def __del__(self):
+        try:
+            for s in self.splits.values():
+                self.lib.mmpbsa_split_destroy(s)
+            if self.h:
+                self.lib.mmpbsa_ctx_destroy(self.h)
+        except Exception:
+            pass
+
+    def split(self, rec, lig):
+        key = (tuple(int(r) for r in rec), tuple(int(r) for r in lig))
+        s = self.splits.get(key)
+        if s is None:
+            ra = np.ascontiguousarray(key[0], dtype=np.int32)
+            la = np.ascontiguousarray(key[1], dtype=np.int32)
+            s = C.c_void_p()
+            _check(self.lib.mmpbsa_split_create(self.h, ra.ctypes.data_as(_abi.c_int32_p), len(ra),
+                                                la.ctypes.data_as(_abi.c_int32_p), len(la),
+                                                C.byref(s)), self.lib, self.h)
+            self.splits[key] = s
+        return s
+
+
+def _ctx(top: AmberTopology) -> _Ctx:
+    c = getattr(top, "_mmpbsa_ctx", None)
+    if c is None:
+        c = _Ctx(top)
+        object.__setattr__(top, "_mmpbsa_ctx", c)
+    return c
+
+
+def _angstrom(coords, n):
+    x = np.asarray(coords, dtype=np.float64)
+    if x.ndim != 2 or x.shape[1] != 3:
+        raise ValueError("coords must be an (N, 3) array")
+    if x.shape[0] != n:
+        raise ValueError(f"coords has {x.shape[0]} atoms, topology has {n}")
+    return np.ascontiguousarray(x * 10.0)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_abi.c_double_p)
+
+
+# ---- functions (lib.rs:2065-2278) -------------------------------------------
+def compute_mm_energy(topology: AmberTopology, coords) -> MmEnergy:
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    out = MmTermsC()
+    _check(c.lib.mmpbsa_mm_energy(c.h, _dp(x), C.byref(out)), c.lib, c.h)
+    return MmEnergy(out)
+
+
+def compute_gb_energy(topology: AmberTopology, coords, gb_params: GbParams | None = None) -> GbEnergy:
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    p = (gb_params or GbParams())._c
+    radii = np.zeros(topology.n_atoms)
+    tot = C.c_double()
+    _check(c.lib.mmpbsa_gb_energy(c.h, _dp(x), C.byref(p), _dp(radii), C.byref(tot)), c.lib, c.h)
+    return GbEnergy(tot.value, radii)
+
+
+def compute_sa_energy(topology: AmberTopology, coords, sa_params: SaParams | None = None) -> SaEnergy:
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    p = (sa_params or SaParams())._c
+    per = np.zeros(topology.n_atoms)
+    ts, t = C.c_double(), C.c_double()
+    _check(c.lib.mmpbsa_sa_energy(c.h, _dp(x), C.byref(p), _dp(per), C.byref(ts), C.byref(t)),
+           c.lib, c.h)
+    return SaEnergy(t.value, ts.value, per)
+
+
+def compute_pb_energy(topology: AmberTopology, coords, pb_params: PbParams | None = None,
+                      solute_residues=None) -> PbEnergy:
+    """compute_pb_energy_solvated (pb_energy.rs:469-482)."""
+    p = (pb_params or PbParams())._c
+    if solute_residues is not None:
+        sel = topology.build_selection(solute_residues)
+        sub = _extract_subtopology(topology, sel)
+        x_nm = np.asarray(coords, dtype=np.float64)[sel]
+        return compute_pb_energy(sub, x_nm, pb_params, None)
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    out = PbResultC()
+    _check(c.lib.mmpbsa_pb_energy(c.h, _dp(x), C.byref(p), C.byref(out)), c.lib, c.h)
+    return PbEnergy(out)
+
+
+def _method(gb_params, pb_params):
+    # build_solvation_method (lib.rs:2105-2115): PB wins when both are given
+    if pb_params is not None:
+        return 1, GbParams()._c, pb_params._c
+    return 0, (gb_params or GbParams())._c, PbParams()._c
+
+
+def compute_binding_energy_single_frame(topology: AmberTopology, coords, receptor_residues,
+                                        ligand_residues, gb_params=None, pb_params=None,
+                                        sa_params=None) -> FrameEnergy:
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    s = c.split(receptor_residues, ligand_residues)
+    m, g, p = _method(gb_params, pb_params)
+    a = (sa_params or SaParams())._c
+    out = FrameTermsC()
+    _check(c.lib.mmpbsa_frame_energy(s, _dp(x), m, C.byref(g), C.byref(p), C.byref(a),
+                                     C.byref(out)), c.lib, c.h)
+    return FrameEnergy(out)
+
+
+def compute_binding_energy_frames(topology: AmberTopology, frames, receptor_residues,
+                                  ligand_residues, gb_params=None, pb_params=None,
+                                  sa_params=None, batched=True) -> BindingResult:
+    """compute_binding_energy (binding.rs:461-744) over in-memory frames
+    ([F, N, 3] nm) instead of a trajectory file; batched=True overlaps frames
+    on concurrent device streams."""
+    c = _ctx(topology)
+    f = np.asarray(frames, dtype=np.float64)
+    if f.ndim != 3 or f.shape[1] != topology.n_atoms or f.shape[2] != 3:
+        raise ValueError("frames must be an (F, N, 3) array")
+    if f.shape[0] == 0:
+        raise ValueError("No frames found in trajectory")
+    x = np.ascontiguousarray(f * 10.0)
+    s = c.split(receptor_residues, ligand_residues)
+    m, g, p = _method(gb_params, pb_params)
+    a = (sa_params or SaParams())._c
+    out = (FrameTermsC * f.shape[0])()
+    _check(c.lib.mmpbsa_frames_energy(s, _dp(x), f.shape[0], m, C.byref(g), C.byref(p),
+                                      C.byref(a), 1 if batched else 0, out), c.lib, c.h)
+    return BindingResult([FrameEnergy(t) for t in out])
+
+
+def decompose_binding_energy(topology: AmberTopology, coords, receptor_residues,
+                             ligand_residues, gb_params=None, sa_params=None) -> DecompositionResult:
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    s = c.split(receptor_residues, ligand_residues)
+    g = (gb_params or GbParams())._c
+    a = (sa_params or SaParams())._c
+    rec = (ResidueContribC * max(len(receptor_residues), 1))()
+    lig = (ResidueContribC * max(len(ligand_residues), 1))()
+    _check(c.lib.mmpbsa_decompose(s, _dp(x), C.byref(g), C.byref(a), rec, lig), c.lib, c.h)
+    lab = topology.residue_labels
+    return DecompositionResult(
+        [ResidueContribution(rec[k], lab[int(rec[k].residue_index)])
+         for k in range(len(receptor_residues))],
+        [ResidueContribution(lig[k], lab[int(lig[k].residue_index)])
+         for k in range(len(ligand_residues))])
+
+
+def extract_subtopology(topology: AmberTopology, atom_indices) -> AmberTopology:
+    return _extract_subtopology(topology, atom_indices)

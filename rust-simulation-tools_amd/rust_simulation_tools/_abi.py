@@ -1,0 +1,215 @@
+"""ctypes mirror of include/mmpbsa_hip.h and the loader of the HIP library.
+
+The product library ``libmmpbsa_hip.so`` is built in-tree next to its sources
+(``rust-simulation-tools_amd/csrc``).  Loading fails loudly when it is absent:
+there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libmmpbsa_hip.so")
+
+c_double_p = C.POINTER(C.c_double)
+c_int32_p = C.POINTER(C.c_int32)
+c_int64_p = C.POINTER(C.c_int64)
+
+MMPBSA_OK = 0
+ERRORS = {
+    1: "invalid argument",
+    2: "residue index out of range",
+    3: "device error",
+    4: "out of device memory",
+    5: "no gfx950 device",
+    6: "not ready",
+    7: "request too large",
+}
+
+
+class TopologyView(C.Structure):
+    _fields_ = [
+        ("n_atoms", C.c_int32),
+        ("n_residues", C.c_int32),
+        ("n_types", C.c_int32),
+        ("charges", c_double_p),
+        ("charges_amber", c_double_p),
+        ("radii", c_double_p),
+        ("screen", c_double_p),
+        ("atom_type", c_int32_p),
+        ("nb_parm_index", c_int64_p),
+        ("lj_acoef", c_double_p),
+        ("lj_bcoef", c_double_p),
+        ("n_lj", C.c_int32),
+        ("residue_pointers", c_int32_p),
+        ("n_bonds", C.c_int32),
+        ("bonds", c_int32_p),
+        ("bond_k", c_double_p),
+        ("bond_r0", c_double_p),
+        ("n_bond_types", C.c_int32),
+        ("n_angles", C.c_int32),
+        ("angles", c_int32_p),
+        ("angle_k", c_double_p),
+        ("angle_theta0", c_double_p),
+        ("n_angle_types", C.c_int32),
+        ("n_dihedrals", C.c_int32),
+        ("dihedrals", c_int32_p),
+        ("dihedral_pk", c_double_p),
+        ("dihedral_pn", c_double_p),
+        ("dihedral_phase", c_double_p),
+        ("n_dihedral_types", C.c_int32),
+        ("num_excluded", c_int32_p),
+        ("excluded_atoms", c_int32_p),
+        ("n_excluded", C.c_int64),
+        ("scee", C.c_double),
+        ("scnb", C.c_double),
+    ]
+
+
+class GbParamsC(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("solute_dielectric", C.c_double),
+        ("solvent_dielectric", C.c_double),
+        ("salt_concentration", C.c_double),
+        ("temperature", C.c_double),
+        ("offset", C.c_double),
+        ("rgbmax", C.c_double),
+        ("cutoff", C.c_double),
+    ]
+
+
+class PbParamsC(C.Structure):
+    _fields_ = [
+        ("grid_spacing", C.c_double),
+        ("grid_buffer", C.c_double),
+        ("solute_dielectric", C.c_double),
+        ("solvent_dielectric", C.c_double),
+        ("salt_concentration", C.c_double),
+        ("temperature", C.c_double),
+        ("probe_radius", C.c_double),
+        ("ion_radius", C.c_double),
+        ("tolerance", C.c_double),
+        ("max_iterations", C.c_int64),
+        ("fillratio", C.c_double),
+        ("fscale", C.c_int64),
+        ("fine_dims", C.c_int32 * 3),
+    ]
+
+
+class SaParamsC(C.Structure):
+    _fields_ = [
+        ("surface_tension", C.c_double),
+        ("offset", C.c_double),
+        ("probe_radius", C.c_double),
+        ("n_sphere_points", C.c_int32),
+    ]
+
+
+class MmTermsC(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("bond", "angle", "dihedral", "vdw", "elec", "vdw_14", "elec_14")]
+
+
+class PbSolveInfoC(C.Structure):
+    _fields_ = [
+        ("energy", C.c_double),
+        ("final_residual", C.c_double),
+        ("iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("focused", C.c_int32),
+        ("coarse_iterations", C.c_int32),
+    ]
+
+
+class PbResultC(C.Structure):
+    _fields_ = [
+        ("total", C.c_double),
+        ("solv", PbSolveInfoC),
+        ("ref", PbSolveInfoC),
+        ("fine_dims", C.c_int32 * 3),
+        ("coarse_dims", C.c_int32 * 3),
+    ]
+
+
+class SubsystemEnergyC(C.Structure):
+    _fields_ = [("mm", C.c_double), ("polar", C.c_double), ("sa", C.c_double)]
+
+
+class FrameTermsC(C.Structure):
+    _fields_ = [
+        ("complex_", SubsystemEnergyC),
+        ("receptor", SubsystemEnergyC),
+        ("ligand", SubsystemEnergyC),
+        ("delta_mm", C.c_double),
+        ("delta_polar", C.c_double),
+        ("delta_sa", C.c_double),
+        ("delta_total", C.c_double),
+    ]
+
+
+class ResidueContribC(C.Structure):
+    _fields_ = [
+        ("residue_index", C.c_int32),
+        ("vdw", C.c_double),
+        ("elec", C.c_double),
+        ("polar", C.c_double),
+        ("sa", C.c_double),
+    ]
+
+
+# Every symbol include/mmpbsa_hip.h declares, with its ctypes signature.
+P = C.c_void_p
+SIGNATURES = {
+    "mmpbsa_init": (C.c_int, [c_int32_p]),
+    "mmpbsa_version": (C.c_char_p, []),
+    "mmpbsa_gb_params_default": (None, [C.POINTER(GbParamsC)]),
+    "mmpbsa_pb_params_default": (None, [C.POINTER(PbParamsC)]),
+    "mmpbsa_sa_params_default": (None, [C.POINTER(SaParamsC)]),
+    "mmpbsa_ctx_create": (C.c_int, [C.POINTER(TopologyView), C.c_int32, C.POINTER(P)]),
+    "mmpbsa_ctx_destroy": (C.c_int, [P]),
+    "mmpbsa_last_error": (C.c_char_p, [P]),
+    "mmpbsa_ctx_n_atoms": (C.c_int32, [P]),
+    "mmpbsa_mm_energy": (C.c_int, [P, c_double_p, C.POINTER(MmTermsC)]),
+    "mmpbsa_gb_energy": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), c_double_p, c_double_p]),
+    "mmpbsa_sa_energy": (C.c_int, [P, c_double_p, C.POINTER(SaParamsC), c_double_p, c_double_p,
+                                   c_double_p]),
+    "mmpbsa_pb_energy": (C.c_int, [P, c_double_p, C.POINTER(PbParamsC), C.POINTER(PbResultC)]),
+    "mmpbsa_split_create": (C.c_int, [P, c_int32_p, C.c_int32, c_int32_p, C.c_int32, C.POINTER(P)]),
+    "mmpbsa_split_destroy": (C.c_int, [P]),
+    "mmpbsa_split_atoms": (C.c_int, [P, C.c_int32, c_int32_p, C.c_int32, c_int32_p]),
+    "mmpbsa_frame_energy": (C.c_int, [P, c_double_p, C.c_int32, C.POINTER(GbParamsC),
+                                      C.POINTER(PbParamsC), C.POINTER(SaParamsC),
+                                      C.POINTER(FrameTermsC)]),
+    "mmpbsa_frames_energy": (C.c_int, [P, c_double_p, C.c_int32, C.c_int32, C.POINTER(GbParamsC),
+                                       C.POINTER(PbParamsC), C.POINTER(SaParamsC), C.c_int32,
+                                       C.POINTER(FrameTermsC)]),
+    "mmpbsa_decompose": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), C.POINTER(SaParamsC),
+                                   C.POINTER(ResidueContribC), C.POINTER(ResidueContribC)]),
+}
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load libmmpbsa_hip.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(
+            f"HIP library not built: {p} is missing (run __graft_entry__.build())")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def as_ptr(arr, ctype):
+    return arr.ctypes.data_as(C.POINTER(ctype))
