@@ -135,29 +135,67 @@ double mm_total(const mmpbsa_mm_terms &m)
 }
 
 // compute_frame_energy (binding.rs:333-415)
+// compute_frame_energy (binding.rs:333-415) for nf frames.  With batched, the
+// PB solves of all frames and subsystems advance together (shared launches,
+// graph-replayed cycles); otherwise each solve runs on its own.
+void frames_group(Worker &w, mmpbsa_split *s, const double *xyz, int nf, size_t stride,
+                  int method, const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb,
+                  const mmpbsa_sa_params &sa, mmpbsa_frame_terms *out, bool batched)
+{
+    std::vector<FrameCoords> fc(nf);
+    for (int f = 0; f < nf; ++f) stage_frame(w, s, xyz + stride * f, fc[f]);
+    const DevTopo *tops[3] = {&s->ctop, &s->rtop, &s->ltop};
+    std::vector<mmpbsa_pb_result> pr(3 * (size_t)nf);
+    if (method == MMPBSA_METHOD_PB) {
+        std::vector<PbJob> jobs(3 * (size_t)nf);
+        for (int f = 0; f < nf; ++f) {
+            const double *ds[3] = {fc[f].dc, fc[f].r.get(), fc[f].l.get()};
+            const double *hs[3] = {fc[f].hcp, fc[f].hr.data(), fc[f].hl.data()};
+            for (int k = 0; k < 3; ++k) {
+                PbJob &j = jobs[3 * (size_t)f + k];
+                j.t = tops[k];
+                j.d_xyz = ds[k];
+                j.h_xyz = hs[k];
+                j.p = pb;
+                j.out = &pr[3 * (size_t)f + k];
+            }
+        }
+        if (batched) {
+            run_pb_jobs(w, jobs, true);
+        } else {
+            for (auto &j : jobs) {
+                std::vector<PbJob> one(1, j);
+                run_pb_jobs(w, one, false);
+            }
+        }
+    }
+    for (int f = 0; f < nf; ++f) {
+        const double *ds[3] = {fc[f].dc, fc[f].r.get(), fc[f].l.get()};
+        const double *hs[3] = {fc[f].hcp, fc[f].hr.data(), fc[f].hl.data()};
+        mmpbsa_frame_terms *o = out + f;
+        mmpbsa_subsystem_energy *e[3] = {&o->complex_, &o->receptor, &o->ligand};
+        for (int k = 0; k < 3; ++k) {
+            mmpbsa_mm_terms mm;
+            run_mm(w, *tops[k], ds[k], &mm);
+            e[k]->mm = mm_total(mm);
+            e[k]->polar = method == MMPBSA_METHOD_PB ? pr[3 * (size_t)f + k].total
+                                                     : run_gb(w, *tops[k], ds[k], gb, nullptr, nullptr);
+            double tsasa = 0.0, tsa = 0.0;
+            run_sa(w, *tops[k], ds[k], hs[k], sa, nullptr, nullptr, &tsasa, &tsa);
+            e[k]->sa = tsa;
+        }
+        o->delta_mm = o->complex_.mm - o->receptor.mm - o->ligand.mm;
+        o->delta_polar = o->complex_.polar - o->receptor.polar - o->ligand.polar;
+        o->delta_sa = o->complex_.sa - o->receptor.sa - o->ligand.sa;
+        o->delta_total = o->delta_mm + o->delta_polar + o->delta_sa;
+    }
+}
+
 void frame_energy(Worker &w, mmpbsa_split *s, const double *xyz, int method,
                   const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb,
                   const mmpbsa_sa_params &sa, mmpbsa_frame_terms *out)
 {
-    FrameCoords f;
-    stage_frame(w, s, xyz, f);
-    const DevTopo *tops[3] = {&s->ctop, &s->rtop, &s->ltop};
-    const double *ds[3] = {f.dc, f.r.get(), f.l.get()};
-    const double *hs[3] = {f.hcp, f.hr.data(), f.hl.data()};
-    mmpbsa_subsystem_energy *e[3] = {&out->complex_, &out->receptor, &out->ligand};
-    for (int k = 0; k < 3; ++k) {
-        mmpbsa_mm_terms mm;
-        run_mm(w, *tops[k], ds[k], &mm);
-        e[k]->mm = mm_total(mm);
-        e[k]->polar = polar_of(w, *tops[k], ds[k], hs[k], method, gb, pb);
-        double tsasa = 0.0, tsa = 0.0;
-        run_sa(w, *tops[k], ds[k], hs[k], sa, nullptr, nullptr, &tsasa, &tsa);
-        e[k]->sa = tsa;
-    }
-    out->delta_mm = out->complex_.mm - out->receptor.mm - out->ligand.mm;
-    out->delta_polar = out->complex_.polar - out->receptor.polar - out->ligand.polar;
-    out->delta_sa = out->complex_.sa - out->receptor.sa - out->ligand.sa;
-    out->delta_total = out->delta_mm + out->delta_polar + out->delta_sa;
+    frames_group(w, s, xyz, 1, 0, method, gb, pb, sa, out, false);
 }
 
 void check_xyz(const double *xyz) {
@@ -456,35 +494,13 @@ int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_fr
                 frame_energy(*ctx->w, s, xyz_frames + stride * f, method, g, p, a, out + f);
             return;
         }
-        // batched: frames spread over concurrent device streams
-        const int nw = std::max(1, std::min(n_frames, 4));
-        while ((int)s->workers.size() < nw) s->workers.emplace_back(new Worker());
-        std::atomic<int> next(0);
-        std::vector<std::string> errs(nw);
-        std::vector<int> codes(nw, 0);
-        std::vector<std::thread> th;
-        for (int k = 0; k < nw; ++k) {
-            th.emplace_back([&, k] {
-                try {
-                    HIP_OK(hipSetDevice(ctx->device));
-                    for (;;) {
-                        const int f = next.fetch_add(1);
-                        if (f >= n_frames) break;
-                        frame_energy(*s->workers[k], s, xyz_frames + stride * f, method, g, p, a,
-                                     out + f);
-                    }
-                } catch (const DeviceError &e) {
-                    codes[k] = e.code;
-                    errs[k] = e.what();
-                } catch (const std::exception &e) {
-                    codes[k] = MMPBSA_ERR_DEVICE;
-                    errs[k] = e.what();
-                }
-            });
+        // batched: groups of frames whose PB solves advance together
+        const int group = 8;
+        for (int f0 = 0; f0 < n_frames; f0 += group) {
+            const int nf = std::min(group, n_frames - f0);
+            frames_group(*ctx->w, s, xyz_frames + stride * f0, nf, stride, method, g, p, a,
+                         out + f0, true);
         }
-        for (auto &t : th) t.join();
-        for (int k = 0; k < nw; ++k)
-            if (codes[k]) throw DeviceError(codes[k], errs[k]);
     });
 }
 

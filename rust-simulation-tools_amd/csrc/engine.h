@@ -148,7 +148,50 @@ void run_sa(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xy
 void run_pb(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xyz,
             const mmpbsa_pb_params &p, mmpbsa_pb_result *out);
 
+// One PB energy (compute_pb_energy) of one subsystem in one frame.  Jobs run
+// together through run_pb_jobs: every multigrid solve of a stage (all coarse
+// focusing solves, then all fine solves) advances in shared launches.
+struct PbJob {
+    const DevTopo *t = nullptr;
+    const double *d_xyz = nullptr;
+    const double *h_xyz = nullptr;
+    mmpbsa_pb_params p{};
+    mmpbsa_pb_result *out = nullptr;
+    struct State;
+    std::shared_ptr<State> st;
+};
+// batched: replay solver cycles from a hipGraph (else launch them directly).
+void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched);
+
 // FP64 kappa (gb_energy.rs:126-139)
 double compute_kappa(double salt, double eps, double temperature);
+
+// ---- geometry shared by SA and PB (engine.cpp) -------------------------------
+struct DevCells {
+    DBuf<int> cell_of, start, fill, atoms;
+    CellList cl;
+};
+void bbox(int n, const double *xyz, double *mn, double *mx);
+GridDesc auto_grid(const double *mn, const double *mx, double h, double buffer);
+GridDesc coarse_grid(const double *mn, const double *mx, double h, double fillratio,
+                     int64_t fscale);
+void build_cells(Worker &w, const double *d_xyz, int n, const double *mn, const double *mx,
+                 double cs, DevCells &c);
+
+struct Tuning {
+    int64_t tail_nodes = 16384;
+    int batch_solves = 1 << 20;
+};
+Tuning &tuning();
+
+// Kernel timing of the finest-grid smoother (no counterpart in the reference).
+struct ProfileStats {
+    bool enabled = false;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+    double alg_bytes = 0.0;
+};
+ProfileStats &profile_stats();
+std::mutex &profile_mutex();
 
 }  // namespace mmpbsa

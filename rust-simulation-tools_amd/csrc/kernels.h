@@ -63,8 +63,9 @@ struct GridDesc {
     __host__ __device__ int64_t size() const { return (int64_t)nx * ny * nz; }
 };
 int64_t boundary_count(const GridDesc &g);
+size_t dh_work_size(const GridDesc &g, int n);  // doubles of scratch
 void launch_dh_boundary(const GridDesc &g, int n, const double *xyz, const double *q,
-                        double kappa, double eps_out, double *phi, hipStream_t s);
+                        double kappa, double eps_out, double *phi, double *work, hipStream_t s);
 
 // Decomposition atom sums: for each atom a in A, sum over b in B of the
 // half pair vdW / elec (dd) into work, then reduce into out_vdw / out_elec.

@@ -1,4 +1,7 @@
 // mg.h -- device views and launchers of the multigrid kernels (mg.hip).
+// Every kernel serves a batch of independent solves: blockIdx.y (or
+// blockIdx.x for one-block kernels) selects the solve, and each solve has its
+// own status word, so converged solves drop out while the rest continue.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,28 +10,39 @@
 
 namespace mmpbsa {
 
-// Solver status word shared by every kernel of one solve.
+// Solver status word of one solve.
 // state: 0 running, 1 converged, 2 diverged, 3 out of cycles.
 struct MgStatus {
     int state;
-    int cycles;
+    int cycles;      // V-cycles completed (the reference's iteration count)
     int growing;
-    int pad;
+    int max_cycles;
     double best;
     double prev;
     double rms;
+    double tol;
+    double count;    // interior nodes (RMS denominator)
 };
 
-// One FP32 level of the correction V-cycle.
+// One FP32 level of the correction V-cycle.  nx == 0 marks "no work for this
+// solve at this level" in a batched launch.
 struct MgLevelDev {
     int nx, ny, nz;
     int uni;           // uniform dielectric, zero kappa
     float eps_u;
     float ihx, ihy, ihz;
+    float omega;
     const float *ex, *ey, *ez, *k2;  // coefficients (unused when uni)
     float *e;          // correction
     float *f;          // right-hand side
     float *res;        // residual scratch
+};
+
+constexpr int kMaxTail = 8;
+// Levels handled by the one-workgroup tail kernel (coarse end of the V-cycle).
+struct MgTailDev {
+    int n;                     // number of levels, n == 0: no tail
+    MgLevelDev lv[kMaxTail];   // lv[0] is entered with f set and e zero
 };
 
 // FP64 outer step / SOR view of the finest grid.
@@ -45,18 +59,30 @@ struct MgOuterDev {
     const float *e;                     // correction to add (nullptr: none)
     float *f0;                          // FP32 residual for the V-cycle
     double *partials;                   // per-block sums of r^2
+    int nbk;                            // number of partials
 };
 
-void mg_rb_sweep(const MgLevelDev &L, int color, float omega, const MgStatus *st, hipStream_t s);
-void mg_residual32(const MgLevelDev &L, const MgStatus *st, hipStream_t s);
-void mg_restrict(const MgLevelDev &F, const MgLevelDev &C, const MgStatus *st, hipStream_t s);
-void mg_prolong(const MgLevelDev &C, const MgLevelDev &F, const MgStatus *st, hipStream_t s);
-void mg_inject(const MgLevelDev &F, const MgLevelDev &C, hipStream_t s);
-void mg_zero(float *p, int64_t n, const MgStatus *st, hipStream_t s);
-int mg_outer_blocks(const MgOuterDev &O);
-void mg_outer64(const MgOuterDev &O, const MgStatus *st, hipStream_t s);
-void mg_check(const double *partials, int nbk, double count, double tol, int cycle,
-              int max_cycles, MgStatus *st, hipStream_t s);
+// Batched launchers.  L / C / F / O / T are device arrays with one entry per
+// solve; `max_nodes` is the largest node count among the entries.
+void mg_rb_sweep(const MgLevelDev *L, int n_solves, int64_t max_work, int color, int first,
+                 const MgStatus *st, hipStream_t s);
+void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
+                   hipStream_t s);
+void mg_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_coarse,
+                 const MgStatus *st, hipStream_t s);
+void mg_prolong(const MgLevelDev *C, const MgLevelDev *F, int n_solves, int64_t max_fine,
+                const MgStatus *st, hipStream_t s);
+void mg_tail(const MgTailDev *T, int n_solves, const MgStatus *st, hipStream_t s);
+void mg_outer64(const MgOuterDev *O, int n_solves, int64_t max_nodes, const MgStatus *st,
+                hipStream_t s);
+// as mg_outer64, also clearing the boundary nodes of ezero[solve] (may be null)
+void mg_outer64_z(const MgOuterDev *O, float *const *ezero, int n_solves, int64_t max_nodes,
+                  const MgStatus *st, hipStream_t s);
+void mg_check(const MgOuterDev *O, int n_solves, int count_cycle, MgStatus *st, hipStream_t s);
+void mg_status_init(MgStatus *st, const MgStatus *init_host, int n, hipStream_t s);
+void mg_inject(const MgLevelDev &F, const MgLevelDev &C, hipStream_t s);  // host-side structs
+
+// Single-solve FP64 Chebyshev SOR (fallback path).
 int sor64_blocks(const MgOuterDev &O);
 void mg_sor64(const MgOuterDev &O, int color, double omega, double *partials, const MgStatus *st,
               hipStream_t s);

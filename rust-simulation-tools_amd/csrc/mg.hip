@@ -1,76 +1,83 @@
-// mg.hip -- linearized-PB multigrid on gfx950.
+// mg.hip -- linearized-PB multigrid on gfx950, batched over independent solves.
 //
 // The reference V-cycle (pb_solver.rs:842-939) is affine in (phi, rhs): one
-// cycle maps phi to phi + M (b - A phi).  It is run here in correction form:
+// cycle maps phi to phi + M (b - A phi).  It runs here in correction form:
 //   r_k  = b - A phi_k             FP64, fused with phi_k = phi_{k-1} + e_{k-1}
-//   e_k  = V(0, r_k)               FP32, same smoother, omega per level, 0.25
-//                                  restriction scale and prolongation onto every
-//                                  node (even-sized levels extrapolate, as the
-//                                  reference does) -- so the iterate, its
+//   e_k  = V(0, r_k)               FP32, same smoother and omega per level, the
+//                                  0.25 restriction scale, and prolongation onto
+//                                  every node (even-sized levels extrapolate, as
+//                                  the reference does) -- so the iterate, its
 //                                  boundary drift and the cycle count follow the
 //                                  FP64 reference to FP32 precision of e_k
-//   convergence: RMS of r_k (FP64), the reference's test at the same cycle.
-// A device status word stops every later kernel once the solve has converged,
-// diverged or run out of cycles, so the host can enqueue cycles in batches.
+//   test: RMS of r_k (FP64), the reference's convergence test at that cycle.
+// Each solve has a status word; kernels of a finished solve return at once,
+// so cycles can be enqueued (or replayed from a graph) in batches.
+// Levels below ~16k nodes run inside one workgroup (mg_tail), which replaces
+// roughly a hundred tiny launches per cycle.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
 
-#include "kernels.h"
+#include "common.h"
 #include "mg.h"
 
 namespace mmpbsa {
 
 __device__ __forceinline__ bool stopped(const MgStatus *st) { return st->state != 0; }
 
-// ---- FP32 red-black half-sweep (sor_smooth, pb_solver.rs:214-327) -----------
-__global__ __launch_bounds__(256) void rb_sweep_kernel(MgLevelDev L, int color, float omega,
-                                                       const MgStatus *__restrict__ st)
+// ---- node operations shared by the batched kernels and the tail --------------
+// mode 0: ordinary update; 1: first red half-sweep of a zero correction (no
+// neighbour reads); 2: first black half-sweep (own old value taken as zero).
+__device__ __forceinline__ void rb_node(const MgLevelDev &L, int64_t idx, int mode)
 {
-    if (stopped(st)) return;
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const int half = (nx - 1) / 2;  // slots per row
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t rows = (int64_t)(ny - 2) * (nz - 2);
-    if (t >= rows * half) return;
-    const int s = (int)(t % half);
-    const int64_t row = t / half;
-    const int iy = 1 + (int)(row % (ny - 2));
-    const int iz = 1 + (int)(row / (ny - 2));
-    const int ix = 1 + ((1 + iy + iz + color) & 1) + 2 * s;
-    if (ix > nx - 2) return;
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
-    const int64_t idx = ix + sy * iy + sz * iz;
+    const int64_t sy = L.nx, sz = (int64_t)L.nx * L.ny;
     float *__restrict__ e = L.e;
-    float nb, dg;
+    float nb = 0.0f, dg;
     if (L.uni) {
         const float u = L.eps_u;
-        nb = u * ((e[idx + 1] + e[idx - 1]) * L.ihx + (e[idx + sy] + e[idx - sy]) * L.ihy +
-                  (e[idx + sz] + e[idx - sz]) * L.ihz);
+        if (mode != 1)
+            nb = u * ((e[idx + 1] + e[idx - 1]) * L.ihx + (e[idx + sy] + e[idx - sy]) * L.ihy +
+                      (e[idx + sz] + e[idx - sz]) * L.ihz);
         dg = u * (2.0f * L.ihx + 2.0f * L.ihy + 2.0f * L.ihz);
     } else {
         const float exp_ = L.ex[idx], exm = L.ex[idx - 1];
         const float eyp = L.ey[idx], eym = L.ey[idx - sy];
         const float ezp = L.ez[idx], ezm = L.ez[idx - sz];
-        nb = fmaf(exp_, e[idx + 1], exm * e[idx - 1]) * L.ihx +
-             fmaf(eyp, e[idx + sy], eym * e[idx - sy]) * L.ihy +
-             fmaf(ezp, e[idx + sz], ezm * e[idx - sz]) * L.ihz;
+        if (mode != 1)
+            nb = fmaf(exp_, e[idx + 1], exm * e[idx - 1]) * L.ihx +
+                 fmaf(eyp, e[idx + sy], eym * e[idx - sy]) * L.ihy +
+                 fmaf(ezp, e[idx + sz], ezm * e[idx - sz]) * L.ihz;
         dg = (exp_ + exm) * L.ihx + (eyp + eym) * L.ihy + (ezp + ezm) * L.ihz + L.k2[idx];
     }
-    const float old = e[idx];
+    const float old = mode == 0 ? e[idx] : 0.0f;
     const float gs = (nb + L.f[idx]) / dg;
-    e[idx] = fmaf(omega, gs - old, old);
+    e[idx] = fmaf(L.omega, gs - old, old);
 }
 
-// residual res = f + nb - dg * e on the interior, 0 on the boundary
-__global__ __launch_bounds__(256) void residual32_kernel(MgLevelDev L,
-                                                         const MgStatus *__restrict__ st)
+// work item t of a half-sweep -> node index (or -1)
+__device__ __forceinline__ int64_t rb_index(const MgLevelDev &L, int64_t t, int color)
 {
-    if (stopped(st)) return;
+    const int half = (L.nx - 1) / 2;
+    const int64_t rows = (int64_t)(L.ny - 2) * (L.nz - 2);
+    if (t >= rows * half) return -1;
+    const int s = (int)(t % half);
+    const int64_t row = t / half;
+    const int iy = 1 + (int)(row % (L.ny - 2));
+    const int iz = 1 + (int)(row / (L.ny - 2));
+    const int ix = 1 + ((1 + iy + iz + color) & 1) + 2 * s;
+    if (ix > L.nx - 2) return -1;
+    return ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
+}
+
+__device__ __forceinline__ int64_t rb_work(const MgLevelDev &L)
+{
+    if (L.nx < 3 || L.ny < 3 || L.nz < 3) return 0;
+    return (int64_t)(L.ny - 2) * (L.nz - 2) * ((L.nx - 1) / 2);
+}
+
+__device__ __forceinline__ void residual_node(const MgLevelDev &L, int64_t idx)
+{
     const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const int64_t N = (int64_t)nx * ny * nz;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= N) return;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     const int iz = (int)(idx / sz);
     const int iy = (int)((idx - (int64_t)iz * sz) / nx);
@@ -98,15 +105,9 @@ __global__ __launch_bounds__(256) void residual32_kernel(MgLevelDev L,
     L.res[idx] = L.f[idx] + nb - dg * e[idx];
 }
 
-// restrict_into (pb_solver.rs:578-624): normalised full weighting, x scale.
-// Also clears the coarse correction (the reference zeroes it before recursing).
-__global__ __launch_bounds__(256) void restrict_kernel(MgLevelDev F, MgLevelDev C, float scale,
-                                                       const MgStatus *__restrict__ st)
+// restrict_into (pb_solver.rs:578-624) for coarse node idx; clears its correction.
+__device__ __forceinline__ void restrict_node(const MgLevelDev &F, const MgLevelDev &C, int64_t idx)
 {
-    if (stopped(st)) return;
-    const int64_t N = (int64_t)C.nx * C.ny * C.nz;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= N) return;
     const int64_t csz = (int64_t)C.nx * C.ny;
     const int cz = (int)(idx / csz);
     const int cy = (int)((idx - (int64_t)cz * csz) / C.nx);
@@ -128,19 +129,13 @@ __global__ __launch_bounds__(256) void restrict_kernel(MgLevelDev F, MgLevelDev 
             }
         }
     }
-    C.f[idx] = scale * sum / ws;
+    C.f[idx] = 0.25f * sum / ws;
     C.e[idx] = 0.0f;
 }
 
-// prolongate_add (pb_solver.rs:628-681): trilinear with the coarse cell
-// clamped to n_c - 2, added to every fine node.
-__global__ __launch_bounds__(256) void prolong_kernel(MgLevelDev C, MgLevelDev F,
-                                                      const MgStatus *__restrict__ st)
+// prolongate_add (pb_solver.rs:628-681) for fine node idx
+__device__ __forceinline__ void prolong_node(const MgLevelDev &C, const MgLevelDev &F, int64_t idx)
 {
-    if (stopped(st)) return;
-    const int64_t N = (int64_t)F.nx * F.ny * F.nz;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= N) return;
     const int64_t fsz = (int64_t)F.nx * F.ny;
     const int fz = (int)(idx / fsz);
     const int fy = (int)((idx - (int64_t)fz * fsz) / F.nx);
@@ -160,38 +155,149 @@ __global__ __launch_bounds__(256) void prolong_kernel(MgLevelDev C, MgLevelDev F
     F.e[idx] += c0 * (1.0f - dz) + c1 * dz;
 }
 
-// coarsen_dielectrics / coarsen_kappa by injection (pb_solver.rs:684-748)
-__global__ void inject_kernel(MgLevelDev F, MgLevelDev C)
+__device__ __forceinline__ int64_t nodes(const MgLevelDev &L) { return (int64_t)L.nx * L.ny * L.nz; }
+
+// ---- batched kernels (blockIdx.y = solve) -----------------------------------
+__global__ __launch_bounds__(256) void rb_sweep_kernel(const MgLevelDev *__restrict__ Ls, int color,
+                                                       int first, const MgStatus *__restrict__ st)
 {
-    const int64_t N = (int64_t)C.nx * C.ny * C.nz;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= N) return;
-    const int64_t csz = (int64_t)C.nx * C.ny;
-    const int cz = (int)(idx / csz);
-    const int cy = (int)((idx - (int64_t)cz * csz) / C.nx);
-    const int cx = (int)(idx - (int64_t)cz * csz - (int64_t)cy * C.nx);
-    const int x = min(2 * cx, F.nx - 1), y = min(2 * cy, F.ny - 1), z = min(2 * cz, F.nz - 1);
-    const int64_t f = x + (int64_t)F.nx * (y + (int64_t)F.ny * z);
-    // the coarse level owns these arrays; they are const only for the smoother
-    const_cast<float *>(C.ex)[idx] = F.ex[f];
-    const_cast<float *>(C.ey)[idx] = F.ey[f];
-    const_cast<float *>(C.ez)[idx] = F.ez[f];
-    const_cast<float *>(C.k2)[idx] = F.k2[f];
+    const int s = blockIdx.y;
+    if (stopped(st + s)) return;
+    const MgLevelDev L = Ls[s];
+    if (L.nx == 0) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t idx = rb_index(L, t, color);
+    if (idx < 0) return;
+    rb_node(L, idx, first ? (color == 0 ? 1 : 2) : 0);
 }
 
-__global__ void zero_kernel(float *p, int64_t n, const MgStatus *__restrict__ st)
+__global__ __launch_bounds__(256) void residual32_kernel(const MgLevelDev *__restrict__ Ls,
+                                                         const MgStatus *__restrict__ st)
 {
-    if (st && stopped(st)) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
+    const int s = blockIdx.y;
+    if (stopped(st + s)) return;
+    const MgLevelDev L = Ls[s];
+    if (L.nx == 0) return;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nodes(L)) return;
+    residual_node(L, idx);
+}
+
+__global__ __launch_bounds__(256) void restrict_kernel(const MgLevelDev *__restrict__ Fs,
+                                                       const MgLevelDev *__restrict__ Cs,
+                                                       const MgStatus *__restrict__ st)
+{
+    const int s = blockIdx.y;
+    if (stopped(st + s)) return;
+    const MgLevelDev C = Cs[s];
+    if (C.nx == 0) return;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nodes(C)) return;
+    restrict_node(Fs[s], C, idx);
+}
+
+__global__ __launch_bounds__(256) void prolong_kernel(const MgLevelDev *__restrict__ Cs,
+                                                      const MgLevelDev *__restrict__ Fs,
+                                                      const MgStatus *__restrict__ st)
+{
+    const int s = blockIdx.y;
+    if (stopped(st + s)) return;
+    const MgLevelDev F = Fs[s];
+    if (F.nx == 0) return;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nodes(F)) return;
+    prolong_node(Cs[s], F, idx);
+}
+
+// One workgroup per solve runs the V-cycle over its tail levels.
+__global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict__ Ts,
+                                                    const MgStatus *__restrict__ st)
+{
+    constexpr int kLds = 16384;  // floats: the coarsest levels live in LDS
+    __shared__ float lds[kLds];
+    const int s = blockIdx.x;
+    if (stopped(st + s)) return;
+    const int n = Ts[s].n;
+    if (n == 0) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    MgLevelDev lv[kMaxTail];
+    for (int l = 0; l < n; ++l) lv[l] = Ts[s].lv[l];
+    {
+        // from the coarsest level upwards, move levels (except the entry level,
+        // whose correction is read back by the caller) into LDS while they fit
+        int used = 0;
+        for (int l = n - 1; l >= 1; --l) {
+            const int N = (int)nodes(lv[l]);
+            const int need = (lv[l].uni ? 3 : 7) * N;
+            if (used + need > kLds) break;
+            float *base = lds + used;
+            if (!lv[l].uni) {
+                float *cx = base + 3 * N, *cy = base + 4 * N, *cz = base + 5 * N, *ck = base + 6 * N;
+                for (int i = tid; i < N; i += nt) {
+                    cx[i] = lv[l].ex[i];
+                    cy[i] = lv[l].ey[i];
+                    cz[i] = lv[l].ez[i];
+                    ck[i] = lv[l].k2[i];
+                }
+                lv[l].ex = cx;
+                lv[l].ey = cy;
+                lv[l].ez = cz;
+                lv[l].k2 = ck;
+            }
+            lv[l].e = base;
+            lv[l].f = base + N;
+            lv[l].res = base + 2 * N;
+            used += need;
+        }
+        __syncthreads();
+    }
+    auto sweeps = [&](const MgLevelDev &L, int count) {
+        const int64_t w = rb_work(L);
+        for (int k = 0; k < count; ++k)
+            for (int color = 0; color < 2; ++color) {
+                for (int64_t t = tid; t < w; t += nt) {
+                    const int64_t idx = rb_index(L, t, color);
+                    if (idx >= 0) rb_node(L, idx, 0);
+                }
+                __syncthreads();
+            }
+    };
+    {
+        const MgLevelDev L0 = lv[0];
+        for (int64_t i = tid; i < nodes(L0); i += nt) L0.e[i] = 0.0f;
+        __syncthreads();
+    }
+    for (int l = 0; l + 1 < n; ++l) {
+        const MgLevelDev L = lv[l];
+        const MgLevelDev C = lv[l + 1];
+        sweeps(L, 3);
+        for (int64_t i = tid; i < nodes(L); i += nt) residual_node(L, i);
+        __syncthreads();
+        for (int64_t i = tid; i < nodes(C); i += nt) restrict_node(L, C, i);
+        __syncthreads();
+    }
+    sweeps(lv[n - 1], 50);
+    for (int l = n - 2; l >= 0; --l) {
+        const MgLevelDev L = lv[l];
+        const MgLevelDev C = lv[l + 1];
+        for (int64_t i = tid; i < nodes(L); i += nt) prolong_node(C, L, i);
+        __syncthreads();
+        sweeps(L, 3);
+    }
 }
 
 // ---- FP64 outer step: phi_out = phi_in + e, f0 = b - A phi_out (stencil_at,
-// pb_solver.rs:159-194), block partial sums of r^2.
-__global__ __launch_bounds__(256) void outer64_kernel(MgOuterDev O, const MgStatus *__restrict__ st)
+// pb_solver.rs:159-194), block partial sums of r^2; clears the boundary of the
+// next correction buffer (ezero) for the next V-cycle.
+__global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restrict__ Os,
+                                                      float *const *__restrict__ ezero,
+                                                      const MgStatus *__restrict__ st)
 {
     __shared__ double red[4];
-    if (stopped(st)) return;
+    const int s = blockIdx.y;
+    if (stopped(st + s)) return;
+    const MgOuterDev O = Os[s];
+    if (blockIdx.x >= (unsigned)O.nbk) return;
     const int nx = O.nx, ny = O.ny, nz = O.nz;
     const int64_t N = (int64_t)nx * ny * nz;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,6 +313,7 @@ __global__ __launch_bounds__(256) void outer64_kernel(MgOuterDev O, const MgStat
         const int ix = (int)(idx - (int64_t)iz * sz - (int64_t)iy * nx);
         if (ix == 0 || iy == 0 || iz == 0 || ix == nx - 1 || iy == ny - 1 || iz == nz - 1) {
             O.f0[idx] = 0.0f;
+            if (ezero && ezero[s]) ezero[s][idx] = 0.0f;
         } else {
             double pxp, pxm, pyp, pym, pzp, pzm;
             if (e) {
@@ -251,13 +358,17 @@ __global__ __launch_bounds__(256) void outer64_kernel(MgOuterDev O, const MgStat
     if (threadIdx.x == 0) O.partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// Convergence / divergence test of solve_lpbe_multigrid (pb_solver.rs:1085-1147).
-__global__ __launch_bounds__(256) void mg_check_kernel(const double *__restrict__ partials, int nbk,
-                                                       double count, double tol, int cycle,
-                                                       int max_cycles, MgStatus *st)
+// Convergence / divergence test of solve_lpbe_multigrid (pb_solver.rs:1085-1147),
+// one block per solve.  count_cycle = 0 records the initial residual only.
+__global__ __launch_bounds__(256) void mg_check_kernel(const MgOuterDev *__restrict__ Os,
+                                                       int count_cycle, MgStatus *__restrict__ sts)
 {
     __shared__ double red[256];
+    const int s = blockIdx.x;
+    MgStatus *st = sts + s;
     if (st->state != 0) return;
+    const double *__restrict__ partials = Os[s].partials;
+    const int nbk = Os[s].nbk;
     double a = 0.0;
     for (int b = threadIdx.x; b < nbk; b += 256) a += partials[b];
     red[threadIdx.x] = a;
@@ -267,9 +378,11 @@ __global__ __launch_bounds__(256) void mg_check_kernel(const double *__restrict_
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
-    const double rms = count > 0.0 ? sqrt(red[0] / count) : 0.0;
+    const double rms = st->count > 0.0 ? sqrt(red[0] / st->count) : 0.0;
     st->rms = rms;
-    if (cycle == 0) return;  // residual of the initial potential: no test
+    if (!count_cycle) return;
+    const int cycle = st->cycles + 1;
+    st->cycles = cycle;
     if (rms < st->best) {
         st->best = rms;
         st->growing = 0;
@@ -277,17 +390,35 @@ __global__ __launch_bounds__(256) void mg_check_kernel(const double *__restrict_
         st->growing += 1;
     }
     const bool div = !isfinite(rms) || rms > st->best * 1e4 || (st->growing >= 5 && rms > st->prev);
-    st->cycles = cycle;
     if (div) {
         st->state = 2;
         return;
     }
     st->prev = rms;
-    if (rms < tol) {
+    if (rms < st->tol) {
         st->state = 1;
         return;
     }
-    if (cycle >= max_cycles) st->state = 3;
+    if (cycle >= st->max_cycles) st->state = 3;
+}
+
+// coarsen_dielectrics / coarsen_kappa by injection (pb_solver.rs:684-748)
+__global__ void inject_kernel(MgLevelDev F, MgLevelDev C)
+{
+    const int64_t N = nodes(C);
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    const int64_t csz = (int64_t)C.nx * C.ny;
+    const int cz = (int)(idx / csz);
+    const int cy = (int)((idx - (int64_t)cz * csz) / C.nx);
+    const int cx = (int)(idx - (int64_t)cz * csz - (int64_t)cy * C.nx);
+    const int x = min(2 * cx, F.nx - 1), y = min(2 * cy, F.ny - 1), z = min(2 * cz, F.nz - 1);
+    const int64_t f = x + (int64_t)F.nx * (y + (int64_t)F.ny * z);
+    // the coarse level owns these arrays; they are const only for the smoother
+    const_cast<float *>(C.ex)[idx] = F.ex[f];
+    const_cast<float *>(C.ey)[idx] = F.ey[f];
+    const_cast<float *>(C.ez)[idx] = F.ez[f];
+    const_cast<float *>(C.k2)[idx] = F.k2[f];
 }
 
 // ---- FP64 Chebyshev SOR (solve_lpbe_initial, pb_solver.rs:373-447) ----------
@@ -369,63 +500,78 @@ __global__ __launch_bounds__(256) void sor_check_kernel(const double *__restrict
 }
 
 // ---- launchers ---------------------------------------------------------------
-static inline dim3 grid1(int64_t n) { return dim3((unsigned)ceil_div(n, 256)); }
+static inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_div(n, 256)); }
 
-void mg_rb_sweep(const MgLevelDev &L, int color, float omega, const MgStatus *st, hipStream_t s)
+void mg_rb_sweep(const MgLevelDev *L, int n_solves, int64_t max_work, int color, int first,
+                 const MgStatus *st, hipStream_t s)
 {
-    const int64_t work = (int64_t)(L.ny - 2) * (L.nz - 2) * ((L.nx - 1) / 2);
-    if (work <= 0) return;
-    hipLaunchKernelGGL(rb_sweep_kernel, grid1(work), dim3(256), 0, s, L, color, omega, st);
+    if (max_work <= 0 || n_solves <= 0) return;
+    hipLaunchKernelGGL(rb_sweep_kernel, dim3(nblk(max_work), n_solves), dim3(256), 0, s, L, color,
+                       first, st);
     LAUNCH_OK();
 }
 
-void mg_residual32(const MgLevelDev &L, const MgStatus *st, hipStream_t s)
+void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
+                   hipStream_t s)
 {
-    hipLaunchKernelGGL(residual32_kernel, grid1((int64_t)L.nx * L.ny * L.nz), dim3(256), 0, s, L,
+    if (max_nodes <= 0 || n_solves <= 0) return;
+    hipLaunchKernelGGL(residual32_kernel, dim3(nblk(max_nodes), n_solves), dim3(256), 0, s, L, st);
+    LAUNCH_OK();
+}
+
+void mg_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_coarse,
+                 const MgStatus *st, hipStream_t s)
+{
+    if (max_coarse <= 0 || n_solves <= 0) return;
+    hipLaunchKernelGGL(restrict_kernel, dim3(nblk(max_coarse), n_solves), dim3(256), 0, s, F, C, st);
+    LAUNCH_OK();
+}
+
+void mg_prolong(const MgLevelDev *C, const MgLevelDev *F, int n_solves, int64_t max_fine,
+                const MgStatus *st, hipStream_t s)
+{
+    if (max_fine <= 0 || n_solves <= 0) return;
+    hipLaunchKernelGGL(prolong_kernel, dim3(nblk(max_fine), n_solves), dim3(256), 0, s, C, F, st);
+    LAUNCH_OK();
+}
+
+void mg_tail(const MgTailDev *T, int n_solves, const MgStatus *st, hipStream_t s)
+{
+    if (n_solves <= 0) return;
+    hipLaunchKernelGGL(tail_kernel, dim3(n_solves), dim3(1024), 0, s, T, st);
+    LAUNCH_OK();
+}
+
+void mg_outer64(const MgOuterDev *O, int n_solves, int64_t max_nodes, const MgStatus *st,
+                hipStream_t s)
+{
+    mg_outer64_z(O, nullptr, n_solves, max_nodes, st, s);
+}
+
+void mg_outer64_z(const MgOuterDev *O, float *const *ezero, int n_solves, int64_t max_nodes,
+                  const MgStatus *st, hipStream_t s)
+{
+    if (n_solves <= 0) return;
+    hipLaunchKernelGGL(outer64_kernel, dim3(nblk(max_nodes), n_solves), dim3(256), 0, s, O, ezero,
                        st);
     LAUNCH_OK();
 }
 
-void mg_restrict(const MgLevelDev &F, const MgLevelDev &C, const MgStatus *st, hipStream_t s)
+void mg_check(const MgOuterDev *O, int n_solves, int count_cycle, MgStatus *st, hipStream_t s)
 {
-    hipLaunchKernelGGL(restrict_kernel, grid1((int64_t)C.nx * C.ny * C.nz), dim3(256), 0, s, F, C,
-                       0.25f, st);
+    if (n_solves <= 0) return;
+    hipLaunchKernelGGL(mg_check_kernel, dim3(n_solves), dim3(256), 0, s, O, count_cycle, st);
     LAUNCH_OK();
 }
 
-void mg_prolong(const MgLevelDev &C, const MgLevelDev &F, const MgStatus *st, hipStream_t s)
+void mg_status_init(MgStatus *st, const MgStatus *init_host, int n, hipStream_t s)
 {
-    hipLaunchKernelGGL(prolong_kernel, grid1((int64_t)F.nx * F.ny * F.nz), dim3(256), 0, s, C, F,
-                       st);
-    LAUNCH_OK();
+    HIP_OK(hipMemcpyAsync(st, init_host, sizeof(MgStatus) * n, hipMemcpyHostToDevice, s));
 }
 
 void mg_inject(const MgLevelDev &F, const MgLevelDev &C, hipStream_t s)
 {
-    hipLaunchKernelGGL(inject_kernel, grid1((int64_t)C.nx * C.ny * C.nz), dim3(256), 0, s, F, C);
-    LAUNCH_OK();
-}
-
-void mg_zero(float *p, int64_t n, const MgStatus *st, hipStream_t s)
-{
-    if (n <= 0) return;
-    hipLaunchKernelGGL(zero_kernel, grid1(n), dim3(256), 0, s, p, n, st);
-    LAUNCH_OK();
-}
-
-int mg_outer_blocks(const MgOuterDev &O) { return ceil_div((int64_t)O.nx * O.ny * O.nz, 256); }
-
-void mg_outer64(const MgOuterDev &O, const MgStatus *st, hipStream_t s)
-{
-    hipLaunchKernelGGL(outer64_kernel, dim3(mg_outer_blocks(O)), dim3(256), 0, s, O, st);
-    LAUNCH_OK();
-}
-
-void mg_check(const double *partials, int nbk, double count, double tol, int cycle,
-              int max_cycles, MgStatus *st, hipStream_t s)
-{
-    hipLaunchKernelGGL(mg_check_kernel, dim3(1), dim3(256), 0, s, partials, nbk, count, tol, cycle,
-                       max_cycles, st);
+    hipLaunchKernelGGL(inject_kernel, dim3(nblk((int64_t)C.nx * C.ny * C.nz)), dim3(256), 0, s, F, C);
     LAUNCH_OK();
 }
 

@@ -501,11 +501,14 @@ int64_t boundary_count(const GridDesc &g) { return bnd_count(g.nx, g.ny, g.nz); 
 
 // set_dh_boundary, pb_solver.rs:68-93.  Positions relative to the grid origin
 // in FP32 (|dx| error ~1e-5 A at r >= the grid buffer), terms FP32, sums FP64.
-__global__ __launch_bounds__(256) void dh_boundary_kernel(GridDesc g, int n,
+// Grid (boundary blocks, atom chunks): each block sums one atom chunk into
+// work[chunk][node]; dh_reduce_kernel adds the chunks in order.
+template <bool SALT>
+__global__ __launch_bounds__(256) void dh_boundary_kernel(GridDesc g, int n, int chunk,
                                                           const double *__restrict__ xyz,
                                                           const double *__restrict__ q,
                                                           float kappa, double cq_scale,
-                                                          double *__restrict__ phi)
+                                                          double *__restrict__ work)
 {
     __shared__ float4 sa[256];
     const int64_t nbnd = bnd_count(g.nx, g.ny, g.nz);
@@ -515,14 +518,15 @@ __global__ __launch_bounds__(256) void dh_boundary_kernel(GridDesc g, int n,
     if (act) bnd_node(g.nx, g.ny, g.nz, b, ix, iy, iz);
     const float px = (float)((double)ix * g.h), py = (float)((double)iy * g.h),
                 pz = (float)((double)iz * g.h);
+    const int a_lo = blockIdx.y * chunk, a_hi = min(n, a_lo + chunk);
     double acc = 0.0;
-    for (int a0 = 0; a0 < n; a0 += 256) {
+    for (int a0 = a_lo; a0 < a_hi; a0 += 256) {
         const int a = a0 + threadIdx.x;
-        if (a < n)
+        if (a < a_hi)
             sa[threadIdx.x] = make_float4((float)(xyz[3 * a] - g.ox), (float)(xyz[3 * a + 1] - g.oy),
                                           (float)(xyz[3 * a + 2] - g.oz), (float)(cq_scale * q[a]));
         __syncthreads();
-        const int m = min(256, n - a0);
+        const int m = min(256, a_hi - a0);
         float part = 0.0f;
         if (act) {
             for (int k = 0; k < m; ++k) {
@@ -530,25 +534,58 @@ __global__ __launch_bounds__(256) void dh_boundary_kernel(GridDesc g, int n,
                 const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
                 const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 const float r = __builtin_sqrtf(r2);
-                if (r > 1e-10f) {
-                    const float e = kappa > 0.0f ? __expf(-kappa * r) : 1.0f;
-                    part = fmaf(A.w * e, __builtin_amdgcn_rcpf(r), part);
-                }
+                const float w = SALT ? A.w * __expf(-kappa * r) : A.w;
+                part = r > 1e-10f ? fmaf(w, __builtin_amdgcn_rcpf(r), part) : part;
             }
             acc += (double)part;
         }
         __syncthreads();
     }
-    if (act) phi[(int64_t)ix + (int64_t)g.nx * (iy + (int64_t)g.ny * iz)] = acc;
+    if (act) work[(size_t)blockIdx.y * nbnd + b] = acc;
+}
+
+__global__ void dh_reduce_kernel(GridDesc g, int chunks, const double *__restrict__ work,
+                                 double *__restrict__ phi)
+{
+    const int64_t nbnd = bnd_count(g.nx, g.ny, g.nz);
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbnd) return;
+    double s = 0.0;
+    for (int c = 0; c < chunks; ++c) s += work[(size_t)c * nbnd + b];
+    int ix, iy, iz;
+    bnd_node(g.nx, g.ny, g.nz, b, ix, iy, iz);
+    phi[(int64_t)ix + (int64_t)g.nx * (iy + (int64_t)g.ny * iz)] = s;
+}
+
+static int dh_chunks(const GridDesc &g, int n)
+{
+    const int64_t blocks = std::max<int64_t>(1, ceil_div(boundary_count(g), 256));
+    const int want = (int)std::max<int64_t>(1, ceil_div(4096, blocks));
+    return std::max(1, std::min({want, 64, ceil_div(std::max(n, 1), 256)}));
+}
+
+size_t dh_work_size(const GridDesc &g, int n)
+{
+    return (size_t)dh_chunks(g, n) * (size_t)std::max<int64_t>(1, boundary_count(g));
 }
 
 void launch_dh_boundary(const GridDesc &g, int n, const double *xyz, const double *q,
-                        double kappa, double eps_out, double *phi, hipStream_t s)
+                        double kappa, double eps_out, double *phi, double *work, hipStream_t s)
 {
     const int64_t nb = boundary_count(g);
     if (nb == 0) return;
-    hipLaunchKernelGGL(dh_boundary_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, g, n, xyz, q,
-                       (float)kappa, kCoulomb / eps_out, phi);
+    const int chunks = dh_chunks(g, n);
+    const int chunk = ceil_div(ceil_div(std::max(n, 1), chunks), 256) * 256;
+    const int used = std::max(1, ceil_div(std::max(n, 1), chunk));
+    const dim3 grid(ceil_div(nb, 256), used);
+    if (kappa > 0.0)
+        hipLaunchKernelGGL(dh_boundary_kernel<true>, grid, dim3(256), 0, s, g, n, chunk, xyz, q,
+                           (float)kappa, kCoulomb / eps_out, work);
+    else
+        hipLaunchKernelGGL(dh_boundary_kernel<false>, grid, dim3(256), 0, s, g, n, chunk, xyz, q,
+                           0.0f, kCoulomb / eps_out, work);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(dh_reduce_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, g, used, work, phi);
     LAUNCH_OK();
 }
 

@@ -1,0 +1,840 @@
+// pb.cpp -- the Poisson-Boltzmann pipeline (pb_energy.rs:134-438): grid
+// assembly, batched multigrid solves (pb_solver.rs:1015-1167) and the energy
+// gather, for one or many (subsystem, frame) jobs at a time.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "engine.h"
+
+namespace mmpbsa {
+
+static const double kFourPiEc = 4.0 * kPi * kCoulomb;
+// Launch-shape knobs of the batched solver (tuned per device, see bench.py):
+// levels with at most tail_nodes nodes run inside the one-workgroup tail;
+// at most batch_solves solves share a launch.
+Tuning &tuning()
+{
+    static Tuning t = [] {
+        Tuning v;
+        if (const char *e = std::getenv("MMPBSA_TAIL_NODES")) v.tail_nodes = std::atoll(e);
+        if (const char *e = std::getenv("MMPBSA_BATCH_SOLVES")) v.batch_solves = std::atoi(e);
+        return v;
+    }();
+    return t;
+}
+
+ProfileStats &profile_stats()
+{
+    static ProfileStats s;
+    return s;
+}
+
+std::mutex &profile_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+struct MapSet {
+    bool uni = false;  // uniform dielectric and zero kappa
+    double eps_u = 1.0;
+    DBuf<double> ex, ey, ez, k2;
+    DBuf<float> fx, fy, fz, k2f;
+};
+
+static double spectral_radius(int nx, int ny, int nz)
+{
+    return (std::cos(kPi / (double)nx) + std::cos(kPi / (double)ny) + std::cos(kPi / (double)nz)) /
+           3.0;
+}
+
+static double estimate_omega(int nx, int ny, int nz)
+{
+    const double r = spectral_radius(nx, ny, nz);
+    return 2.0 / (1.0 + std::sqrt(1.0 - r * r));
+}
+
+// assign_dielectrics + assign_kappa for one grid (pb_grid.rs:827-985)
+static void build_maps(Worker &w, const DevTopo &t, const double *d_xyz, const DevCells &c,
+                       const GridDesc &g, const mmpbsa_pb_params &p, double ein, double eout,
+                       bool ionic, double kappa, MapSet &m)
+{
+    const int64_t N = g.size();
+    const bool uni_eps = std::fabs(ein - eout) < DBL_EPSILON;
+    if (uni_eps && !ionic) {
+        m.uni = true;
+        m.eps_u = ein;
+        return;
+    }
+    m.uni = false;
+    m.ex = DBuf<double>(w.pool, N);
+    m.ey = DBuf<double>(w.pool, N);
+    m.ez = DBuf<double>(w.pool, N);
+    m.k2 = DBuf<double>(w.pool, N);
+    m.fx = DBuf<float>(w.pool, N);
+    m.fy = DBuf<float>(w.pool, N);
+    m.fz = DBuf<float>(w.pool, N);
+    m.k2f = DBuf<float>(w.pool, N);
+    if (uni_eps) {
+        launch_fill_f64(m.ex.get(), ein, N, w.stream);
+        launch_fill_f64(m.ey.get(), ein, N, w.stream);
+        launch_fill_f64(m.ez.get(), ein, N, w.stream);
+        launch_fill_f32(m.fx.get(), (float)ein, N, w.stream);
+        launch_fill_f32(m.fy.get(), (float)ein, N, w.stream);
+        launch_fill_f32(m.fz.get(), (float)ein, N, w.stream);
+    } else {
+        DBuf<uint8_t> sas(w.pool, N), inside(w.pool, N);
+        const double probe = p.probe_radius;
+        if (probe <= 0.0) {
+            launch_inside(g, d_xyz, t.radii.p, 0.0, t.max_radius, c.cl, inside.get(), w.stream);
+        } else {
+            const double th2 = (probe / g.h) * (probe / g.h);
+            const double T = std::floor(th2);
+            if (T > 250.0)
+                throw DeviceError(MMPBSA_ERR_TOO_LARGE, "probe_radius / grid_spacing above 15.8");
+            launch_inside(g, d_xyz, t.radii.p, probe, t.max_radius, c.cl, sas.get(), w.stream);
+            DBuf<uint8_t> ext(w.pool, N), wa(w.pool, N), wb(w.pool, N);
+            DBuf<int> flag(w.pool, 1);
+            launch_ses(g, sas.get(), ext.get(), wa.get(), wb.get(), (int)T, inside.get(),
+                       flag.get(), w.pin_i + 8, w.stream);
+        }
+        DBuf<int> ovf(w.pool, 1);
+        launch_fill_i32(ovf.get(), 0, 1, w.stream);
+        launch_faces(g, inside.get(), d_xyz, t.radii.p, t.max_radius, c.cl, ein, eout, m.ex.get(),
+                     m.ey.get(), m.ez.get(), m.fx.get(), m.fy.get(), m.fz.get(), ovf.get(),
+                     w.stream);
+        HIP_OK(hipMemcpyAsync(w.pin_i + 9, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        w.sync();
+        if (w.pin_i[9]) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "face interval list overflow");
+    }
+    if (ionic) {
+        DBuf<uint8_t> ion(w.pool, N);
+        launch_inside(g, d_xyz, t.radii.p, p.ion_radius, t.max_radius, c.cl, ion.get(), w.stream);
+        const double kb2 = p.solvent_dielectric * kappa * kappa;
+        launch_kappa_map(g, ion.get(), kb2, m.k2.get(), m.k2f.get(), w.stream);
+    } else {
+        launch_fill_f64(m.k2.get(), 0.0, N, w.stream);
+        launch_fill_f32(m.k2f.get(), 0.0f, N, w.stream);
+    }
+}
+
+static MgOuterDev outer_view(const GridDesc &g, const MapSet &m, const double *rho)
+{
+    MgOuterDev O{};
+    O.nx = g.nx;
+    O.ny = g.ny;
+    O.nz = g.nz;
+    O.uni = m.uni ? 1 : 0;
+    O.eps_u = m.eps_u;
+    const double ih = 1.0 / (g.h * g.h);
+    O.ihx = O.ihy = O.ihz = ih;
+    O.scale = kFourPiEc;
+    O.ex = m.ex.get();
+    O.ey = m.ey.get();
+    O.ez = m.ez.get();
+    O.k2 = m.k2.get();
+    O.rho = rho;
+    return O;
+}
+
+__global__ void copy_boundary_kernel(GridDesc g, const double *__restrict__ src,
+                                     double *__restrict__ dst)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.size()) return;
+    const int64_t nxy = (int64_t)g.nx * g.ny;
+    const int z = (int)(idx / nxy);
+    const int y = (int)((idx - z * nxy) / g.nx);
+    const int x = (int)(idx - z * nxy - (int64_t)y * g.nx);
+    if (x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1)
+        dst[idx] = src[idx];
+}
+
+static MgStatus status_template(double tol, double count, int max_cycles)
+{
+    MgStatus s{};
+    s.state = 0;
+    s.cycles = 0;
+    s.growing = 0;
+    s.max_cycles = max_cycles;
+    s.best = DBL_MAX;
+    s.prev = DBL_MAX;
+    s.rms = 0.0;
+    s.tol = tol;
+    s.count = count;
+    return s;
+}
+
+// solve_lpbe_initial (pb_solver.rs:373-447): Chebyshev-ramped FP64 SOR from phi.
+static SolveInfo sor_solve(Worker &w, const GridDesc &g, const MapSet &m, const double *rho,
+                           double *phi, double tol, int64_t max_it)
+{
+    SolveInfo info;
+    info.final_residual = DBL_MAX;
+    if (max_it <= 0) return info;
+    MgOuterDev O = outer_view(g, m, rho);
+    O.phi_out = phi;
+    const int nbk = sor64_blocks(O);
+    DBuf<double> part(w.pool, 2 * (size_t)nbk);
+    DBuf<MgStatus> st(w.pool, 1);
+    const MgStatus init = status_template(tol, 0.0, 0);
+    mg_status_init(st.get(), &init, 1, w.stream);
+    const double nint = (g.nx > 2 && g.ny > 2 && g.nz > 2)
+                            ? (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2)
+                            : 0.0;
+    const double rj = spectral_radius(g.nx, g.ny, g.nz);
+    const double r2 = rj * rj;
+    double omega = 1.0;
+    int64_t k = 0;
+    MgStatus hs{};
+    while (k < max_it) {
+        const int64_t chunk_end = std::min<int64_t>(max_it, k + 64);
+        for (; k < chunk_end; ++k) {
+            mg_sor64(O, 0, omega, part.get(), st.get(), w.stream);
+            mg_sor64(O, 1, omega, part.get() + nbk, st.get(), w.stream);
+            mg_sor_check(part.get(), 2 * nbk, nint, tol, (int)(k + 1), (int)max_it, st.get(),
+                         w.stream);
+            omega = (k == 0) ? 1.0 / (1.0 - r2 / 2.0) : 1.0 / (1.0 - r2 * omega / 4.0);
+        }
+        HIP_OK(hipMemcpyAsync(w.pin, st.get(), sizeof(MgStatus), hipMemcpyDeviceToHost, w.stream));
+        w.sync();
+        std::memcpy(&hs, w.pin, sizeof(MgStatus));
+        if (hs.state != 0) break;
+    }
+    info.iterations = hs.cycles;
+    info.converged = hs.state == 1;
+    info.final_residual = hs.rms;
+    return info;
+}
+
+// ---- batched multigrid -------------------------------------------------------
+struct MgTask {
+    GridDesc g{};
+    const MapSet *m = nullptr;
+    const double *rho = nullptr;
+    double *phi = nullptr;  // boundary values in, solution out
+    double tol = 1e-6;
+    int64_t max_it = 10000;
+    SolveInfo info;
+};
+
+namespace {
+
+struct TaskLevel {
+    GridDesc g;
+    DBuf<float> ex, ey, ez, k2, e, e_alt, f, res;
+    MgLevelDev d;
+};
+
+struct TaskWork {
+    MgTask *t = nullptr;
+    std::vector<TaskLevel> L;
+    int tail_start = 0;
+    int maxv = 1;
+    DBuf<double> buf1, bc, partials;
+    MgOuterDev O[2];        // per cycle parity
+    MgOuterDev O0;          // cycle-0 residual
+};
+
+int64_t lv_nodes(const GridDesc &g) { return g.size(); }
+
+int64_t lv_work(const GridDesc &g)
+{
+    if (g.nx < 3 || g.ny < 3 || g.nz < 3) return 0;
+    return (int64_t)(g.ny - 2) * (g.nz - 2) * ((g.nx - 1) / 2);
+}
+
+MgLevelDev level_with_e(const MgLevelDev &d, float *e)
+{
+    MgLevelDev r = d;
+    r.e = e;
+    return r;
+}
+
+MgLevelDev null_level()
+{
+    MgLevelDev z{};
+    return z;
+}
+
+}  // namespace
+
+static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_graph);
+
+static void mg_solve_batch(Worker &w, std::vector<MgTask *> &tasks, bool use_graph)
+{
+    const size_t g = (size_t)std::max(1, tuning().batch_solves);
+    for (size_t a = 0; a < tasks.size(); a += g) {
+        std::vector<MgTask *> part(tasks.begin() + a, tasks.begin() + std::min(tasks.size(), a + g));
+        mg_solve_group(w, part, use_graph);
+    }
+}
+
+static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_graph)
+{
+    std::vector<MgTask *> mgt;
+    for (MgTask *t : tasks) {
+        const int md = std::min(t->g.nx, std::min(t->g.ny, t->g.nz));
+        if (md <= 8)
+            t->info = sor_solve(w, t->g, *t->m, t->rho, t->phi, t->tol, t->max_it);
+        else
+            mgt.push_back(t);
+    }
+    const int S = (int)mgt.size();
+    if (S == 0) return;
+    ProfileStats &prof = profile_stats();
+    const bool profiling = prof.enabled;
+    if (profiling) use_graph = false;
+
+    std::vector<TaskWork> W(S);
+    int maxT = 0;
+    for (int s = 0; s < S; ++s) {
+        TaskWork &tw = W[s];
+        MgTask *t = mgt[s];
+        tw.t = t;
+        tw.maxv = (int)std::min<int64_t>(200, std::max<int64_t>(1, t->max_it / 6));
+        int d[3] = {t->g.nx, t->g.ny, t->g.nz};
+        double h = t->g.h;
+        for (;;) {  // build_multigrid_levels (pb_solver.rs:776-836), min size 5
+            TaskLevel lv;
+            lv.g = t->g;
+            lv.g.nx = d[0];
+            lv.g.ny = d[1];
+            lv.g.nz = d[2];
+            lv.g.h = h;
+            tw.L.push_back(std::move(lv));
+            if (std::min(d[0], std::min(d[1], d[2])) <= 5) break;
+            for (int k = 0; k < 3; ++k) d[k] = (d[k] + 1) / 2;
+            h *= 2.0;
+        }
+        const int nl = (int)tw.L.size();
+        tw.tail_start = nl - 1;
+        for (int l = 0; l < nl; ++l)
+            if (lv_nodes(tw.L[l].g) <= tuning().tail_nodes) {
+                tw.tail_start = l;
+                break;
+            }
+        maxT = std::max(maxT, tw.tail_start);
+        const MapSet &m = *t->m;
+        for (int l = 0; l < nl; ++l) {
+            TaskLevel &lv = tw.L[l];
+            const int64_t N = lv_nodes(lv.g);
+            lv.e = DBuf<float>(w.pool, N);
+            if (l == 0) lv.e_alt = DBuf<float>(w.pool, N);
+            lv.f = DBuf<float>(w.pool, N);
+            if (l + 1 < nl) lv.res = DBuf<float>(w.pool, N);
+            MgLevelDev &dv = lv.d;
+            dv = MgLevelDev{};
+            dv.nx = lv.g.nx;
+            dv.ny = lv.g.ny;
+            dv.nz = lv.g.nz;
+            dv.uni = m.uni ? 1 : 0;
+            dv.eps_u = (float)m.eps_u;
+            const float ih = (float)(1.0 / (lv.g.h * lv.g.h));
+            dv.ihx = dv.ihy = dv.ihz = ih;
+            dv.omega = (float)estimate_omega(lv.g.nx, lv.g.ny, lv.g.nz);
+            dv.e = lv.e.get();
+            dv.f = lv.f.get();
+            dv.res = lv.res.get();
+            if (!m.uni) {
+                if (l == 0) {
+                    dv.ex = m.fx.get();
+                    dv.ey = m.fy.get();
+                    dv.ez = m.fz.get();
+                    dv.k2 = m.k2f.get();
+                } else {
+                    lv.ex = DBuf<float>(w.pool, N);
+                    lv.ey = DBuf<float>(w.pool, N);
+                    lv.ez = DBuf<float>(w.pool, N);
+                    lv.k2 = DBuf<float>(w.pool, N);
+                    dv.ex = lv.ex.get();
+                    dv.ey = lv.ey.get();
+                    dv.ez = lv.ez.get();
+                    dv.k2 = lv.k2.get();
+                    mg_inject(tw.L[l - 1].d, dv, w.stream);
+                }
+            }
+        }
+        const int64_t N0 = lv_nodes(t->g);
+        tw.buf1 = DBuf<double>(w.pool, N0);
+        tw.bc = DBuf<double>(w.pool, N0);
+        HIP_OK(hipMemcpyAsync(tw.bc.get(), t->phi, sizeof(double) * N0, hipMemcpyDeviceToDevice,
+                              w.stream));
+        MgOuterDev O = outer_view(t->g, m, t->rho);
+        O.f0 = tw.L[0].f.get();
+        O.nbk = ceil_div(N0, 256);
+        tw.partials = DBuf<double>(w.pool, O.nbk);
+        O.partials = tw.partials.get();
+        double *bufs[2] = {t->phi, tw.buf1.get()};
+        float *E[2] = {tw.L[0].e.get(), tw.L[0].e_alt.get()};
+        tw.O0 = O;
+        tw.O0.phi_in = tw.O0.phi_out = bufs[0];
+        tw.O0.e = nullptr;
+        for (int p = 0; p < 2; ++p) {
+            tw.O[p] = O;
+            tw.O[p].phi_in = bufs[1 - p];
+            tw.O[p].phi_out = bufs[p];
+            tw.O[p].e = E[p];
+        }
+    }
+
+    // ---- device descriptor tables (two parities) ----
+    // per parity: F[maxT][S], C[maxT][S], T[S], O[S], ezero[S]; plus O0[S], ezero0[S]
+    const size_t nF = (size_t)maxT * S;
+    struct Host {
+        std::vector<MgLevelDev> F, C;
+        std::vector<MgTailDev> T;
+        std::vector<MgOuterDev> O;
+        std::vector<float *> ez;
+    } H[2];
+    std::vector<int64_t> maxwork(maxT, 0), maxnodes(maxT, 0), maxcoarse(maxT, 0);
+    int64_t maxN0 = 0;
+    for (int p = 0; p < 2; ++p) {
+        H[p].F.assign(nF, null_level());
+        H[p].C.assign(nF, null_level());
+        H[p].T.assign(S, MgTailDev{});
+        H[p].O.resize(S);
+        H[p].ez.resize(S);
+        for (int s = 0; s < S; ++s) {
+            TaskWork &tw = W[s];
+            float *E[2] = {tw.L[0].e.get(), tw.L[0].e_alt.get()};
+            auto lev = [&](int l) {
+                return l == 0 ? level_with_e(tw.L[0].d, E[p]) : tw.L[l].d;
+            };
+            for (int l = 0; l < tw.tail_start; ++l) {
+                H[p].F[(size_t)l * S + s] = lev(l);
+                H[p].C[(size_t)l * S + s] = lev(l + 1);
+                maxwork[l] = std::max(maxwork[l], lv_work(tw.L[l].g));
+                maxnodes[l] = std::max(maxnodes[l], lv_nodes(tw.L[l].g));
+                maxcoarse[l] = std::max(maxcoarse[l], lv_nodes(tw.L[l + 1].g));
+            }
+            MgTailDev &T = H[p].T[s];
+            const int nl = (int)tw.L.size();
+            T.n = nl - tw.tail_start;
+            if (T.n > kMaxTail) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "too many multigrid levels");
+            for (int l = tw.tail_start; l < nl; ++l) T.lv[l - tw.tail_start] = lev(l);
+            H[p].O[s] = tw.O[p];
+            H[p].ez[s] = E[1 - p];
+            maxN0 = std::max(maxN0, lv_nodes(tw.t->g));
+        }
+    }
+    std::vector<MgOuterDev> O0(S);
+    std::vector<float *> ez0(S);
+    for (int s = 0; s < S; ++s) {
+        O0[s] = W[s].O0;
+        ez0[s] = W[s].L[0].e_alt.get();  // cycle 1 (parity 1) writes e_alt
+    }
+    // one device blob holding every table
+    auto bytes_of = [](size_t n, size_t sz) { return (n * sz + 255) & ~(size_t)255; };
+    const size_t szF = bytes_of(nF ? nF : 1, sizeof(MgLevelDev));
+    const size_t szT = bytes_of(S, sizeof(MgTailDev));
+    const size_t szO = bytes_of(S, sizeof(MgOuterDev));
+    const size_t szE = bytes_of(S, sizeof(float *));
+    const size_t per = 2 * szF + szT + szO + szE;
+    const size_t total = 2 * per + szO + szE + bytes_of(S, sizeof(MgStatus));
+    DBuf<char> blob(w.pool, total);
+    std::vector<char> hb(total, 0);
+    char *dbase = blob.get();
+    size_t off = 0;
+    struct Dev {
+        MgLevelDev *F, *C;
+        MgTailDev *T;
+        MgOuterDev *O;
+        float **ez;
+    } D[2];
+    for (int p = 0; p < 2; ++p) {
+        std::memcpy(hb.data() + off, H[p].F.data(), nF * sizeof(MgLevelDev));
+        D[p].F = (MgLevelDev *)(dbase + off);
+        off += szF;
+        std::memcpy(hb.data() + off, H[p].C.data(), nF * sizeof(MgLevelDev));
+        D[p].C = (MgLevelDev *)(dbase + off);
+        off += szF;
+        std::memcpy(hb.data() + off, H[p].T.data(), S * sizeof(MgTailDev));
+        D[p].T = (MgTailDev *)(dbase + off);
+        off += szT;
+        std::memcpy(hb.data() + off, H[p].O.data(), S * sizeof(MgOuterDev));
+        D[p].O = (MgOuterDev *)(dbase + off);
+        off += szO;
+        std::memcpy(hb.data() + off, H[p].ez.data(), S * sizeof(float *));
+        D[p].ez = (float **)(dbase + off);
+        off += szE;
+    }
+    std::memcpy(hb.data() + off, O0.data(), S * sizeof(MgOuterDev));
+    MgOuterDev *dO0 = (MgOuterDev *)(dbase + off);
+    off += szO;
+    std::memcpy(hb.data() + off, ez0.data(), S * sizeof(float *));
+    float **dez0 = (float **)(dbase + off);
+    off += szE;
+    std::vector<MgStatus> init(S);
+    for (int s = 0; s < S; ++s) {
+        const GridDesc &g = W[s].t->g;
+        init[s] = status_template(W[s].t->tol, (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2),
+                                  W[s].maxv);
+    }
+    std::memcpy(hb.data() + off, init.data(), S * sizeof(MgStatus));
+    MgStatus *dst = (MgStatus *)(dbase + off);
+    HIP_OK(hipMemcpyAsync(dbase, hb.data(), total, hipMemcpyHostToDevice, w.stream));
+
+    // cycle 0: residual of the boundary-only potential (no test)
+    mg_outer64_z(dO0, dez0, S, maxN0, dst, w.stream);
+    mg_check(dO0, S, 0, dst, w.stream);
+
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    std::vector<int> ev_level_cycle;
+    int cycle_counter = 0;
+    auto enqueue_cycle = [&](int p) {
+        ++cycle_counter;
+        for (int l = 0; l < maxT; ++l) {
+            const MgLevelDev *F = D[p].F + (size_t)l * S;
+            const MgLevelDev *C = D[p].C + (size_t)l * S;
+            for (int k = 0; k < 3; ++k)
+                for (int color = 0; color < 2; ++color) {
+                    hipEvent_t a = nullptr, b = nullptr;
+                    if (profiling && l == 0) {
+                        HIP_OK(hipEventCreate(&a));
+                        HIP_OK(hipEventCreate(&b));
+                        HIP_OK(hipEventRecord(a, w.stream));
+                    }
+                    mg_rb_sweep(F, S, maxwork[l], color, (l == 0 && k == 0) ? 1 : 0, dst, w.stream);
+                    if (profiling && l == 0) {
+                        HIP_OK(hipEventRecord(b, w.stream));
+                        evs.push_back({a, b});
+                        ev_level_cycle.push_back(cycle_counter);
+                    }
+                }
+            mg_residual32(F, S, maxnodes[l], dst, w.stream);
+            mg_restrict(F, C, S, maxcoarse[l], dst, w.stream);
+        }
+        mg_tail(D[p].T, S, dst, w.stream);
+        for (int l = maxT - 1; l >= 0; --l) {
+            const MgLevelDev *F = D[p].F + (size_t)l * S;
+            const MgLevelDev *C = D[p].C + (size_t)l * S;
+            mg_prolong(C, F, S, maxnodes[l], dst, w.stream);
+            for (int k = 0; k < 3; ++k)
+                for (int color = 0; color < 2; ++color) {
+                    hipEvent_t a = nullptr, b = nullptr;
+                    if (profiling && l == 0) {
+                        HIP_OK(hipEventCreate(&a));
+                        HIP_OK(hipEventCreate(&b));
+                        HIP_OK(hipEventRecord(a, w.stream));
+                    }
+                    mg_rb_sweep(F, S, maxwork[l], color, 0, dst, w.stream);
+                    if (profiling && l == 0) {
+                        HIP_OK(hipEventRecord(b, w.stream));
+                        evs.push_back({a, b});
+                        ev_level_cycle.push_back(cycle_counter);
+                    }
+                }
+        }
+        mg_outer64_z(D[p].O, D[p].ez, S, maxN0, dst, w.stream);
+        mg_check(D[p].O, S, 1, dst, w.stream);
+    };
+
+    int maxv = 0;
+    for (auto &tw : W) maxv = std::max(maxv, tw.maxv);
+    std::vector<MgStatus> hs(S);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+        HIP_OK(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_cycle(1);
+            enqueue_cycle(0);
+        } catch (...) {
+            hipGraph_t g2 = nullptr;
+            (void)hipStreamEndCapture(w.stream, &g2);
+            if (g2) (void)hipGraphDestroy(g2);
+            throw;
+        }
+        HIP_OK(hipStreamEndCapture(w.stream, &graph));
+        HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    const int chunk_cycles = 8;
+    int done_cycles = 0;
+    while (done_cycles < maxv) {
+        const int n = std::min(chunk_cycles, maxv - done_cycles + (maxv - done_cycles) % 2);
+        if (use_graph) {
+            for (int k = 0; k < n; k += 2) HIP_OK(hipGraphLaunch(exec, w.stream));
+        } else {
+            for (int k = 0; k < n; ++k) enqueue_cycle((done_cycles + k + 1) & 1);
+        }
+        done_cycles += n;
+        HIP_OK(hipMemcpyAsync(hs.data(), dst, sizeof(MgStatus) * S, hipMemcpyDeviceToHost,
+                              w.stream));
+        w.sync();
+        bool all = true;
+        for (int s = 0; s < S; ++s) all = all && hs[s].state != 0;
+        if (all) break;
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (profiling && !evs.empty()) {
+        double ms_tot = 0.0, bytes = 0.0;
+        for (size_t k = 0; k < evs.size(); ++k) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, evs[k].first, evs[k].second));
+            ms_tot += ms;
+            const int c = ev_level_cycle[k];
+            for (int s = 0; s < S; ++s) {
+                if (W[s].tail_start == 0 || c > hs[s].cycles) continue;
+                const GridDesc &g = W[s].t->g;
+                const double interior = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+                bytes += interior * (W[s].t->m->uni ? 6.0 : 14.0);
+            }
+            (void)hipEventDestroy(evs[k].first);
+            (void)hipEventDestroy(evs[k].second);
+        }
+        std::lock_guard<std::mutex> lk(profile_mutex());
+        prof.total_ms += ms_tot;
+        prof.launches += (int64_t)evs.size();
+        prof.alg_bytes += bytes;
+    }
+    for (int s = 0; s < S; ++s) {
+        TaskWork &tw = W[s];
+        MgTask *t = tw.t;
+        const int fin = hs[s].cycles;
+        const int64_t N0 = lv_nodes(t->g);
+        if (fin & 1)
+            HIP_OK(hipMemcpyAsync(t->phi, tw.buf1.get(), sizeof(double) * N0,
+                                  hipMemcpyDeviceToDevice, w.stream));
+        if (hs[s].state == 2) {
+            // divergence (pb_solver.rs:1110-1138): Chebyshev SOR from the partial
+            // potential with the boundary condition re-applied
+            hipLaunchKernelGGL(copy_boundary_kernel, dim3(ceil_div(N0, 256)), dim3(256), 0,
+                               w.stream, t->g, tw.bc.get(), t->phi);
+            LAUNCH_OK();
+            t->info = sor_solve(w, t->g, *t->m, t->rho, t->phi, t->tol, t->max_it);
+            continue;
+        }
+        t->info.iterations = hs[s].state == 1 ? fin : tw.maxv;
+        t->info.converged = hs[s].state == 1;
+        t->info.final_residual = hs[s].rms;
+    }
+    w.sync();
+}
+
+// ---- PB jobs -------------------------------------------------------------------
+struct PbJob::State {
+    GridDesc fine{}, cg{};
+    bool focus = false;
+    double kappa = 0.0;
+    double mn[3], mx[3];
+    DevCells cells;
+    DBuf<double> rho_f, rho_c;
+    struct Side {
+        double ein = 1.0, eout = 1.0;
+        bool ionic = false;
+        MapSet cm, fm;
+        DBuf<double> cphi, phi;
+        MgTask ct, ft;
+        bool have_bc = false;
+        mmpbsa_pb_solve_info *info = nullptr;
+    } side[2];
+};
+
+void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
+{
+    std::vector<PbJob *> act;
+    for (PbJob &j : jobs) {
+        std::memset(j.out, 0, sizeof(*j.out));
+        if (!(j.p.grid_spacing > 0.0)) throw DeviceError(MMPBSA_ERR_INVALID_ARG, "grid_spacing <= 0");
+        if (j.t->h.n == 0) continue;
+        act.push_back(&j);
+    }
+    // geometry, cell lists, charge maps (pb_energy.rs:332-372)
+    for (PbJob *j : act) {
+        j->st = std::make_shared<PbJob::State>();
+        PbJob::State &S = *j->st;
+        const mmpbsa_pb_params &p = j->p;
+        const int n = j->t->h.n;
+        S.kappa = compute_kappa(p.salt_concentration, p.solvent_dielectric, p.temperature);
+        bbox(n, j->h_xyz, S.mn, S.mx);
+        S.fine = auto_grid(S.mn, S.mx, p.grid_spacing, p.grid_buffer);
+        if (p.fine_dims[0] > 0) {
+            S.fine.nx = p.fine_dims[0];
+            S.fine.ny = p.fine_dims[1] > 0 ? p.fine_dims[1] : p.fine_dims[0];
+            S.fine.nz = p.fine_dims[2] > 0 ? p.fine_dims[2] : p.fine_dims[0];
+            const int d[3] = {S.fine.nx, S.fine.ny, S.fine.nz};
+            double o[3];
+            for (int k = 0; k < 3; ++k)
+                o[k] = 0.5 * (S.mn[k] + S.mx[k]) - 0.5 * ((double)(d[k] - 1) * p.grid_spacing);
+            S.fine.ox = o[0];
+            S.fine.oy = o[1];
+            S.fine.oz = o[2];
+        }
+        S.focus = p.fillratio > 0.0 && p.fscale > 1;
+        if (S.focus) S.cg = coarse_grid(S.mn, S.mx, p.grid_spacing, p.fillratio, p.fscale);
+        if (S.fine.size() > (int64_t)1 << 31 || (S.focus && S.cg.size() > (int64_t)1 << 31))
+            throw DeviceError(MMPBSA_ERR_TOO_LARGE, "PB grid above 2^31 nodes");
+        j->out->fine_dims[0] = S.fine.nx;
+        j->out->fine_dims[1] = S.fine.ny;
+        j->out->fine_dims[2] = S.fine.nz;
+        if (S.focus) {
+            j->out->coarse_dims[0] = S.cg.nx;
+            j->out->coarse_dims[1] = S.cg.ny;
+            j->out->coarse_dims[2] = S.cg.nz;
+        }
+        build_cells(w, j->d_xyz, n, S.mn, S.mx,
+                    j->t->max_radius + std::max(p.probe_radius, std::max(p.ion_radius, 0.0)),
+                    S.cells);
+        {
+            DBuf<unsigned long long> acc(w.pool, S.fine.size());
+            S.rho_f = DBuf<double>(w.pool, S.fine.size());
+            launch_spread(S.fine, n, j->d_xyz, j->t->q.p, acc.get(), S.rho_f.get(), w.stream);
+        }
+        if (S.focus) {
+            DBuf<unsigned long long> acc(w.pool, S.cg.size());
+            S.rho_c = DBuf<double>(w.pool, S.cg.size());
+            launch_spread(S.cg, n, j->d_xyz, j->t->q.p, acc.get(), S.rho_c.get(), w.stream);
+        }
+        // pb_energy.rs:375-406: solvated (eps_in, eps_out, ions) and reference (eps_in)
+        S.side[0].ein = p.solute_dielectric;
+        S.side[0].eout = p.solvent_dielectric;
+        S.side[0].ionic = true;
+        S.side[0].info = &j->out->solv;
+        S.side[1].ein = p.solute_dielectric;
+        S.side[1].eout = p.solute_dielectric;
+        S.side[1].ionic = false;
+        S.side[1].info = &j->out->ref;
+    }
+    // coarse focusing solves with Debye-Hueckel boundaries (pb_energy.rs:150-186)
+    std::vector<MgTask *> tasks;
+    for (PbJob *j : act) {
+        PbJob::State &S = *j->st;
+        if (!S.focus) continue;
+        for (auto &sd : S.side) {
+            build_maps(w, *j->t, j->d_xyz, S.cells, S.cg, j->p, sd.ein, sd.eout, sd.ionic, S.kappa,
+                       sd.cm);
+            sd.cphi = DBuf<double>(w.pool, S.cg.size());
+            launch_fill_f64(sd.cphi.get(), 0.0, S.cg.size(), w.stream);
+            DBuf<double> dhw(w.pool, dh_work_size(S.cg, j->t->h.n));
+            launch_dh_boundary(S.cg, j->t->h.n, j->d_xyz, j->t->q.p, sd.ionic ? S.kappa : 0.0,
+                               sd.eout, sd.cphi.get(), dhw.get(), w.stream);
+            sd.ct.g = S.cg;
+            sd.ct.m = &sd.cm;
+            sd.ct.rho = S.rho_c.get();
+            sd.ct.phi = sd.cphi.get();
+            sd.ct.tol = j->p.tolerance * 10.0;
+            sd.ct.max_it = j->p.max_iterations;
+            tasks.push_back(&sd.ct);
+        }
+    }
+    if (!tasks.empty()) mg_solve_batch(w, tasks, batched);
+    // coarse validity (pb_energy.rs:188-194), then fine boundary values
+    {
+        std::vector<DBuf<double>> parts;
+        DBuf<double> res(w.pool, 2 * act.size() + 1);
+        int k = 0;
+        for (PbJob *j : act) {
+            PbJob::State &S = *j->st;
+            for (auto &sd : S.side) {
+                if (S.focus && sd.ct.info.converged) {
+                    parts.emplace_back(w.pool, 1024);
+                    int nbk = 0;
+                    launch_maxabs(sd.cphi.get(), S.cg.size(), parts.back().get(), &nbk, w.stream);
+                    launch_max_partials(parts.back().get(), nbk, res.get() + k, w.stream);
+                }
+                ++k;
+            }
+        }
+        std::vector<double> mx(2 * act.size() + 1, 0.0);
+        HIP_OK(hipMemcpyAsync(mx.data(), res.get(), sizeof(double) * mx.size(),
+                              hipMemcpyDeviceToHost, w.stream));
+        w.sync();
+        k = 0;
+        for (PbJob *j : act) {
+            PbJob::State &S = *j->st;
+            for (auto &sd : S.side) {
+                sd.info->focused = 0;
+                sd.info->coarse_iterations = S.focus ? sd.ct.info.iterations : 0;
+                const bool ok = S.focus && sd.ct.info.converged && std::isfinite(mx[k]) &&
+                                mx[k] < 1e10;
+                sd.phi = DBuf<double>(w.pool, S.fine.size());
+                launch_fill_f64(sd.phi.get(), 0.0, S.fine.size(), w.stream);
+                if (ok) {
+                    launch_interp_boundary(S.fine, S.cg, sd.cphi.get(), sd.phi.get(), w.stream);
+                    sd.info->focused = 1;
+                } else {
+                    DBuf<double> dhw(w.pool, dh_work_size(S.fine, j->t->h.n));
+                    launch_dh_boundary(S.fine, j->t->h.n, j->d_xyz, j->t->q.p,
+                                       sd.ionic ? S.kappa : 0.0, sd.eout, sd.phi.get(), dhw.get(),
+                                       w.stream);
+                }
+                sd.cm = MapSet();
+                sd.cphi.release();
+                ++k;
+            }
+        }
+    }
+    // fine solves
+    tasks.clear();
+    for (PbJob *j : act) {
+        PbJob::State &S = *j->st;
+        for (auto &sd : S.side) {
+            build_maps(w, *j->t, j->d_xyz, S.cells, S.fine, j->p, sd.ein, sd.eout, sd.ionic,
+                       S.kappa, sd.fm);
+            sd.ft.g = S.fine;
+            sd.ft.m = &sd.fm;
+            sd.ft.rho = S.rho_f.get();
+            sd.ft.phi = sd.phi.get();
+            sd.ft.tol = j->p.tolerance;
+            sd.ft.max_it = j->p.max_iterations;
+            tasks.push_back(&sd.ft);
+        }
+    }
+    if (!tasks.empty()) mg_solve_batch(w, tasks, batched);
+    // energies, compute_elec_energy (pb_solver.rs:1174-1186)
+    {
+        const size_t nres = 2 * act.size() + 1;
+        DBuf<double> res(w.pool, nres);
+        std::vector<DBuf<double>> parts;
+        int k = 0;
+        for (PbJob *j : act) {
+            PbJob::State &S = *j->st;
+            for (auto &sd : S.side) {
+                sd.fm = MapSet();
+                parts.emplace_back(w.pool, std::max(1, ceil_div(j->t->h.n, 256)));
+                int nbk = 0;
+                launch_energy_gather(S.fine, sd.phi.get(), j->t->h.n, j->d_xyz, j->t->q.p,
+                                     parts.back().get(), &nbk, w.stream);
+                launch_sum_partials(parts.back().get(), nbk, 1, 1, res.get() + k, w.stream);
+                ++k;
+            }
+        }
+        std::vector<double> e(nres, 0.0);
+        HIP_OK(hipMemcpyAsync(e.data(), res.get(), sizeof(double) * nres, hipMemcpyDeviceToHost,
+                              w.stream));
+        w.sync();
+        k = 0;
+        for (PbJob *j : act) {
+            PbJob::State &S = *j->st;
+            for (auto &sd : S.side) {
+                sd.info->energy = 0.5 * e[k];
+                sd.info->iterations = sd.ft.info.iterations;
+                sd.info->converged = sd.ft.info.converged;
+                sd.info->final_residual = sd.ft.info.final_residual;
+                ++k;
+            }
+            j->out->total = j->out->solv.energy - j->out->ref.energy;
+            j->st.reset();
+        }
+    }
+}
+
+// compute_pb_energy (pb_energy.rs:332-438) for one subsystem, unbatched
+void run_pb(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xyz,
+            const mmpbsa_pb_params &p, mmpbsa_pb_result *out)
+{
+    std::vector<PbJob> jobs(1);
+    jobs[0].t = &t;
+    jobs[0].d_xyz = d_xyz;
+    jobs[0].h_xyz = h_xyz;
+    jobs[0].p = p;
+    jobs[0].out = out;
+    run_pb_jobs(w, jobs, false);
+}
+
+}  // namespace mmpbsa

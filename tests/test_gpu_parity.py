@@ -108,3 +108,19 @@ def test_decomposition(cfg0, oracle):
             worst = max(worst, abs(getattr(g, k) - r[k]))
     print("decomposition worst abs err", worst)
     assert worst < TOL
+
+
+def test_batched_frames_match_single(cfg0):
+    """Each frame's result is the same alone and inside a batch (shared
+    launches, graph-replayed cycles)."""
+    from rust_simulation_tools import synthetic
+    top, _, rec, lig, seed = cfg0
+    base = synthetic.config_system(0)[1]
+    frames = np.stack([synthetic.noisy_frame(base, seed, f) for f in range(3)]) / 10.0
+    pb = rst.PbParams(**PB_CFG0)
+    batch = rst.compute_binding_energy_frames(top, frames, rec, lig, pb_params=pb, batched=True)
+    plain = rst.compute_binding_energy_frames(top, frames, rec, lig, pb_params=pb, batched=False)
+    for fb, fp in zip(batch.frames(), plain.frames()):
+        for k in ("complex_polar", "receptor_polar", "ligand_polar", "delta_total", "delta_mm",
+                  "delta_sa"):
+            assert abs(getattr(fb, k) - getattr(fp, k)) <= 1e-9, k
