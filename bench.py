@@ -4,9 +4,11 @@ Workload: the 25,000-atom synthetic complex (23,500 receptor + 1,500 ligand
 atoms, 45 A sphere), frames = base + N(0, 0.3^2) A noise, PB on a 161^3 grid
 at 0.5 A (eps_in 4, eps_out 80, 0.15 M, two-level focusing), MM and SA for the
 complex, receptor and ligand of every frame.  One step = a batch of frames
-through the batched path (frames overlapped on device streams).  The plain
-path (one frame at a time, host-synchronous) and the FP64 CPU oracle on the
-host cores are timed beside it.
+through the batched path (every PB solve of the batch advances in shared,
+graph-replayed launches).  Beside it: the plain path (one frame at a time,
+host-synchronous, no graphs), the smoother's roofline from HIP events in one
+extra profiled step of the same workload, the 2,500-atom complex's batched /
+plain ratio, and the FP64 CPU oracle on the host cores.
 """
 import argparse
 import json
@@ -20,41 +22,45 @@ for p in (ROOT, PKG):
     if p not in sys.path:
         sys.path.insert(0, p)
 
+import ctypes as C  # noqa: E402
+
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+CFG1_PB = dict(grid_spacing=0.5, solute_dielectric=4.0, solvent_dielectric=80.0,
+               salt_concentration=0.15, fine_dims=(161, 161, 161))
+CFG0_PB = dict(grid_spacing=0.6, solute_dielectric=4.0, solvent_dielectric=80.0,
+               salt_concentration=0.15, fine_dims=(65, 65, 65))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--frames-per-step", type=int, default=4)
+    ap.add_argument("--frames-per-step", type=int, default=8)
     ap.add_argument("--plain-frames", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip roofline and small-system legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(frames shard across GPUs with no exchange beyond the per-frame results)",
+              "(frames would shard across GPUs with only per-frame results exchanged)",
               file=sys.stderr)
         sys.exit(2)
 
     import rust_simulation_tools as rst
     from rust_simulation_tools import _abi, synthetic
     lib = _abi.load_library()
-    rc = lib.mmpbsa_init(None)
-    if rc != 0:
+    if lib.mmpbsa_init(None) != 0:
         raise SystemExit(f"no usable gfx950 device: {lib.mmpbsa_last_error(None)}")
 
     top, base, rec, lig, seed = synthetic.config_system(1)
     B = args.frames_per_step
-    nfr = (args.warmup + args.steps) * B
+    nfr = (args.warmup + args.steps + 1) * B
     frames_nm = np.stack([synthetic.noisy_frame(base, seed, f) for f in range(nfr)]) / 10.0
-    pbp = dict(grid_spacing=0.5, solute_dielectric=4.0, solvent_dielectric=80.0,
-               salt_concentration=0.15, fine_dims=(161, 161, 161))
-    pb = rst.PbParams(**pbp)
+    pb = rst.PbParams(**CFG1_PB)
 
     def step(k, batched=True):
         return rst.compute_binding_energy_frames(top, frames_nm[k * B:(k + 1) * B], rec, lig,
@@ -80,10 +86,38 @@ def main():
             first = r
     sync()
     elapsed = time.perf_counter() - t0
-    frames = args.steps * B
-    value = frames / elapsed
+    value = args.steps * B / elapsed
 
-    # plain path: one frame at a time, host-synchronous
+    roofline = None
+    extras = {}
+    if not args.no_extras:
+        # dominant kernel: finest-grid smoother half-sweeps, timed with HIP events
+        lib.mmpbsa_profile_enable(1)
+        step(args.warmup + args.steps)
+        ms, n, by = C.c_double(), C.c_int64(), C.c_double()
+        lib.mmpbsa_profile_read(C.byref(ms), C.byref(n), C.byref(by))
+        lib.mmpbsa_profile_enable(0)
+        if n.value > 0 and ms.value > 0:
+            achieved = by.value / (ms.value * 1e-3) / 1e9
+            roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "rb_sweep_kernel (finest level, batched over solves)",
+                        "launches": n.value, "avg_launch_us": ms.value * 1e3 / n.value,
+                        "alg_bytes_per_launch": by.value / n.value}
+        # small system: batched vs plain where launches dominate
+        t0s, b0, r0, l0, s0 = synthetic.config_system(0)
+        f0 = np.stack([synthetic.noisy_frame(b0, s0, f) for f in range(16)]) / 10.0
+        p0 = rst.PbParams(**CFG0_PB)
+        rst.compute_binding_energy_frames(t0s, f0[:2], r0, l0, pb_params=p0, batched=True)
+        ts = time.perf_counter()
+        rst.compute_binding_energy_frames(t0s, f0, r0, l0, pb_params=p0, batched=True)
+        fb = 16 / (time.perf_counter() - ts)
+        ts = time.perf_counter()
+        rst.compute_binding_energy_frames(t0s, f0[:4], r0, l0, pb_params=p0, batched=False)
+        fp = 4 / (time.perf_counter() - ts)
+        extras = {"small_complex_2500_atoms": {"batched_frames_per_s": fb, "plain_frames_per_s": fp,
+                                               "batched_over_plain": fb / fp}}
+
     plain_lat = []
     for f in range(args.plain_frames):
         ts = time.perf_counter()
@@ -97,31 +131,35 @@ def main():
         from oracle import oracle as O
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         O.lib().orc_set_threads(threads)
-        k0 = args.warmup
-        x = frames_nm[k0 * B] * 10.0
+        x = frames_nm[args.warmup * B] * 10.0
         ts = time.perf_counter()
-        o = O.frame_energy(top, x, rec, lig, method=1, pb=O.pb_params(**pbp))
+        o = O.frame_energy(top, x, rec, lig, method=1, pb=O.pb_params(**CFG1_PB))
         cpu_s = time.perf_counter() - ts
         g = first.frames()[0]
         dg_err = abs(g.delta_total - o["delta_total"])
         cpu = {"value": 1.0 / cpu_s, "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": "1 frame of the 25,000-atom complex (MM + PB + SA, 3 subsystems)"}
+               "sample": "1 frame of the 25,000-atom complex (MM + PB + SA for complex, "
+                         "receptor, ligand) on the FP64 C oracle, OpenMP"}
 
-    print(json.dumps({
+    out = {
         "metric": "MM-PBSA frames/s (batched) at dG err <= 0.01 kcal/mol",
         "value": value, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32+fp64", "data": "synthetic",
-        "config": {"workload": "25k-atom complex, 161^3 PB grid at 0.5 A, focusing, MM+PB+SA",
+        "dtype": "fp32 smoother + fp64 residual/energies", "data": "synthetic",
+        "config": {"workload": "25,000-atom complex (23,500 + 1,500), 161^3 PB grid at 0.5 A, "
+                               "focusing, MM + PB + SA per subsystem",
                    "frames_per_step": B},
         "plain_frames_per_s": plain_fps,
         "batched_over_plain": (value / plain_fps) if plain_fps else None,
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "of": f"one step of {B} frames"},
         "dG_err_vs_fp64_oracle": dg_err,
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": cpu,
-    }))
+    }
+    out.update(extras)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

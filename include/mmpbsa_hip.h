@@ -218,6 +218,16 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
                      const mmpbsa_sa_params *sa, mmpbsa_residue_contrib *rec_out,
                      mmpbsa_residue_contrib *lig_out);
 
+/* ---- measurement (no counterpart in the reference) ----------------------
+ * While enabled, each launch of the finest-grid smoother half-sweep (the
+ * dominant kernel) is bracketed by HIP events on its own stream and solver
+ * cycles are launched directly rather than replayed from a graph.
+ * mmpbsa_profile_read returns the summed event time, the number of launches
+ * and their algorithmic bytes (14 B per interior node of a variable-
+ * dielectric grid, 6 B for a uniform one: half of a full red-black sweep). */
+int mmpbsa_profile_enable(int32_t on);
+int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes);
+
 #ifdef __cplusplus
 }
 #endif

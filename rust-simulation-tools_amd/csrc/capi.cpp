@@ -586,4 +586,25 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
     });
 }
 
+int mmpbsa_profile_enable(int32_t on)
+{
+    std::lock_guard<std::mutex> lk(profile_mutex());
+    ProfileStats &p = profile_stats();
+    p.enabled = on != 0;
+    p.total_ms = 0.0;
+    p.launches = 0;
+    p.alg_bytes = 0.0;
+    return MMPBSA_OK;
+}
+
+int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes)
+{
+    std::lock_guard<std::mutex> lk(profile_mutex());
+    const ProfileStats &p = profile_stats();
+    if (total_ms) *total_ms = p.total_ms;
+    if (launches) *launches = p.launches;
+    if (alg_bytes) *alg_bytes = p.alg_bytes;
+    return MMPBSA_OK;
+}
+
 }  // extern "C"

@@ -74,4 +74,27 @@ __device__ inline dd wave_sum_dd(dd v)
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// Unsigned 32-bit division by a runtime-constant divisor with a multiply and a
+// shift (Granlund-Montgomery): q = (umulhi(n, m) + n) >> s, exact for every
+// n < 2^32.  Grid index arithmetic uses it instead of integer division.
+struct FastDiv {
+    uint32_t d = 1, m = 0, s = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t div) : d(div)
+    {
+        s = 0;
+        while ((1ull << s) < div) ++s;
+        m = (uint32_t)(((1ull << 32) * ((1ull << s) - div)) / div + 1);
+    }
+    __host__ __device__ inline uint32_t div(uint32_t n) const
+    {
+#ifdef __HIP_DEVICE_COMPILE__
+        const uint32_t hi = __umulhi(n, m);
+#else
+        const uint32_t hi = (uint32_t)(((uint64_t)n * m) >> 32);
+#endif
+        return (uint32_t)(((uint64_t)hi + n) >> s);
+    }
+};
+
 }  // namespace mmpbsa

@@ -8,6 +8,8 @@
 
 #include <cstdint>
 
+#include "common.h"
+
 namespace mmpbsa {
 
 // Solver status word of one solve.
@@ -33,10 +35,19 @@ struct MgLevelDev {
     float ihx, ihy, ihz;
     float omega;
     const float *ex, *ey, *ez, *k2;  // coefficients (unused when uni)
+    // Packed coefficients (finest level): when cw is set, cw[idx] holds 2-bit
+    // codes of the six faces of node idx (x+, x-, y+, y-, z+, z-; 0 = eps_out,
+    // 1 = eps_in, 2 = look up ex/ey/ez) and bit 12 = kappa-bar^2 present, so a
+    // sweep reads one word per node instead of four coefficient arrays.
+    const uint32_t *cw;
+    float c_in, c_out, c_k2;
     float *e;          // correction
     float *f;          // right-hand side
     float *res;        // residual scratch
+    // index helpers: (nx-1)/2 slots per row, ny-2 interior rows, nx, nx*ny
+    FastDiv fd_half, fd_row, fd_nx, fd_nxy;
 };
+void mg_level_index(MgLevelDev &L);  // fills the FastDiv helpers from nx, ny
 
 constexpr int kMaxTail = 8;
 // Levels handled by the one-workgroup tail kernel (coarse end of the V-cycle).
@@ -53,6 +64,8 @@ struct MgOuterDev {
     double ihx, ihy, ihz;
     double scale;                       // 4 pi * 332.0522
     const double *ex, *ey, *ez, *k2;    // FP64 coefficients (unused when uni)
+    const uint32_t *cw;                 // packed codes as in MgLevelDev (or null)
+    double c_in, c_out, c_k2;
     const double *rho;                  // charge density
     const double *phi_in;
     double *phi_out;
@@ -60,6 +73,7 @@ struct MgOuterDev {
     float *f0;                          // FP32 residual for the V-cycle
     double *partials;                   // per-block sums of r^2
     int nbk;                            // number of partials
+    FastDiv fd_nx, fd_nxy;
 };
 
 // Batched launchers.  L / C / F / O / T are device arrays with one entry per
@@ -81,6 +95,11 @@ void mg_outer64_z(const MgOuterDev *O, float *const *ezero, int n_solves, int64_
 void mg_check(const MgOuterDev *O, int n_solves, int count_cycle, MgStatus *st, hipStream_t s);
 void mg_status_init(MgStatus *st, const MgStatus *init_host, int n, hipStream_t s);
 void mg_inject(const MgLevelDev &F, const MgLevelDev &C, hipStream_t s);  // host-side structs
+
+// Pack the finest-level coefficients into MgLevelDev::cw words.
+void mg_pack_codes(int nx, int ny, int nz, const double *ex, const double *ey, const double *ez,
+                   const double *k2, double c_in, double c_out, double c_k2, uint32_t *cw,
+                   hipStream_t s);
 
 // Single-solve FP64 Chebyshev SOR (fallback path).
 int sor64_blocks(const MgOuterDev &O);

@@ -26,6 +26,38 @@ namespace mmpbsa {
 __device__ __forceinline__ bool stopped(const MgStatus *st) { return st->state != 0; }
 
 // ---- node operations shared by the batched kernels and the tail --------------
+template <typename T>
+__device__ __forceinline__ T face_of(uint32_t code, T cin, T cout, const T *__restrict__ a, int64_t i)
+{
+    return code == 0u ? cout : (code == 1u ? cin : a[i]);
+}
+
+// The six face dielectrics and kappa-bar^2 of interior node idx.
+template <typename T, typename V>
+__device__ __forceinline__ void coeffs(const V &L, int64_t idx, T &exp_, T &exm, T &eyp, T &eym,
+                                       T &ezp, T &ezm, T &k2)
+{
+    const int64_t sy = L.nx, sz = (int64_t)L.nx * L.ny;
+    if (L.cw) {
+        const uint32_t w = L.cw[idx];
+        exp_ = face_of<T>(w & 3u, L.c_in, L.c_out, L.ex, idx);
+        exm = face_of<T>((w >> 2) & 3u, L.c_in, L.c_out, L.ex, idx - 1);
+        eyp = face_of<T>((w >> 4) & 3u, L.c_in, L.c_out, L.ey, idx);
+        eym = face_of<T>((w >> 6) & 3u, L.c_in, L.c_out, L.ey, idx - sy);
+        ezp = face_of<T>((w >> 8) & 3u, L.c_in, L.c_out, L.ez, idx);
+        ezm = face_of<T>((w >> 10) & 3u, L.c_in, L.c_out, L.ez, idx - sz);
+        k2 = ((w >> 12) & 1u) ? (T)L.c_k2 : (T)0;
+    } else {
+        exp_ = L.ex[idx];
+        exm = L.ex[idx - 1];
+        eyp = L.ey[idx];
+        eym = L.ey[idx - sy];
+        ezp = L.ez[idx];
+        ezm = L.ez[idx - sz];
+        k2 = L.k2[idx];
+    }
+}
+
 // mode 0: ordinary update; 1: first red half-sweep of a zero correction (no
 // neighbour reads); 2: first black half-sweep (own old value taken as zero).
 __device__ __forceinline__ void rb_node(const MgLevelDev &L, int64_t idx, int mode)
@@ -40,14 +72,13 @@ __device__ __forceinline__ void rb_node(const MgLevelDev &L, int64_t idx, int mo
                       (e[idx + sz] + e[idx - sz]) * L.ihz);
         dg = u * (2.0f * L.ihx + 2.0f * L.ihy + 2.0f * L.ihz);
     } else {
-        const float exp_ = L.ex[idx], exm = L.ex[idx - 1];
-        const float eyp = L.ey[idx], eym = L.ey[idx - sy];
-        const float ezp = L.ez[idx], ezm = L.ez[idx - sz];
+        float exp_, exm, eyp, eym, ezp, ezm, k2;
+        coeffs<float>(L, idx, exp_, exm, eyp, eym, ezp, ezm, k2);
         if (mode != 1)
             nb = fmaf(exp_, e[idx + 1], exm * e[idx - 1]) * L.ihx +
                  fmaf(eyp, e[idx + sy], eym * e[idx - sy]) * L.ihy +
                  fmaf(ezp, e[idx + sz], ezm * e[idx - sz]) * L.ihz;
-        dg = (exp_ + exm) * L.ihx + (eyp + eym) * L.ihy + (ezp + ezm) * L.ihz + L.k2[idx];
+        dg = (exp_ + exm) * L.ihx + (eyp + eym) * L.ihy + (ezp + ezm) * L.ihz + k2;
     }
     const float old = mode == 0 ? e[idx] : 0.0f;
     const float gs = (nb + L.f[idx]) / dg;
@@ -57,16 +88,39 @@ __device__ __forceinline__ void rb_node(const MgLevelDev &L, int64_t idx, int mo
 // work item t of a half-sweep -> node index (or -1)
 __device__ __forceinline__ int64_t rb_index(const MgLevelDev &L, int64_t t, int color)
 {
-    const int half = (L.nx - 1) / 2;
-    const int64_t rows = (int64_t)(L.ny - 2) * (L.nz - 2);
-    if (t >= rows * half) return -1;
-    const int s = (int)(t % half);
-    const int64_t row = t / half;
-    const int iy = 1 + (int)(row % (L.ny - 2));
-    const int iz = 1 + (int)(row / (L.ny - 2));
-    const int ix = 1 + ((1 + iy + iz + color) & 1) + 2 * s;
+    const uint32_t half = L.fd_half.d;
+    const uint32_t rows = (uint32_t)(L.ny - 2) * (uint32_t)(L.nz - 2);
+    if (t >= (int64_t)rows * half) return -1;
+    const uint32_t ut = (uint32_t)t;
+    const uint32_t row = L.fd_half.div(ut);
+    const uint32_t s = ut - row * half;
+    const uint32_t zq = L.fd_row.div(row);
+    const int iy = 1 + (int)(row - zq * (uint32_t)(L.ny - 2));
+    const int iz = 1 + (int)zq;
+    const int ix = 1 + ((1 + iy + iz + color) & 1) + 2 * (int)s;
     if (ix > L.nx - 2) return -1;
-    return ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
+    return (int64_t)ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
+}
+
+// node index -> (ix, iy, iz) with the level's FastDiv helpers
+template <typename V>
+__device__ __forceinline__ void split3(const V &L, int64_t idx, int &ix, int &iy, int &iz)
+{
+    const uint32_t u = (uint32_t)idx;
+    const uint32_t z = L.fd_nxy.div(u);
+    const uint32_t r = u - z * L.fd_nxy.d;
+    const uint32_t y = L.fd_nx.div(r);
+    iz = (int)z;
+    iy = (int)y;
+    ix = (int)(r - y * L.fd_nx.d);
+}
+
+void mg_level_index(MgLevelDev &L)
+{
+    L.fd_half = FastDiv((uint32_t)std::max(1, (L.nx - 1) / 2));
+    L.fd_row = FastDiv((uint32_t)std::max(1, L.ny - 2));
+    L.fd_nx = FastDiv((uint32_t)std::max(1, L.nx));
+    L.fd_nxy = FastDiv((uint32_t)std::max(1, L.nx * L.ny));
 }
 
 __device__ __forceinline__ int64_t rb_work(const MgLevelDev &L)
@@ -79,9 +133,8 @@ __device__ __forceinline__ void residual_node(const MgLevelDev &L, int64_t idx)
 {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
-    const int iz = (int)(idx / sz);
-    const int iy = (int)((idx - (int64_t)iz * sz) / nx);
-    const int ix = (int)(idx - (int64_t)iz * sz - (int64_t)iy * nx);
+    int ix, iy, iz;
+    split3(L, idx, ix, iy, iz);
     if (ix == 0 || iy == 0 || iz == 0 || ix == nx - 1 || iy == ny - 1 || iz == nz - 1) {
         L.res[idx] = 0.0f;
         return;
@@ -94,13 +147,12 @@ __device__ __forceinline__ void residual_node(const MgLevelDev &L, int64_t idx)
                   (e[idx + sz] + e[idx - sz]) * L.ihz);
         dg = u * (2.0f * L.ihx + 2.0f * L.ihy + 2.0f * L.ihz);
     } else {
-        const float exp_ = L.ex[idx], exm = L.ex[idx - 1];
-        const float eyp = L.ey[idx], eym = L.ey[idx - sy];
-        const float ezp = L.ez[idx], ezm = L.ez[idx - sz];
+        float exp_, exm, eyp, eym, ezp, ezm, k2;
+        coeffs<float>(L, idx, exp_, exm, eyp, eym, ezp, ezm, k2);
         nb = fmaf(exp_, e[idx + 1], exm * e[idx - 1]) * L.ihx +
              fmaf(eyp, e[idx + sy], eym * e[idx - sy]) * L.ihy +
              fmaf(ezp, e[idx + sz], ezm * e[idx - sz]) * L.ihz;
-        dg = (exp_ + exm) * L.ihx + (eyp + eym) * L.ihy + (ezp + ezm) * L.ihz + L.k2[idx];
+        dg = (exp_ + exm) * L.ihx + (eyp + eym) * L.ihy + (ezp + ezm) * L.ihz + k2;
     }
     L.res[idx] = L.f[idx] + nb - dg * e[idx];
 }
@@ -108,10 +160,8 @@ __device__ __forceinline__ void residual_node(const MgLevelDev &L, int64_t idx)
 // restrict_into (pb_solver.rs:578-624) for coarse node idx; clears its correction.
 __device__ __forceinline__ void restrict_node(const MgLevelDev &F, const MgLevelDev &C, int64_t idx)
 {
-    const int64_t csz = (int64_t)C.nx * C.ny;
-    const int cz = (int)(idx / csz);
-    const int cy = (int)((idx - (int64_t)cz * csz) / C.nx);
-    const int cx = (int)(idx - (int64_t)cz * csz - (int64_t)cy * C.nx);
+    int cx, cy, cz;
+    split3(C, idx, cx, cy, cz);
     float sum = 0.0f, ws = 0.0f;
     for (int dz = -1; dz <= 1; ++dz) {
         const int z = 2 * cz + dz;
@@ -136,10 +186,8 @@ __device__ __forceinline__ void restrict_node(const MgLevelDev &F, const MgLevel
 // prolongate_add (pb_solver.rs:628-681) for fine node idx
 __device__ __forceinline__ void prolong_node(const MgLevelDev &C, const MgLevelDev &F, int64_t idx)
 {
-    const int64_t fsz = (int64_t)F.nx * F.ny;
-    const int fz = (int)(idx / fsz);
-    const int fy = (int)((idx - (int64_t)fz * fsz) / F.nx);
-    const int fx = (int)(idx - (int64_t)fz * fsz - (int64_t)fy * F.nx);
+    int fx, fy, fz;
+    split3(F, idx, fx, fy, fz);
     const int x0 = min(fx >> 1, max(C.nx - 2, 0));
     const int y0 = min(fy >> 1, max(C.ny - 2, 0));
     const int z0 = min(fz >> 1, max(C.nz - 2, 0));
@@ -308,9 +356,8 @@ __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restri
         const double *__restrict__ p = O.phi_in;
         const double pc = e ? p[idx] + (double)e[idx] : p[idx];
         if (e) O.phi_out[idx] = pc;
-        const int iz = (int)(idx / sz);
-        const int iy = (int)((idx - (int64_t)iz * sz) / nx);
-        const int ix = (int)(idx - (int64_t)iz * sz - (int64_t)iy * nx);
+        int ix, iy, iz;
+        split3(O, idx, ix, iy, iz);
         if (ix == 0 || iy == 0 || iz == 0 || ix == nx - 1 || iy == ny - 1 || iz == nz - 1) {
             O.f0[idx] = 0.0f;
             if (ezero && ezero[s]) ezero[s][idx] = 0.0f;
@@ -336,13 +383,7 @@ __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restri
                 exp_ = exm = eyp = eym = ezp = ezm = O.eps_u;
                 k2 = 0.0;
             } else {
-                exp_ = O.ex[idx];
-                exm = O.ex[idx - 1];
-                eyp = O.ey[idx];
-                eym = O.ey[idx - sy];
-                ezp = O.ez[idx];
-                ezm = O.ez[idx - sz];
-                k2 = O.k2[idx];
+                coeffs<double>(O, idx, exp_, exm, eyp, eym, ezp, ezm, k2);
             }
             const double nb = (exp_ * pxp + exm * pxm) * O.ihx + (eyp * pyp + eym * pym) * O.ihy +
                               (ezp * pzp + ezm * pzm) * O.ihz;
@@ -421,6 +462,39 @@ __global__ void inject_kernel(MgLevelDev F, MgLevelDev C)
     const_cast<float *>(C.k2)[idx] = F.k2[f];
 }
 
+// Packed finest-level coefficient words (layout in MgLevelDev).  Face values
+// equal to eps_in / eps_out get codes 1 / 0 (exact FP64 equality), any other
+// value code 2; kappa-bar^2 is either 0 or c_k2 by construction.
+__global__ void pack_codes_kernel(int nx, int ny, int nz, const double *__restrict__ ex,
+                                  const double *__restrict__ ey, const double *__restrict__ ez,
+                                  const double *__restrict__ k2, double cin, double cout,
+                                  double ck2, uint32_t *__restrict__ cw)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    auto code = [&](const double *a, int64_t i) -> uint32_t {
+        if (i < 0) return 0u;
+        const double v = a[i];
+        return v == cout ? 0u : (v == cin ? 1u : 2u);
+    };
+    uint32_t w = code(ex, idx) | (code(ex, idx - 1) << 2) | (code(ey, idx) << 4) |
+                 (code(ey, idx - sy) << 6) | (code(ez, idx) << 8) | (code(ez, idx - sz) << 10);
+    if (k2[idx] != 0.0) w |= 1u << 12;
+    cw[idx] = w;
+}
+
+void mg_pack_codes(int nx, int ny, int nz, const double *ex, const double *ey, const double *ez,
+                   const double *k2, double c_in, double c_out, double c_k2, uint32_t *cw,
+                   hipStream_t s)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(pack_codes_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, nx, ny,
+                       nz, ex, ey, ez, k2, c_in, c_out, c_k2, cw);
+    LAUNCH_OK();
+}
+
 // ---- FP64 Chebyshev SOR (solve_lpbe_initial, pb_solver.rs:373-447) ----------
 __global__ __launch_bounds__(256) void sor64_kernel(MgOuterDev O, int color, double omega,
                                                     double *__restrict__ partials,
@@ -448,13 +522,7 @@ __global__ __launch_bounds__(256) void sor64_kernel(MgOuterDev O, int color, dou
                 exp_ = exm = eyp = eym = ezp = ezm = O.eps_u;
                 k2 = 0.0;
             } else {
-                exp_ = O.ex[idx];
-                exm = O.ex[idx - 1];
-                eyp = O.ey[idx];
-                eym = O.ey[idx - sy];
-                ezp = O.ez[idx];
-                ezm = O.ez[idx - sz];
-                k2 = O.k2[idx];
+                coeffs<double>(O, idx, exp_, exm, eyp, eym, ezp, ezm, k2);
             }
             const double nb = (exp_ * p[idx + 1] + exm * p[idx - 1]) * O.ihx +
                               (eyp * p[idx + sy] + eym * p[idx - sy]) * O.ihy +

@@ -43,6 +43,8 @@ struct MapSet {
     double eps_u = 1.0;
     DBuf<double> ex, ey, ez, k2;
     DBuf<float> fx, fy, fz, k2f;
+    DBuf<uint32_t> cw;  // packed face codes of the finest level
+    double c_in = 1.0, c_out = 1.0, c_k2 = 0.0;
 };
 
 static double spectral_radius(int nx, int ny, int nz)
@@ -119,6 +121,12 @@ static void build_maps(Worker &w, const DevTopo &t, const double *d_xyz, const D
         launch_fill_f64(m.k2.get(), 0.0, N, w.stream);
         launch_fill_f32(m.k2f.get(), 0.0f, N, w.stream);
     }
+    m.c_in = ein;
+    m.c_out = eout;
+    m.c_k2 = ionic ? p.solvent_dielectric * kappa * kappa : 0.0;
+    m.cw = DBuf<uint32_t>(w.pool, N);
+    mg_pack_codes(g.nx, g.ny, g.nz, m.ex.get(), m.ey.get(), m.ez.get(), m.k2.get(), m.c_in,
+                  m.c_out, m.c_k2, m.cw.get(), w.stream);
 }
 
 static MgOuterDev outer_view(const GridDesc &g, const MapSet &m, const double *rho)
@@ -136,7 +144,13 @@ static MgOuterDev outer_view(const GridDesc &g, const MapSet &m, const double *r
     O.ey = m.ey.get();
     O.ez = m.ez.get();
     O.k2 = m.k2.get();
+    O.cw = m.uni ? nullptr : m.cw.get();
+    O.c_in = m.c_in;
+    O.c_out = m.c_out;
+    O.c_k2 = m.c_k2;
     O.rho = rho;
+    O.fd_nx = FastDiv((uint32_t)std::max(1, g.nx));
+    O.fd_nxy = FastDiv((uint32_t)std::max(1, g.nx * g.ny));
     return O;
 }
 
@@ -339,12 +353,17 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             dv.e = lv.e.get();
             dv.f = lv.f.get();
             dv.res = lv.res.get();
+            mg_level_index(dv);
             if (!m.uni) {
                 if (l == 0) {
                     dv.ex = m.fx.get();
                     dv.ey = m.fy.get();
                     dv.ez = m.fz.get();
                     dv.k2 = m.k2f.get();
+                    dv.cw = m.cw.get();
+                    dv.c_in = (float)m.c_in;
+                    dv.c_out = (float)m.c_out;
+                    dv.c_k2 = (float)m.c_k2;
                 } else {
                     lv.ex = DBuf<float>(w.pool, N);
                     lv.ey = DBuf<float>(w.pool, N);

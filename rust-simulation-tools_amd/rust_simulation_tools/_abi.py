@@ -187,6 +187,8 @@ SIGNATURES = {
                                        C.POINTER(FrameTermsC)]),
     "mmpbsa_decompose": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), C.POINTER(SaParamsC),
                                    C.POINTER(ResidueContribC), C.POINTER(ResidueContribC)]),
+    "mmpbsa_profile_enable": (C.c_int, [C.c_int32]),
+    "mmpbsa_profile_read": (C.c_int, [c_double_p, c_int64_p, c_double_p]),
 }
 
 _lib = None
