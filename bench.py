@@ -43,12 +43,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip roofline and small-system legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy a buffer of this size on the GPU (a plain copy kernel of "
+                         "known bytes, to scale hardware traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(frames would shard across GPUs with only per-frame results exchanged)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration_mib > 0:
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     import rust_simulation_tools as rst
     from rust_simulation_tools import _abi, synthetic
@@ -91,7 +103,7 @@ def main():
     roofline = None
     extras = {}
     if not args.no_extras:
-        # dominant kernel: finest-grid smoother half-sweeps, timed with HIP events
+        # dominant kernel: finest-level smoother passes, timed with HIP events
         lib.mmpbsa_profile_enable(1)
         step(args.warmup + args.steps)
         ms, n, by = C.c_double(), C.c_int64(), C.c_double()
@@ -101,7 +113,8 @@ def main():
             achieved = by.value / (ms.value * 1e-3) / 1e9
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                        "kernel": "rb_sweep_kernel (finest level, batched over solves)",
+                        "kernel": "smooth_kernel (finest level: one red-black sweep per "
+                                  "launch, batched over solves)",
                         "launches": n.value, "avg_launch_us": ms.value * 1e3 / n.value,
                         "alg_bytes_per_launch": by.value / n.value}
         # small system: batched vs plain where launches dominate

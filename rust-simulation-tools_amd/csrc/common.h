@@ -74,6 +74,28 @@ __device__ inline dd wave_sum_dd(dd v)
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// A pointer loaded from a descriptor is generic to the compiler, and generic
+// (flat) loads also count against the LDS counter that every barrier drains,
+// so a prefetch issued before a barrier would be waited for at it.  to_global
+// tells the compiler the data is in device memory (global loads).
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *to_global(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// MI355X dispatches workgroups to its 8 XCDs round-robin (flat id mod 8), and
+// each XCD has its own L2.  xcd_remap turns the flat workgroup id into a work
+// id such that every XCD walks one contiguous range of work ids, so tiles that
+// share halo data run on the same L2.  A bijection on [0, total).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t flat, uint32_t total)
+{
+    const uint32_t q = total / kXcds, r = total % kXcds;
+    const uint32_t x = flat % kXcds, k = flat / kXcds;
+    return x * q + min(x, r) + k;
+}
+
 // Unsigned 32-bit division by a runtime-constant divisor with a multiply and a
 // shift (Granlund-Montgomery): q = (umulhi(n, m) + n) >> s, exact for every
 // n < 2^32.  Grid index arithmetic uses it instead of integer division.

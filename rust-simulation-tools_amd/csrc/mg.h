@@ -35,13 +35,13 @@ struct MgLevelDev {
     float ihx, ihy, ihz;
     float omega;
     const float *ex, *ey, *ez, *k2;  // coefficients (unused when uni)
-    // Packed coefficients (finest level): when cw is set, cw[idx] holds 2-bit
-    // codes of the six faces of node idx (x+, x-, y+, y-, z+, z-; 0 = eps_out,
-    // 1 = eps_in, 2 = look up ex/ey/ez) and bit 12 = kappa-bar^2 present, so a
-    // sweep reads one word per node instead of four coefficient arrays.
-    const uint32_t *cw;
+    // Packed coefficients: when cw is set, cw[idx] is the code byte of node
+    // idx (layout at kCodeSpecial), so a sweep reads one byte per node
+    // instead of four coefficient arrays.
+    const uint8_t *cw;
     float c_in, c_out, c_k2;
-    float *e;          // correction
+    float *e;          // correction (result of post-smoothing; in-place for the tail)
+    float *e2;         // correction after pre-smoothing (batched levels)
     float *f;          // right-hand side
     float *res;        // residual scratch
     // index helpers: (nx-1)/2 slots per row, ny-2 interior rows, nx, nx*ny
@@ -64,7 +64,7 @@ struct MgOuterDev {
     double ihx, ihy, ihz;
     double scale;                       // 4 pi * 332.0522
     const double *ex, *ey, *ez, *k2;    // FP64 coefficients (unused when uni)
-    const uint32_t *cw;                 // packed codes as in MgLevelDev (or null)
+    const uint8_t *cw;                  // code bytes as in MgLevelDev (or null)
     double c_in, c_out, c_k2;
     const double *rho;                  // charge density
     const double *phi_in;
@@ -78,8 +78,15 @@ struct MgOuterDev {
 
 // Batched launchers.  L / C / F / O / T are device arrays with one entry per
 // solve; `max_nodes` is the largest node count among the entries.
-void mg_rb_sweep(const MgLevelDev *L, int n_solves, int64_t max_work, int color, int first,
-                 const MgStatus *st, hipStream_t s);
+// Tiled smoother: one red-black Gauss-Seidel sweep over the level, out of
+// place.  dir 0 reads e2 and writes e, dir 1 reads e and writes e2; first = 1
+// starts from a zero correction (nothing is read) and writes e2.  Each
+// workgroup owns an (x, y) tile and a range of zc planes, streams it along z
+// through an LDS ring and recomputes a halo, so the result equals that of two
+// in-place half-sweeps over the whole level.
+int smooth_tiles(int nx, int ny);  // (x, y) tiles of one plane
+void mg_smooth(const MgLevelDev *L, int n_solves, int first, int dir, int zc, int max_blocks,
+               const MgStatus *st, hipStream_t s);
 void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
                    hipStream_t s);
 void mg_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_coarse,
@@ -89,17 +96,18 @@ void mg_prolong(const MgLevelDev *C, const MgLevelDev *F, int n_solves, int64_t 
 void mg_tail(const MgTailDev *T, int n_solves, const MgStatus *st, hipStream_t s);
 void mg_outer64(const MgOuterDev *O, int n_solves, int64_t max_nodes, const MgStatus *st,
                 hipStream_t s);
-// as mg_outer64, also clearing the boundary nodes of ezero[solve] (may be null)
-void mg_outer64_z(const MgOuterDev *O, float *const *ezero, int n_solves, int64_t max_nodes,
-                  const MgStatus *st, hipStream_t s);
 void mg_check(const MgOuterDev *O, int n_solves, int count_cycle, MgStatus *st, hipStream_t s);
 void mg_status_init(MgStatus *st, const MgStatus *init_host, int n, hipStream_t s);
 void mg_inject(const MgLevelDev &F, const MgLevelDev &C, hipStream_t s);  // host-side structs
 
-// Pack the finest-level coefficients into MgLevelDev::cw words.
+// Code byte of a node: bits 0-5 = face x+, x-, y+, y-, z+, z- has the solute
+// dielectric (else the solvent one), bit 6 = kappa-bar^2 is c_k2 (else 0),
+// bit 7 = some face has neither value: read all six from ex / ey / ez.
+constexpr uint32_t kCodeKappa = 0x40u, kCodeSpecial = 0x80u;
 void mg_pack_codes(int nx, int ny, int nz, const double *ex, const double *ey, const double *ez,
-                   const double *k2, double c_in, double c_out, double c_k2, uint32_t *cw,
-                   hipStream_t s);
+                   const double *k2, double c_in, double c_out, uint8_t *cw, hipStream_t s);
+void mg_pack_codes(int nx, int ny, int nz, const float *ex, const float *ey, const float *ez,
+                   const float *k2, float c_in, float c_out, uint8_t *cw, hipStream_t s);
 
 // Single-solve FP64 Chebyshev SOR (fallback path).
 int sor64_blocks(const MgOuterDev &O);

@@ -27,9 +27,9 @@ __device__ __forceinline__ bool stopped(const MgStatus *st) { return st->state !
 
 // ---- node operations shared by the batched kernels and the tail --------------
 template <typename T>
-__device__ __forceinline__ T face_of(uint32_t code, T cin, T cout, const T *__restrict__ a, int64_t i)
+__device__ __forceinline__ T face_bit(uint32_t w, int bit, T cin, T cout)
 {
-    return code == 0u ? cout : (code == 1u ? cin : a[i]);
+    return (w >> bit) & 1u ? cin : cout;
 }
 
 // The six face dielectrics and kappa-bar^2 of interior node idx.
@@ -40,47 +40,47 @@ __device__ __forceinline__ void coeffs(const V &L, int64_t idx, T &exp_, T &exm,
     const int64_t sy = L.nx, sz = (int64_t)L.nx * L.ny;
     if (L.cw) {
         const uint32_t w = L.cw[idx];
-        exp_ = face_of<T>(w & 3u, L.c_in, L.c_out, L.ex, idx);
-        exm = face_of<T>((w >> 2) & 3u, L.c_in, L.c_out, L.ex, idx - 1);
-        eyp = face_of<T>((w >> 4) & 3u, L.c_in, L.c_out, L.ey, idx);
-        eym = face_of<T>((w >> 6) & 3u, L.c_in, L.c_out, L.ey, idx - sy);
-        ezp = face_of<T>((w >> 8) & 3u, L.c_in, L.c_out, L.ez, idx);
-        ezm = face_of<T>((w >> 10) & 3u, L.c_in, L.c_out, L.ez, idx - sz);
-        k2 = ((w >> 12) & 1u) ? (T)L.c_k2 : (T)0;
+        k2 = (w & kCodeKappa) ? (T)L.c_k2 : (T)0;
+        if (!(w & kCodeSpecial)) {
+            exp_ = face_bit<T>(w, 0, L.c_in, L.c_out);
+            exm = face_bit<T>(w, 1, L.c_in, L.c_out);
+            eyp = face_bit<T>(w, 2, L.c_in, L.c_out);
+            eym = face_bit<T>(w, 3, L.c_in, L.c_out);
+            ezp = face_bit<T>(w, 4, L.c_in, L.c_out);
+            ezm = face_bit<T>(w, 5, L.c_in, L.c_out);
+            return;
+        }
     } else {
-        exp_ = L.ex[idx];
-        exm = L.ex[idx - 1];
-        eyp = L.ey[idx];
-        eym = L.ey[idx - sy];
-        ezp = L.ez[idx];
-        ezm = L.ez[idx - sz];
         k2 = L.k2[idx];
     }
+    exp_ = L.ex[idx];
+    exm = L.ex[idx - 1];
+    eyp = L.ey[idx];
+    eym = L.ey[idx - sy];
+    ezp = L.ez[idx];
+    ezm = L.ez[idx - sz];
 }
 
-// mode 0: ordinary update; 1: first red half-sweep of a zero correction (no
-// neighbour reads); 2: first black half-sweep (own old value taken as zero).
-__device__ __forceinline__ void rb_node(const MgLevelDev &L, int64_t idx, int mode)
+// In-place Gauss-Seidel / SOR update of interior node idx (the tail's smoother).
+__device__ __forceinline__ void rb_node(const MgLevelDev &L, int64_t idx)
 {
     const int64_t sy = L.nx, sz = (int64_t)L.nx * L.ny;
     float *__restrict__ e = L.e;
-    float nb = 0.0f, dg;
+    float nb, dg;
     if (L.uni) {
         const float u = L.eps_u;
-        if (mode != 1)
-            nb = u * ((e[idx + 1] + e[idx - 1]) * L.ihx + (e[idx + sy] + e[idx - sy]) * L.ihy +
-                      (e[idx + sz] + e[idx - sz]) * L.ihz);
+        nb = u * ((e[idx + 1] + e[idx - 1]) * L.ihx + (e[idx + sy] + e[idx - sy]) * L.ihy +
+                  (e[idx + sz] + e[idx - sz]) * L.ihz);
         dg = u * (2.0f * L.ihx + 2.0f * L.ihy + 2.0f * L.ihz);
     } else {
         float exp_, exm, eyp, eym, ezp, ezm, k2;
         coeffs<float>(L, idx, exp_, exm, eyp, eym, ezp, ezm, k2);
-        if (mode != 1)
-            nb = fmaf(exp_, e[idx + 1], exm * e[idx - 1]) * L.ihx +
-                 fmaf(eyp, e[idx + sy], eym * e[idx - sy]) * L.ihy +
-                 fmaf(ezp, e[idx + sz], ezm * e[idx - sz]) * L.ihz;
+        nb = fmaf(exp_, e[idx + 1], exm * e[idx - 1]) * L.ihx +
+             fmaf(eyp, e[idx + sy], eym * e[idx - sy]) * L.ihy +
+             fmaf(ezp, e[idx + sz], ezm * e[idx - sz]) * L.ihz;
         dg = (exp_ + exm) * L.ihx + (eyp + eym) * L.ihy + (ezp + ezm) * L.ihz + k2;
     }
-    const float old = mode == 0 ? e[idx] : 0.0f;
+    const float old = e[idx];
     const float gs = (nb + L.f[idx]) / dg;
     e[idx] = fmaf(L.omega, gs - old, old);
 }
@@ -157,8 +157,10 @@ __device__ __forceinline__ void residual_node(const MgLevelDev &L, int64_t idx)
     L.res[idx] = L.f[idx] + nb - dg * e[idx];
 }
 
-// restrict_into (pb_solver.rs:578-624) for coarse node idx; clears its correction.
-__device__ __forceinline__ void restrict_node(const MgLevelDev &F, const MgLevelDev &C, int64_t idx)
+// restrict_into (pb_solver.rs:578-624) for coarse node idx; with clear, also
+// zeroes its correction (the tail smooths in place from zero).
+__device__ __forceinline__ void restrict_node(const MgLevelDev &F, const MgLevelDev &C, int64_t idx,
+                                              bool clear)
 {
     int cx, cy, cz;
     split3(C, idx, cx, cy, cz);
@@ -180,7 +182,7 @@ __device__ __forceinline__ void restrict_node(const MgLevelDev &F, const MgLevel
         }
     }
     C.f[idx] = 0.25f * sum / ws;
-    C.e[idx] = 0.0f;
+    if (clear) C.e[idx] = 0.0f;
 }
 
 // prolongate_add (pb_solver.rs:628-681) for fine node idx
@@ -206,17 +208,251 @@ __device__ __forceinline__ void prolong_node(const MgLevelDev &C, const MgLevelD
 __device__ __forceinline__ int64_t nodes(const MgLevelDev &L) { return (int64_t)L.nx * L.ny * L.nz; }
 
 // ---- batched kernels (blockIdx.y = solve) -----------------------------------
-__global__ __launch_bounds__(256) void rb_sweep_kernel(const MgLevelDev *__restrict__ Ls, int color,
-                                                       int first, const MgStatus *__restrict__ st)
+// ---- tiled z-streaming smoother -------------------------------------------------
+// One pass = one red-black sweep over a level, out of place.  Half-sweep k
+// (k = 1 red, k = 2 black) of step t updates plane t - k + 1, so within a step
+// both half-sweeps touch the nodes with x + y + t even, each reads only values
+// the reference order has already produced, and plane z is final after step
+// z + 1.  The red half-sweep runs on the region shrunk by one node, which is
+// exact on the tile core widened by one; the black one is then exact on the
+// core.  A thread owns column pairs (x, x+1) of the region; in every step
+// exactly one column of each pair is active, alternating with t.
+namespace sm {
+constexpr int K = 2;                  // half-sweeps per step
+constexpr int H = 2;                  // halo
+constexpr int TILE = 28;              // core edge
+constexpr int RW = TILE + 2 * H;      // region row: 32 words
+constexpr int RN = RW * RW;
+constexpr int RING = 4;               // planes t-2 .. t+1
+constexpr int PF = RN / 256;          // plane values per thread
+constexpr int NP = RN / 2;            // column pairs
+constexpr int PP = NP / 256;          // column pairs per thread
+constexpr int WB = (TILE * TILE + 255) / 256;  // core values written back per thread
+static_assert(RN % 256 == 0 && NP % 256 == 0, "region must fill the workgroup");
+}  // namespace sm
+
+int smooth_tiles(int nx, int ny) { return ceil_div(nx, sm::TILE) * ceil_div(ny, sm::TILE); }
+
+// face value: bit set -> cin, clear -> cout (a bitfield select on the bits)
+__device__ __forceinline__ float face_sel(uint32_t w, int bit, uint32_t cin, uint32_t cout)
 {
-    const int s = blockIdx.y;
+    const uint32_t m = (uint32_t)((int32_t)(w << (31 - bit)) >> 31);
+    return __uint_as_float((m & cin) | (~m & cout));
+}
+
+// Every global load of the kernel is unconditional (lanes without work load
+// from a harmless address and discard the value): loads under control flow
+// make the compiler copy their results at the join, and the wait for those
+// copies would also wait for the prefetch issued before the barrier.
+template <bool FIRST>
+__global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restrict__ Ls, int dir,
+                                                     int zc, const MgStatus *__restrict__ st)
+{
+    using namespace sm;
+    __shared__ float ring[RING * RN];
+    const uint32_t flat = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int s = (int)(flat / gridDim.x);
+    const int b = (int)(flat - (uint32_t)s * gridDim.x);
     if (stopped(st + s)) return;
     const MgLevelDev L = Ls[s];
     if (L.nx == 0) return;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t idx = rb_index(L, t, color);
-    if (idx < 0) return;
-    rb_node(L, idx, first ? (color == 0 ? 1 : 2) : 0);
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int ntx = (nx + TILE - 1) / TILE, nty = (ny + TILE - 1) / TILE;
+    const int nch = (nz + zc - 1) / zc;
+    if (b >= ntx * nty * nch) return;
+    const int tile = b % (ntx * nty), ch = b / (ntx * nty);
+    const int cx = (nx + ntx - 1) / ntx, cy = (ny + nty - 1) / nty;
+    const int x0 = (tile % ntx) * cx, x1 = min(nx, x0 + cx);
+    const int y0 = (tile / ntx) * cy, y1 = min(ny, y0 + cy);
+    const int z0 = ch * zc, z1 = min(nz, z0 + zc);
+    const int xl = x0 - H, yl = y0 - H;
+    // element offsets fit 32 bits (mg_smooth checks), so every load is a
+    // 32-bit vector offset from a scalar base
+    const uint32_t sxy = (uint32_t)nx * (uint32_t)ny;
+    const bool uni = L.uni != 0;
+    // non-uniform levels always carry code bytes; uniform ones read no
+    // coefficients, and their dummy loads go to f
+    const auto src = to_global<const float>(dir ? L.e : L.e2);
+    const auto dst = to_global<float>((FIRST || dir) ? L.e2 : L.e);
+    const auto fg = to_global<const float>(L.f);
+    const auto cwg = to_global<const uint8_t>(uni ? (const uint8_t *)L.f : L.cw);
+    const auto exg = to_global<const float>(uni ? L.f : L.ex);
+    const auto eyg = to_global<const float>(uni ? L.f : L.ey);
+    const auto ezg = to_global<const float>(uni ? L.f : L.ez);
+    const float ih = L.ihx;                // the grids are cubic: ihx == ihy == ihz
+    const float omega = L.omega, ck2 = L.c_k2;
+    const uint32_t cin = __float_as_uint(L.c_in), cout = __float_as_uint(L.c_out);
+    const float uih = L.eps_u * ih, udg_inv = 1.0f / (6.0f * L.eps_u * ih);
+    const int tid = threadIdx.x;
+    const int t0 = z0 - K + 1;
+    const int t1 = z1 + K - 1;  // last step: writes plane z1 - 1
+
+    // per column pair and step parity q = (t - t0) & 1: region index and
+    // in-plane offset of the active node, and whether it is interior to both
+    // the grid and the shrunk region
+    int ri[2][PP], go[2][PP];
+    bool ok[2][PP];
+#pragma unroll
+    for (int m = 0; m < PP; ++m) {
+        const int P = tid + 256 * m;
+        const int ry = P / (RW / 2);
+        const int y = yl + ry;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int rx = 2 * (P - ry * (RW / 2)) + ((y + t0 + q - xl) & 1);
+            const int x = xl + rx;
+            ri[q][m] = ry * RW + rx;
+            go[q][m] = x + nx * y;
+            ok[q][m] = ry >= 1 && ry <= RW - 2 && rx >= 1 && rx <= RW - 2 && x >= 1 &&
+                       x <= nx - 2 && y >= 1 && y <= ny - 2;
+        }
+    }
+    // plane loads: region value n = tid + 256 j sits in region row
+    // (tid >> 5) + 8 j, column tid & 31
+    static_assert(RW == 32, "plane load layout assumes 32-word rows");
+    const int lx = xl + (tid & 31), ly = yl + (tid >> 5);
+    const bool lxok = lx >= 0 && lx < nx;
+    const int lgo0 = lx + nx * ly;
+    // write-back: core value n = tid + 256 j at core row n / TILE
+    const int wx = x1 - x0, wy = y1 - y0;
+
+    auto slot = [&](int z) { return ring + ((z + 2 * RING) & (RING - 1)) * RN; };
+    float pf[PF];
+    auto load_plane = [&](int z) {
+        if (FIRST) {
+#pragma unroll
+            for (int j = 0; j < PF; ++j) pf[j] = 0.0f;
+            return;
+        }
+        const bool zin = z >= 0 && z < nz;
+        const uint32_t zo = sxy * (uint32_t)(zin ? z : 0);
+#pragma unroll
+        for (int j = 0; j < PF; ++j) {
+            const int y = ly + 8 * j;
+            const bool v = zin && lxok && y >= 0 && y < ny;
+            const float x = src[v ? zo + (uint32_t)(lgo0 + 8 * j * nx) : 0u];
+            pf[j] = v ? x : 0.0f;
+        }
+    };
+    auto store_plane = [&](int z) {
+        float *p = slot(z);
+#pragma unroll
+        for (int j = 0; j < PF; ++j) p[tid + 256 * j] = pf[j];
+    };
+    auto write_back = [&](int z) {
+        if (z < z0 || z >= z1) return;
+        const float *p = slot(z);
+        const uint32_t zo = sxy * (uint32_t)z;
+#pragma unroll
+        for (int j = 0; j < WB; ++j) {
+            const int n = tid + 256 * j;
+            const int r = n / TILE, c = n - r * TILE;
+            if (r < wy && c < wx)
+                dst[zo + (uint32_t)((x0 + c) + nx * (y0 + r))] = p[(y0 + r - yl) * RW + (x0 + c - xl)];
+        }
+    };
+    auto zok = [&](int k, int z) {
+        return z >= 1 && z <= nz - 2 && z >= z0 - (K - k) && z <= z1 - 1 + (K - k);
+    };
+    // right-hand side and code bytes of both half-sweeps of step t (parity q)
+    auto prefetch = [&](int t, int q, float (&F)[K][PP], uint32_t (&W)[K][PP]) {
+#pragma unroll
+        for (int k = 1; k <= K; ++k) {
+            const int z = t - k + 1;
+            const bool zk = zok(k, z);
+            const uint32_t zo = sxy * (uint32_t)(zk ? z : 0);
+#pragma unroll
+            for (int m = 0; m < PP; ++m) {
+                const uint32_t g = (zk && ok[q][m]) ? zo + (uint32_t)go[q][m] : 0u;
+                F[k - 1][m] = fg[g];
+                W[k - 1][m] = cwg[g];
+            }
+        }
+    };
+    auto stage = [&](int k, int t, int q, const float (&F)[K][PP], const uint32_t (&W)[K][PP]) {
+        const int z = t - k + 1;
+        if (!zok(k, z)) return;
+        const float *__restrict__ pm = slot(z - 1);
+        float *__restrict__ pc = slot(z);
+        const float *__restrict__ pp = slot(z + 1);
+#pragma unroll
+        for (int m = 0; m < PP; ++m) {
+            if (!ok[q][m]) continue;
+            const int i = ri[q][m];
+            const float exx = pc[i + 1], emx = pc[i - 1], eyy = pc[i + RW], emy = pc[i - RW];
+            const float ezz = pp[i], emz = pm[i], old = pc[i];
+            float gs;
+            if (uni) {
+                const float sum = ((exx + emx) + (eyy + emy)) + (ezz + emz);
+                gs = fmaf(sum, uih, F[k - 1][m]) * udg_inv;
+            } else {
+                const uint32_t w = W[k - 1][m];
+                float a0, a1, a2, a3, a4, a5;
+                if (!(w & kCodeSpecial)) {
+                    a0 = face_sel(w, 0, cin, cout);
+                    a1 = face_sel(w, 1, cin, cout);
+                    a2 = face_sel(w, 2, cin, cout);
+                    a3 = face_sel(w, 3, cin, cout);
+                    a4 = face_sel(w, 4, cin, cout);
+                    a5 = face_sel(w, 5, cin, cout);
+                } else {  // rare: faces cut by the dielectric boundary
+                    const uint32_t g = sxy * (uint32_t)z + (uint32_t)go[q][m];
+                    a0 = exg[g];
+                    a1 = exg[g - 1];
+                    a2 = eyg[g];
+                    a3 = eyg[g - nx];
+                    a4 = ezg[g];
+                    a5 = ezg[g - sxy];
+                }
+                const float k2 = (w & kCodeKappa) ? ck2 : 0.0f;
+                const float sum =
+                    fmaf(a0, exx, fmaf(a1, emx, fmaf(a2, eyy, fmaf(a3, emy, fmaf(a4, ezz, a5 * emz)))));
+                const float dg = fmaf(((a0 + a1) + (a2 + a3)) + (a4 + a5), ih, k2);
+                gs = fmaf(sum, ih, F[k - 1][m]) * __builtin_amdgcn_rcpf(dg);
+            }
+            pc[i] = fmaf(omega, gs - old, old);
+        }
+    };
+    // one step: stage plane t+1, write plane t-2 back, start the loads of
+    // plane t+2 and of the next step's coefficients, then the two half-sweeps
+    auto step = [&](int t, int q, const float (&F)[K][PP], const uint32_t (&W)[K][PP],
+                    float (&Fn)[K][PP], uint32_t (&Wn)[K][PP]) {
+        store_plane(t + 1);
+        write_back(t - K);
+        load_plane(t + 2);
+        prefetch(t + 1, q ^ 1, Fn, Wn);
+        __syncthreads();
+#pragma unroll
+        for (int k = 1; k <= K; ++k) {
+            stage(k, t, q, F, W);
+            __syncthreads();
+        }
+    };
+    float fa[K][PP], fb[K][PP];
+    uint32_t wa[K][PP], wb[K][PP];
+    load_plane(t0 - 1);
+    store_plane(t0 - 1);
+    load_plane(t0);
+    store_plane(t0);
+    load_plane(t0 + 1);
+    prefetch(t0, 0, fa, wa);
+    for (int t = t0; t <= t1; t += 2) {
+        step(t, 0, fa, wa, fb, wb);
+        if (t + 1 <= t1) step(t + 1, 1, fb, wb, fa, wa);
+    }
+}
+
+void mg_smooth(const MgLevelDev *L, int n_solves, int first, int dir, int zc, int max_blocks,
+               const MgStatus *st, hipStream_t s)
+{
+    if (n_solves <= 0 || max_blocks <= 0) return;
+    if (first)
+        hipLaunchKernelGGL(smooth_kernel<true>, dim3(max_blocks, n_solves), dim3(256), 0, s, L,
+                           dir, zc, st);
+    else
+        hipLaunchKernelGGL(smooth_kernel<false>, dim3(max_blocks, n_solves), dim3(256), 0, s, L,
+                           dir, zc, st);
+    LAUNCH_OK();
 }
 
 __global__ __launch_bounds__(256) void residual32_kernel(const MgLevelDev *__restrict__ Ls,
@@ -224,10 +460,11 @@ __global__ __launch_bounds__(256) void residual32_kernel(const MgLevelDev *__res
 {
     const int s = blockIdx.y;
     if (stopped(st + s)) return;
-    const MgLevelDev L = Ls[s];
+    MgLevelDev L = Ls[s];
     if (L.nx == 0) return;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nodes(L)) return;
+    L.e = L.e2;  // the pre-smoothed correction
     residual_node(L, idx);
 }
 
@@ -241,7 +478,9 @@ __global__ __launch_bounds__(256) void restrict_kernel(const MgLevelDev *__restr
     if (C.nx == 0) return;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nodes(C)) return;
-    restrict_node(Fs[s], C, idx);
+    // batched levels start smoothing from zero without reading e, and the
+    // tail clears its entry level itself
+    restrict_node(Fs[s], C, idx, false);
 }
 
 __global__ __launch_bounds__(256) void prolong_kernel(const MgLevelDev *__restrict__ Cs,
@@ -250,10 +489,11 @@ __global__ __launch_bounds__(256) void prolong_kernel(const MgLevelDev *__restri
 {
     const int s = blockIdx.y;
     if (stopped(st + s)) return;
-    const MgLevelDev F = Fs[s];
+    MgLevelDev F = Fs[s];
     if (F.nx == 0) return;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nodes(F)) return;
+    F.e = F.e2;  // added to the pre-smoothed correction
     prolong_node(Cs[s], F, idx);
 }
 
@@ -305,7 +545,7 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
             for (int color = 0; color < 2; ++color) {
                 for (int64_t t = tid; t < w; t += nt) {
                     const int64_t idx = rb_index(L, t, color);
-                    if (idx >= 0) rb_node(L, idx, 0);
+                    if (idx >= 0) rb_node(L, idx);
                 }
                 __syncthreads();
             }
@@ -321,7 +561,7 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
         sweeps(L, 3);
         for (int64_t i = tid; i < nodes(L); i += nt) residual_node(L, i);
         __syncthreads();
-        for (int64_t i = tid; i < nodes(C); i += nt) restrict_node(L, C, i);
+        for (int64_t i = tid; i < nodes(C); i += nt) restrict_node(L, C, i, true);
         __syncthreads();
     }
     sweeps(lv[n - 1], 50);
@@ -335,10 +575,8 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
 }
 
 // ---- FP64 outer step: phi_out = phi_in + e, f0 = b - A phi_out (stencil_at,
-// pb_solver.rs:159-194), block partial sums of r^2; clears the boundary of the
-// next correction buffer (ezero) for the next V-cycle.
+// pb_solver.rs:159-194), block partial sums of r^2.
 __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restrict__ Os,
-                                                      float *const *__restrict__ ezero,
                                                       const MgStatus *__restrict__ st)
 {
     __shared__ double red[4];
@@ -360,7 +598,6 @@ __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restri
         split3(O, idx, ix, iy, iz);
         if (ix == 0 || iy == 0 || iz == 0 || ix == nx - 1 || iy == ny - 1 || iz == nz - 1) {
             O.f0[idx] = 0.0f;
-            if (ezero && ezero[s]) ezero[s][idx] = 0.0f;
         } else {
             double pxp, pxm, pyp, pym, pzp, pzm;
             if (e) {
@@ -462,37 +699,57 @@ __global__ void inject_kernel(MgLevelDev F, MgLevelDev C)
     const_cast<float *>(C.k2)[idx] = F.k2[f];
 }
 
-// Packed finest-level coefficient words (layout in MgLevelDev).  Face values
-// equal to eps_in / eps_out get codes 1 / 0 (exact FP64 equality), any other
-// value code 2; kappa-bar^2 is either 0 or c_k2 by construction.
-__global__ void pack_codes_kernel(int nx, int ny, int nz, const double *__restrict__ ex,
-                                  const double *__restrict__ ey, const double *__restrict__ ez,
-                                  const double *__restrict__ k2, double cin, double cout,
-                                  double ck2, uint32_t *__restrict__ cw)
+// Code bytes (layout at kCodeSpecial).  Face values equal to eps_in / eps_out
+// (exact equality) set / clear their bit; any other value marks the node
+// special; kappa-bar^2 is either 0 or c_k2 by construction.
+template <typename T>
+__global__ void pack_codes_kernel(int nx, int ny, int nz, const T *__restrict__ ex,
+                                  const T *__restrict__ ey, const T *__restrict__ ez,
+                                  const T *__restrict__ k2, T cin, T cout, uint8_t *__restrict__ cw)
 {
     const int64_t N = (int64_t)nx * ny * nz;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= N) return;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
-    auto code = [&](const double *a, int64_t i) -> uint32_t {
-        if (i < 0) return 0u;
-        const double v = a[i];
-        return v == cout ? 0u : (v == cin ? 1u : 2u);
+    uint32_t w = 0u;
+    auto face = [&](const T *a, int64_t i, int bit) {
+        if (i < 0) return;
+        const T v = a[i];
+        if (v == cin)
+            w |= 1u << bit;
+        else if (v != cout)
+            w |= kCodeSpecial;
     };
-    uint32_t w = code(ex, idx) | (code(ex, idx - 1) << 2) | (code(ey, idx) << 4) |
-                 (code(ey, idx - sy) << 6) | (code(ez, idx) << 8) | (code(ez, idx - sz) << 10);
-    if (k2[idx] != 0.0) w |= 1u << 12;
-    cw[idx] = w;
+    face(ex, idx, 0);
+    face(ex, idx - 1, 1);
+    face(ey, idx, 2);
+    face(ey, idx - sy, 3);
+    face(ez, idx, 4);
+    face(ez, idx - sz, 5);
+    if (k2[idx] != (T)0) w |= kCodeKappa;
+    cw[idx] = (uint8_t)w;
+}
+
+template <typename T>
+static void pack_codes(int nx, int ny, int nz, const T *ex, const T *ey, const T *ez, const T *k2,
+                       T c_in, T c_out, uint8_t *cw, hipStream_t s)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(pack_codes_kernel<T>, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, nx,
+                       ny, nz, ex, ey, ez, k2, c_in, c_out, cw);
+    LAUNCH_OK();
 }
 
 void mg_pack_codes(int nx, int ny, int nz, const double *ex, const double *ey, const double *ez,
-                   const double *k2, double c_in, double c_out, double c_k2, uint32_t *cw,
-                   hipStream_t s)
+                   const double *k2, double c_in, double c_out, uint8_t *cw, hipStream_t s)
 {
-    const int64_t N = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(pack_codes_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, s, nx, ny,
-                       nz, ex, ey, ez, k2, c_in, c_out, c_k2, cw);
-    LAUNCH_OK();
+    pack_codes<double>(nx, ny, nz, ex, ey, ez, k2, c_in, c_out, cw, s);
+}
+
+void mg_pack_codes(int nx, int ny, int nz, const float *ex, const float *ey, const float *ez,
+                   const float *k2, float c_in, float c_out, uint8_t *cw, hipStream_t s)
+{
+    pack_codes<float>(nx, ny, nz, ex, ey, ez, k2, c_in, c_out, cw, s);
 }
 
 // ---- FP64 Chebyshev SOR (solve_lpbe_initial, pb_solver.rs:373-447) ----------
@@ -570,15 +827,6 @@ __global__ __launch_bounds__(256) void sor_check_kernel(const double *__restrict
 // ---- launchers ---------------------------------------------------------------
 static inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_div(n, 256)); }
 
-void mg_rb_sweep(const MgLevelDev *L, int n_solves, int64_t max_work, int color, int first,
-                 const MgStatus *st, hipStream_t s)
-{
-    if (max_work <= 0 || n_solves <= 0) return;
-    hipLaunchKernelGGL(rb_sweep_kernel, dim3(nblk(max_work), n_solves), dim3(256), 0, s, L, color,
-                       first, st);
-    LAUNCH_OK();
-}
-
 void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
                    hipStream_t s)
 {
@@ -613,15 +861,8 @@ void mg_tail(const MgTailDev *T, int n_solves, const MgStatus *st, hipStream_t s
 void mg_outer64(const MgOuterDev *O, int n_solves, int64_t max_nodes, const MgStatus *st,
                 hipStream_t s)
 {
-    mg_outer64_z(O, nullptr, n_solves, max_nodes, st, s);
-}
-
-void mg_outer64_z(const MgOuterDev *O, float *const *ezero, int n_solves, int64_t max_nodes,
-                  const MgStatus *st, hipStream_t s)
-{
     if (n_solves <= 0) return;
-    hipLaunchKernelGGL(outer64_kernel, dim3(nblk(max_nodes), n_solves), dim3(256), 0, s, O, ezero,
-                       st);
+    hipLaunchKernelGGL(outer64_kernel, dim3(nblk(max_nodes), n_solves), dim3(256), 0, s, O, st);
     LAUNCH_OK();
 }
 

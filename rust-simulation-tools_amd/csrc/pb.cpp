@@ -43,7 +43,7 @@ struct MapSet {
     double eps_u = 1.0;
     DBuf<double> ex, ey, ez, k2;
     DBuf<float> fx, fy, fz, k2f;
-    DBuf<uint32_t> cw;  // packed face codes of the finest level
+    DBuf<uint8_t> cw;   // code bytes of the finest level
     double c_in = 1.0, c_out = 1.0, c_k2 = 0.0;
 };
 
@@ -124,9 +124,9 @@ static void build_maps(Worker &w, const DevTopo &t, const double *d_xyz, const D
     m.c_in = ein;
     m.c_out = eout;
     m.c_k2 = ionic ? p.solvent_dielectric * kappa * kappa : 0.0;
-    m.cw = DBuf<uint32_t>(w.pool, N);
+    m.cw = DBuf<uint8_t>(w.pool, N);
     mg_pack_codes(g.nx, g.ny, g.nz, m.ex.get(), m.ey.get(), m.ez.get(), m.k2.get(), m.c_in,
-                  m.c_out, m.c_k2, m.cw.get(), w.stream);
+                  m.c_out, m.cw.get(), w.stream);
 }
 
 static MgOuterDev outer_view(const GridDesc &g, const MapSet &m, const double *rho)
@@ -239,7 +239,8 @@ namespace {
 
 struct TaskLevel {
     GridDesc g;
-    DBuf<float> ex, ey, ez, k2, e, e_alt, f, res;
+    DBuf<float> ex, ey, ez, k2, e, e2, f, res;
+    DBuf<uint8_t> cw;
     MgLevelDev d;
 };
 
@@ -261,10 +262,11 @@ int64_t lv_work(const GridDesc &g)
     return (int64_t)(g.ny - 2) * (g.nz - 2) * ((g.nx - 1) / 2);
 }
 
-MgLevelDev level_with_e(const MgLevelDev &d, float *e)
+MgLevelDev level_with_e(const MgLevelDev &d, float *e, float *e2)
 {
     MgLevelDev r = d;
     r.e = e;
+    r.e2 = e2;
     return r;
 }
 
@@ -337,7 +339,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             TaskLevel &lv = tw.L[l];
             const int64_t N = lv_nodes(lv.g);
             lv.e = DBuf<float>(w.pool, N);
-            if (l == 0) lv.e_alt = DBuf<float>(w.pool, N);
+            if (l < tw.tail_start || l == 0) lv.e2 = DBuf<float>(w.pool, N);
             lv.f = DBuf<float>(w.pool, N);
             if (l + 1 < nl) lv.res = DBuf<float>(w.pool, N);
             MgLevelDev &dv = lv.d;
@@ -351,6 +353,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             dv.ihx = dv.ihy = dv.ihz = ih;
             dv.omega = (float)estimate_omega(lv.g.nx, lv.g.ny, lv.g.nz);
             dv.e = lv.e.get();
+            dv.e2 = lv.e2.get();
             dv.f = lv.f.get();
             dv.res = lv.res.get();
             mg_level_index(dv);
@@ -374,6 +377,13 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                     dv.ez = lv.ez.get();
                     dv.k2 = lv.k2.get();
                     mg_inject(tw.L[l - 1].d, dv, w.stream);
+                    lv.cw = DBuf<uint8_t>(w.pool, N);
+                    dv.cw = lv.cw.get();
+                    dv.c_in = (float)m.c_in;
+                    dv.c_out = (float)m.c_out;
+                    dv.c_k2 = (float)m.c_k2;
+                    mg_pack_codes(dv.nx, dv.ny, dv.nz, dv.ex, dv.ey, dv.ez, dv.k2, dv.c_in,
+                                  dv.c_out, lv.cw.get(), w.stream);
                 }
             }
         }
@@ -388,7 +398,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         tw.partials = DBuf<double>(w.pool, O.nbk);
         O.partials = tw.partials.get();
         double *bufs[2] = {t->phi, tw.buf1.get()};
-        float *E[2] = {tw.L[0].e.get(), tw.L[0].e_alt.get()};
+        float *E[2] = {tw.L[0].e.get(), tw.L[0].e2.get()};
         tw.O0 = O;
         tw.O0.phi_in = tw.O0.phi_out = bufs[0];
         tw.O0.e = nullptr;
@@ -407,26 +417,30 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         std::vector<MgLevelDev> F, C;
         std::vector<MgTailDev> T;
         std::vector<MgOuterDev> O;
-        std::vector<float *> ez;
     } H[2];
-    std::vector<int64_t> maxwork(maxT, 0), maxnodes(maxT, 0), maxcoarse(maxT, 0);
+    std::vector<int64_t> maxnodes(maxT, 0), maxcoarse(maxT, 0);
+    std::vector<int> maxtiles(maxT, 0), maxnz(maxT, 0);
     int64_t maxN0 = 0;
     for (int p = 0; p < 2; ++p) {
         H[p].F.assign(nF, null_level());
         H[p].C.assign(nF, null_level());
         H[p].T.assign(S, MgTailDev{});
         H[p].O.resize(S);
-        H[p].ez.resize(S);
         for (int s = 0; s < S; ++s) {
             TaskWork &tw = W[s];
-            float *E[2] = {tw.L[0].e.get(), tw.L[0].e_alt.get()};
+            // level 0 alternates its two correction buffers with the cycle
+            // parity: the post-smoothed correction of parity p is read by that
+            // cycle's outer step while the other buffer serves as e2
+            float *E[2] = {tw.L[0].e.get(), tw.L[0].e2.get()};
             auto lev = [&](int l) {
-                return l == 0 ? level_with_e(tw.L[0].d, E[p]) : tw.L[l].d;
+                return l == 0 ? level_with_e(tw.L[0].d, E[p], E[1 - p]) : tw.L[l].d;
             };
             for (int l = 0; l < tw.tail_start; ++l) {
                 H[p].F[(size_t)l * S + s] = lev(l);
                 H[p].C[(size_t)l * S + s] = lev(l + 1);
-                maxwork[l] = std::max(maxwork[l], lv_work(tw.L[l].g));
+                const GridDesc &g = tw.L[l].g;
+                maxtiles[l] = std::max(maxtiles[l], smooth_tiles(g.nx, g.ny));
+                maxnz[l] = std::max(maxnz[l], g.nz);
                 maxnodes[l] = std::max(maxnodes[l], lv_nodes(tw.L[l].g));
                 maxcoarse[l] = std::max(maxcoarse[l], lv_nodes(tw.L[l + 1].g));
             }
@@ -436,24 +450,26 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             if (T.n > kMaxTail) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "too many multigrid levels");
             for (int l = tw.tail_start; l < nl; ++l) T.lv[l - tw.tail_start] = lev(l);
             H[p].O[s] = tw.O[p];
-            H[p].ez[s] = E[1 - p];
             maxN0 = std::max(maxN0, lv_nodes(tw.t->g));
         }
     }
     std::vector<MgOuterDev> O0(S);
-    std::vector<float *> ez0(S);
-    for (int s = 0; s < S; ++s) {
-        O0[s] = W[s].O0;
-        ez0[s] = W[s].L[0].e_alt.get();  // cycle 1 (parity 1) writes e_alt
+    for (int s = 0; s < S; ++s) O0[s] = W[s].O0;
+    // smoother launch shape per level: enough z-chunks for ~8 workgroups per CU
+    std::vector<int> zc(maxT, 1), sblocks(maxT, 0);
+    for (int l = 0; l < maxT; ++l) {
+        const int want = ceil_div(2048, std::max<int64_t>(1, (int64_t)maxtiles[l] * S));
+        const int nch = std::max(1, std::min(want, maxnz[l] / 16));
+        zc[l] = ceil_div(maxnz[l], nch);
+        sblocks[l] = maxtiles[l] * ceil_div(maxnz[l], zc[l]);
     }
     // one device blob holding every table
     auto bytes_of = [](size_t n, size_t sz) { return (n * sz + 255) & ~(size_t)255; };
     const size_t szF = bytes_of(nF ? nF : 1, sizeof(MgLevelDev));
     const size_t szT = bytes_of(S, sizeof(MgTailDev));
     const size_t szO = bytes_of(S, sizeof(MgOuterDev));
-    const size_t szE = bytes_of(S, sizeof(float *));
-    const size_t per = 2 * szF + szT + szO + szE;
-    const size_t total = 2 * per + szO + szE + bytes_of(S, sizeof(MgStatus));
+    const size_t per = 2 * szF + szT + szO;
+    const size_t total = 2 * per + szO + bytes_of(S, sizeof(MgStatus));
     DBuf<char> blob(w.pool, total);
     std::vector<char> hb(total, 0);
     char *dbase = blob.get();
@@ -462,7 +478,6 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         MgLevelDev *F, *C;
         MgTailDev *T;
         MgOuterDev *O;
-        float **ez;
     } D[2];
     for (int p = 0; p < 2; ++p) {
         std::memcpy(hb.data() + off, H[p].F.data(), nF * sizeof(MgLevelDev));
@@ -477,16 +492,10 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         std::memcpy(hb.data() + off, H[p].O.data(), S * sizeof(MgOuterDev));
         D[p].O = (MgOuterDev *)(dbase + off);
         off += szO;
-        std::memcpy(hb.data() + off, H[p].ez.data(), S * sizeof(float *));
-        D[p].ez = (float **)(dbase + off);
-        off += szE;
     }
     std::memcpy(hb.data() + off, O0.data(), S * sizeof(MgOuterDev));
     MgOuterDev *dO0 = (MgOuterDev *)(dbase + off);
     off += szO;
-    std::memcpy(hb.data() + off, ez0.data(), S * sizeof(float *));
-    float **dez0 = (float **)(dbase + off);
-    off += szE;
     std::vector<MgStatus> init(S);
     for (int s = 0; s < S; ++s) {
         const GridDesc &g = W[s].t->g;
@@ -498,32 +507,43 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     HIP_OK(hipMemcpyAsync(dbase, hb.data(), total, hipMemcpyHostToDevice, w.stream));
 
     // cycle 0: residual of the boundary-only potential (no test)
-    mg_outer64_z(dO0, dez0, S, maxN0, dst, w.stream);
+    mg_outer64(dO0, S, maxN0, dst, w.stream);
     mg_check(dO0, S, 0, dst, w.stream);
 
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
-    std::vector<int> ev_level_cycle;
+    // profiled level-0 smoother passes: events, cycle number, pass reads e
+    struct Ev {
+        hipEvent_t a, b;
+        int cycle;
+        bool reads_e;
+    };
+    std::vector<Ev> evs;
     int cycle_counter = 0;
+    // three sweeps per smoothing phase (pb_solver.rs:842-939), one pass each:
+    // pre-smoothing ends in e2 and post-smoothing in e, alternating in between
+    constexpr int npass = 3;
+    auto smooth_phase = [&](const MgLevelDev *F, int l, bool pre) {
+        for (int k = 0; k < npass; ++k) {
+            const bool first = pre && k == 0;
+            const int dir = ((npass - 1 - k) & 1) ^ (pre ? 1 : 0);
+            Ev ev{nullptr, nullptr, cycle_counter, !first};
+            if (profiling && l == 0) {
+                HIP_OK(hipEventCreate(&ev.a));
+                HIP_OK(hipEventCreate(&ev.b));
+                HIP_OK(hipEventRecord(ev.a, w.stream));
+            }
+            mg_smooth(F, S, first ? 1 : 0, dir, zc[l], sblocks[l], dst, w.stream);
+            if (profiling && l == 0) {
+                HIP_OK(hipEventRecord(ev.b, w.stream));
+                evs.push_back(ev);
+            }
+        }
+    };
     auto enqueue_cycle = [&](int p) {
         ++cycle_counter;
         for (int l = 0; l < maxT; ++l) {
             const MgLevelDev *F = D[p].F + (size_t)l * S;
             const MgLevelDev *C = D[p].C + (size_t)l * S;
-            for (int k = 0; k < 3; ++k)
-                for (int color = 0; color < 2; ++color) {
-                    hipEvent_t a = nullptr, b = nullptr;
-                    if (profiling && l == 0) {
-                        HIP_OK(hipEventCreate(&a));
-                        HIP_OK(hipEventCreate(&b));
-                        HIP_OK(hipEventRecord(a, w.stream));
-                    }
-                    mg_rb_sweep(F, S, maxwork[l], color, (l == 0 && k == 0) ? 1 : 0, dst, w.stream);
-                    if (profiling && l == 0) {
-                        HIP_OK(hipEventRecord(b, w.stream));
-                        evs.push_back({a, b});
-                        ev_level_cycle.push_back(cycle_counter);
-                    }
-                }
+            smooth_phase(F, l, true);
             mg_residual32(F, S, maxnodes[l], dst, w.stream);
             mg_restrict(F, C, S, maxcoarse[l], dst, w.stream);
         }
@@ -532,23 +552,9 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const MgLevelDev *F = D[p].F + (size_t)l * S;
             const MgLevelDev *C = D[p].C + (size_t)l * S;
             mg_prolong(C, F, S, maxnodes[l], dst, w.stream);
-            for (int k = 0; k < 3; ++k)
-                for (int color = 0; color < 2; ++color) {
-                    hipEvent_t a = nullptr, b = nullptr;
-                    if (profiling && l == 0) {
-                        HIP_OK(hipEventCreate(&a));
-                        HIP_OK(hipEventCreate(&b));
-                        HIP_OK(hipEventRecord(a, w.stream));
-                    }
-                    mg_rb_sweep(F, S, maxwork[l], color, 0, dst, w.stream);
-                    if (profiling && l == 0) {
-                        HIP_OK(hipEventRecord(b, w.stream));
-                        evs.push_back({a, b});
-                        ev_level_cycle.push_back(cycle_counter);
-                    }
-                }
+            smooth_phase(F, l, false);
         }
-        mg_outer64_z(D[p].O, D[p].ez, S, maxN0, dst, w.stream);
+        mg_outer64(D[p].O, S, maxN0, dst, w.stream);
         mg_check(D[p].O, S, 1, dst, w.stream);
     };
 
@@ -591,20 +597,21 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (profiling && !evs.empty()) {
+        // algorithmic bytes of one pass over a level-0 solve, per node: the
+        // correction read (unless starting from zero) and written, f, and the
+        // code byte (non-uniform maps)
         double ms_tot = 0.0, bytes = 0.0;
-        for (size_t k = 0; k < evs.size(); ++k) {
+        for (const Ev &ev : evs) {
             float ms = 0.0f;
-            HIP_OK(hipEventElapsedTime(&ms, evs[k].first, evs[k].second));
+            HIP_OK(hipEventElapsedTime(&ms, ev.a, ev.b));
             ms_tot += ms;
-            const int c = ev_level_cycle[k];
             for (int s = 0; s < S; ++s) {
-                if (W[s].tail_start == 0 || c > hs[s].cycles) continue;
-                const GridDesc &g = W[s].t->g;
-                const double interior = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
-                bytes += interior * (W[s].t->m->uni ? 6.0 : 14.0);
+                if (W[s].tail_start == 0 || ev.cycle > hs[s].cycles) continue;
+                const double n = (double)lv_nodes(W[s].t->g);
+                bytes += n * ((ev.reads_e ? 4.0 : 0.0) + 4.0 + 4.0 + (W[s].t->m->uni ? 0.0 : 1.0));
             }
-            (void)hipEventDestroy(evs[k].first);
-            (void)hipEventDestroy(evs[k].second);
+            (void)hipEventDestroy(ev.a);
+            (void)hipEventDestroy(ev.b);
         }
         std::lock_guard<std::mutex> lk(profile_mutex());
         prof.total_ms += ms_tot;
