@@ -85,14 +85,14 @@ struct MgOuterDev {
 // through an LDS ring and recomputes a halo, so the result equals that of two
 // in-place half-sweeps over the whole level.
 int smooth_tiles(int nx, int ny);  // (x, y) tiles of one plane
-void mg_smooth(const MgLevelDev *L, int n_solves, int first, int dir, int zc, int max_blocks,
-               const MgStatus *st, hipStream_t s);
+// C (one coarse level per solve, or null): add the coarse correction's
+// prolongation to the source while loading it (first post-smoothing pass).
+void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir, int zc,
+               int max_blocks, const MgStatus *st, hipStream_t s);
 void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
                    hipStream_t s);
 void mg_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_coarse,
                  const MgStatus *st, hipStream_t s);
-void mg_prolong(const MgLevelDev *C, const MgLevelDev *F, int n_solves, int64_t max_fine,
-                const MgStatus *st, hipStream_t s);
 void mg_tail(const MgTailDev *T, int n_solves, const MgStatus *st, hipStream_t s);
 void mg_outer64(const MgOuterDev *O, int n_solves, int64_t max_nodes, const MgStatus *st,
                 hipStream_t s);

@@ -25,6 +25,19 @@ namespace mmpbsa {
 
 __device__ __forceinline__ bool stopped(const MgStatus *st) { return st->state != 0; }
 
+// (solve, block) of a batched launch (grid = blocks x solves), XCD-ordered so
+// that each XCD walks contiguous blocks of one solve and z-neighbour planes
+// are still in its L2.
+struct BatchBlock {
+    int s, b;
+};
+__device__ __forceinline__ BatchBlock batch_block()
+{
+    const uint32_t flat = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const uint32_t s = flat / gridDim.x;
+    return {(int)s, (int)(flat - s * gridDim.x)};
+}
+
 // ---- node operations shared by the batched kernels and the tail --------------
 template <typename T>
 __device__ __forceinline__ T face_bit(uint32_t w, int bit, T cin, T cout)
@@ -185,24 +198,32 @@ __device__ __forceinline__ void restrict_node(const MgLevelDev &F, const MgLevel
     if (clear) C.e[idx] = 0.0f;
 }
 
-// prolongate_add (pb_solver.rs:628-681) for fine node idx
-__device__ __forceinline__ void prolong_node(const MgLevelDev &C, const MgLevelDev &F, int64_t idx)
+// prolongate_add (pb_solver.rs:628-681): the coarse correction interpolated
+// at fine node (fx, fy, fz); cells beyond the last coarse interval (even fine
+// sizes) extrapolate from it
+template <typename P>
+__device__ __forceinline__ float prolong_value(const MgLevelDev &C, P ce, int fx, int fy, int fz)
 {
-    int fx, fy, fz;
-    split3(F, idx, fx, fy, fz);
     const int x0 = min(fx >> 1, max(C.nx - 2, 0));
     const int y0 = min(fy >> 1, max(C.ny - 2, 0));
     const int z0 = min(fz >> 1, max(C.nz - 2, 0));
     const float dx = 0.5f * fx - x0, dy = 0.5f * fy - y0, dz = 0.5f * fz - z0;
-    const int64_t cs = C.nx, csz = (int64_t)C.nx * C.ny;
-    const float *__restrict__ c = C.e + x0 + cs * y0 + csz * z0;
-    const float c00 = c[0] * (1.0f - dx) + c[1] * dx;
-    const float c10 = c[cs] * (1.0f - dx) + c[cs + 1] * dx;
-    const float c01 = c[csz] * (1.0f - dx) + c[csz + 1] * dx;
-    const float c11 = c[csz + cs] * (1.0f - dx) + c[csz + cs + 1] * dx;
+    const uint32_t cs = (uint32_t)C.nx, csz = (uint32_t)C.nx * (uint32_t)C.ny;
+    const uint32_t o = (uint32_t)x0 + cs * (uint32_t)y0 + csz * (uint32_t)z0;
+    const float c00 = ce[o] * (1.0f - dx) + ce[o + 1] * dx;
+    const float c10 = ce[o + cs] * (1.0f - dx) + ce[o + cs + 1] * dx;
+    const float c01 = ce[o + csz] * (1.0f - dx) + ce[o + csz + 1] * dx;
+    const float c11 = ce[o + csz + cs] * (1.0f - dx) + ce[o + csz + cs + 1] * dx;
     const float c0 = c00 * (1.0f - dy) + c10 * dy;
     const float c1 = c01 * (1.0f - dy) + c11 * dy;
-    F.e[idx] += c0 * (1.0f - dz) + c1 * dz;
+    return c0 * (1.0f - dz) + c1 * dz;
+}
+
+__device__ __forceinline__ void prolong_node(const MgLevelDev &C, const MgLevelDev &F, int64_t idx)
+{
+    int fx, fy, fz;
+    split3(F, idx, fx, fy, fz);
+    F.e[idx] += prolong_value(C, (const float *)C.e, fx, fy, fz);
 }
 
 __device__ __forceinline__ int64_t nodes(const MgLevelDev &L) { return (int64_t)L.nx * L.ny * L.nz; }
@@ -244,8 +265,11 @@ __device__ __forceinline__ float face_sel(uint32_t w, int bit, uint32_t cin, uin
 // from a harmless address and discard the value): loads under control flow
 // make the compiler copy their results at the join, and the wait for those
 // copies would also wait for the prefetch issued before the barrier.
-template <bool FIRST>
-__global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restrict__ Ls, int dir,
+// PROLONG: the pass adds the coarse level's prolongated correction to every
+// node it loads (prolongate_add fused into the first post-smoothing sweep).
+template <bool FIRST, bool PROLONG>
+__global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restrict__ Ls,
+                                                     const MgLevelDev *__restrict__ Cs, int dir,
                                                      int zc, const MgStatus *__restrict__ st)
 {
     using namespace sm;
@@ -275,6 +299,9 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     const auto src = to_global<const float>(dir ? L.e : L.e2);
     const auto dst = to_global<float>((FIRST || dir) ? L.e2 : L.e);
     const auto fg = to_global<const float>(L.f);
+    MgLevelDev C{};
+    if (PROLONG) C = Cs[s];
+    const auto ceg = to_global<const float>(PROLONG ? C.e : L.f);
     const auto cwg = to_global<const uint8_t>(uni ? (const uint8_t *)L.f : L.cw);
     const auto exg = to_global<const float>(uni ? L.f : L.ex);
     const auto eyg = to_global<const float>(uni ? L.f : L.ey);
@@ -330,7 +357,9 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
         for (int j = 0; j < PF; ++j) {
             const int y = ly + 8 * j;
             const bool v = zin && lxok && y >= 0 && y < ny;
-            const float x = src[v ? zo + (uint32_t)(lgo0 + 8 * j * nx) : 0u];
+            float x = src[v ? zo + (uint32_t)(lgo0 + 8 * j * nx) : 0u];
+            if (PROLONG)  // in-range indices for lanes without a node
+                x += prolong_value(C, ceg, v ? lx : 0, v ? y : 0, v ? z : 0);
             pf[j] = v ? x : 0.0f;
         }
     };
@@ -442,27 +471,29 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     }
 }
 
-void mg_smooth(const MgLevelDev *L, int n_solves, int first, int dir, int zc, int max_blocks,
-               const MgStatus *st, hipStream_t s)
+void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir, int zc,
+               int max_blocks, const MgStatus *st, hipStream_t s)
 {
     if (n_solves <= 0 || max_blocks <= 0) return;
+    const dim3 grid(max_blocks, n_solves);
     if (first)
-        hipLaunchKernelGGL(smooth_kernel<true>, dim3(max_blocks, n_solves), dim3(256), 0, s, L,
-                           dir, zc, st);
+        hipLaunchKernelGGL((smooth_kernel<true, false>), grid, dim3(256), 0, s, L, C, dir, zc, st);
+    else if (C)
+        hipLaunchKernelGGL((smooth_kernel<false, true>), grid, dim3(256), 0, s, L, C, dir, zc, st);
     else
-        hipLaunchKernelGGL(smooth_kernel<false>, dim3(max_blocks, n_solves), dim3(256), 0, s, L,
-                           dir, zc, st);
+        hipLaunchKernelGGL((smooth_kernel<false, false>), grid, dim3(256), 0, s, L, C, dir, zc, st);
     LAUNCH_OK();
 }
 
 __global__ __launch_bounds__(256) void residual32_kernel(const MgLevelDev *__restrict__ Ls,
                                                          const MgStatus *__restrict__ st)
 {
-    const int s = blockIdx.y;
+    const BatchBlock bb = batch_block();
+    const int s = bb.s;
     if (stopped(st + s)) return;
     MgLevelDev L = Ls[s];
     if (L.nx == 0) return;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t idx = (int64_t)bb.b * blockDim.x + threadIdx.x;
     if (idx >= nodes(L)) return;
     L.e = L.e2;  // the pre-smoothed correction
     residual_node(L, idx);
@@ -472,29 +503,16 @@ __global__ __launch_bounds__(256) void restrict_kernel(const MgLevelDev *__restr
                                                        const MgLevelDev *__restrict__ Cs,
                                                        const MgStatus *__restrict__ st)
 {
-    const int s = blockIdx.y;
+    const BatchBlock bb = batch_block();
+    const int s = bb.s;
     if (stopped(st + s)) return;
     const MgLevelDev C = Cs[s];
     if (C.nx == 0) return;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t idx = (int64_t)bb.b * blockDim.x + threadIdx.x;
     if (idx >= nodes(C)) return;
     // batched levels start smoothing from zero without reading e, and the
     // tail clears its entry level itself
     restrict_node(Fs[s], C, idx, false);
-}
-
-__global__ __launch_bounds__(256) void prolong_kernel(const MgLevelDev *__restrict__ Cs,
-                                                      const MgLevelDev *__restrict__ Fs,
-                                                      const MgStatus *__restrict__ st)
-{
-    const int s = blockIdx.y;
-    if (stopped(st + s)) return;
-    MgLevelDev F = Fs[s];
-    if (F.nx == 0) return;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nodes(F)) return;
-    F.e = F.e2;  // added to the pre-smoothed correction
-    prolong_node(Cs[s], F, idx);
 }
 
 // One workgroup per solve runs the V-cycle over its tail levels.
@@ -580,13 +598,14 @@ __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restri
                                                       const MgStatus *__restrict__ st)
 {
     __shared__ double red[4];
-    const int s = blockIdx.y;
+    const BatchBlock bb = batch_block();
+    const int s = bb.s;
     if (stopped(st + s)) return;
     const MgOuterDev O = Os[s];
-    if (blockIdx.x >= (unsigned)O.nbk) return;
+    if (bb.b >= O.nbk) return;
     const int nx = O.nx, ny = O.ny, nz = O.nz;
     const int64_t N = (int64_t)nx * ny * nz;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t idx = (int64_t)bb.b * blockDim.x + threadIdx.x;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     double r2acc = 0.0;
     if (idx < N) {
@@ -633,7 +652,7 @@ __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restri
     r2acc = wave_sum(r2acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r2acc;
     __syncthreads();
-    if (threadIdx.x == 0) O.partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) O.partials[bb.b] = red[0] + red[1] + red[2] + red[3];
 }
 
 // Convergence / divergence test of solve_lpbe_multigrid (pb_solver.rs:1085-1147),
@@ -840,14 +859,6 @@ void mg_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t
 {
     if (max_coarse <= 0 || n_solves <= 0) return;
     hipLaunchKernelGGL(restrict_kernel, dim3(nblk(max_coarse), n_solves), dim3(256), 0, s, F, C, st);
-    LAUNCH_OK();
-}
-
-void mg_prolong(const MgLevelDev *C, const MgLevelDev *F, int n_solves, int64_t max_fine,
-                const MgStatus *st, hipStream_t s)
-{
-    if (max_fine <= 0 || n_solves <= 0) return;
-    hipLaunchKernelGGL(prolong_kernel, dim3(nblk(max_fine), n_solves), dim3(256), 0, s, C, F, st);
     LAUNCH_OK();
 }
 

@@ -510,28 +510,32 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     mg_outer64(dO0, S, maxN0, dst, w.stream);
     mg_check(dO0, S, 0, dst, w.stream);
 
-    // profiled level-0 smoother passes: events, cycle number, pass reads e
+    // profiled level-0 smoother passes: events, cycle number, pass reads e,
+    // pass adds the prolongated coarse correction
     struct Ev {
         hipEvent_t a, b;
         int cycle;
-        bool reads_e;
+        bool reads_e, prolong;
     };
     std::vector<Ev> evs;
     int cycle_counter = 0;
     // three sweeps per smoothing phase (pb_solver.rs:842-939), one pass each:
     // pre-smoothing ends in e2 and post-smoothing in e, alternating in between
     constexpr int npass = 3;
-    auto smooth_phase = [&](const MgLevelDev *F, int l, bool pre) {
+    // The first post-smoothing pass also applies prolongate_add from C.
+    auto smooth_phase = [&](const MgLevelDev *F, const MgLevelDev *C, int l, bool pre) {
         for (int k = 0; k < npass; ++k) {
             const bool first = pre && k == 0;
+            const bool prolong = !pre && k == 0;
             const int dir = ((npass - 1 - k) & 1) ^ (pre ? 1 : 0);
-            Ev ev{nullptr, nullptr, cycle_counter, !first};
+            Ev ev{nullptr, nullptr, cycle_counter, !first, prolong};
             if (profiling && l == 0) {
                 HIP_OK(hipEventCreate(&ev.a));
                 HIP_OK(hipEventCreate(&ev.b));
                 HIP_OK(hipEventRecord(ev.a, w.stream));
             }
-            mg_smooth(F, S, first ? 1 : 0, dir, zc[l], sblocks[l], dst, w.stream);
+            mg_smooth(F, prolong ? C : nullptr, S, first ? 1 : 0, dir, zc[l], sblocks[l], dst,
+                      w.stream);
             if (profiling && l == 0) {
                 HIP_OK(hipEventRecord(ev.b, w.stream));
                 evs.push_back(ev);
@@ -543,7 +547,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         for (int l = 0; l < maxT; ++l) {
             const MgLevelDev *F = D[p].F + (size_t)l * S;
             const MgLevelDev *C = D[p].C + (size_t)l * S;
-            smooth_phase(F, l, true);
+            smooth_phase(F, C, l, true);
             mg_residual32(F, S, maxnodes[l], dst, w.stream);
             mg_restrict(F, C, S, maxcoarse[l], dst, w.stream);
         }
@@ -551,8 +555,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         for (int l = maxT - 1; l >= 0; --l) {
             const MgLevelDev *F = D[p].F + (size_t)l * S;
             const MgLevelDev *C = D[p].C + (size_t)l * S;
-            mg_prolong(C, F, S, maxnodes[l], dst, w.stream);
-            smooth_phase(F, l, false);
+            smooth_phase(F, C, l, false);
         }
         mg_outer64(D[p].O, S, maxN0, dst, w.stream);
         mg_check(D[p].O, S, 1, dst, w.stream);
@@ -598,8 +601,9 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     if (graph) (void)hipGraphDestroy(graph);
     if (profiling && !evs.empty()) {
         // algorithmic bytes of one pass over a level-0 solve, per node: the
-        // correction read (unless starting from zero) and written, f, and the
-        // code byte (non-uniform maps)
+        // correction read (unless starting from zero) and written, f, the
+        // code byte (non-uniform maps) and, when prolongating, the coarse
+        // correction (one eighth of a float per fine node)
         double ms_tot = 0.0, bytes = 0.0;
         for (const Ev &ev : evs) {
             float ms = 0.0f;
@@ -608,7 +612,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             for (int s = 0; s < S; ++s) {
                 if (W[s].tail_start == 0 || ev.cycle > hs[s].cycles) continue;
                 const double n = (double)lv_nodes(W[s].t->g);
-                bytes += n * ((ev.reads_e ? 4.0 : 0.0) + 4.0 + 4.0 + (W[s].t->m->uni ? 0.0 : 1.0));
+                bytes += n * ((ev.reads_e ? 4.0 : 0.0) + 4.0 + 4.0 + (W[s].t->m->uni ? 0.0 : 1.0) +
+                              (ev.prolong ? 0.5 : 0.0));
             }
             (void)hipEventDestroy(ev.a);
             (void)hipEventDestroy(ev.b);
