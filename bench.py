@@ -3,12 +3,14 @@
 Workload: the 25,000-atom synthetic complex (23,500 receptor + 1,500 ligand
 atoms, 45 A sphere), frames = base + N(0, 0.3^2) A noise, PB on a 161^3 grid
 at 0.5 A (eps_in 4, eps_out 80, 0.15 M, two-level focusing), MM and SA for the
-complex, receptor and ligand of every frame.  One step = a batch of frames
-through the batched path (every PB solve of the batch advances in shared,
-graph-replayed launches).  Beside it: the plain path (one frame at a time,
-host-synchronous, no graphs), the smoother's roofline from HIP events in one
-extra profiled step of the same workload, the 2,500-atom complex's batched /
-plain ratio, and the FP64 CPU oracle on the host cores.
+complex, receptor and ligand of every frame.  The frames are placed in HBM
+before the timed region; one step = a batch of frames through the batched
+path (every PB solve of the batch advances in shared, graph-replayed
+launches).  Beside it: the same steps from host buffers (PCIe included), the
+plain path (one frame at a time, host-synchronous, no graphs), the smoother's
+roofline from HIP events in one extra profiled step of the same workload, the
+2,500-atom complex's batched / plain ratio, and the FP64 CPU oracle on the
+host cores.
 """
 import argparse
 import json
@@ -74,15 +76,18 @@ def main():
     frames_nm = np.stack([synthetic.noisy_frame(base, seed, f) for f in range(nfr)]) / 10.0
     pb = rst.PbParams(**CFG1_PB)
 
-    def step(k, batched=True):
-        return rst.compute_binding_energy_frames(top, frames_nm[k * B:(k + 1) * B], rec, lig,
-                                                 pb_params=pb, batched=batched)
+    import torch  # device memory for the resident frames (plumbing only)
+    dev = torch.from_numpy(frames_nm * 10.0).to("cuda")  # Angstrom, [F, N, 3] float64
+    torch.cuda.synchronize()
+    sync = torch.cuda.synchronize
 
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only
-        sync = lambda: None  # noqa: E731
+    def step(k, batched=True):
+        return rst.compute_binding_energy_device_frames(top, dev[k * B:(k + 1) * B], rec, lig,
+                                                        pb_params=pb, batched=batched)
+
+    def step_host(k):
+        return rst.compute_binding_energy_frames(top, frames_nm[k * B:(k + 1) * B], rec, lig,
+                                                 pb_params=pb, batched=True)
 
     for k in range(args.warmup):
         step(k)
@@ -103,6 +108,11 @@ def main():
     roofline = None
     extras = {}
     if not args.no_extras:
+        # the same steps from host buffers: PCIe transfers included
+        ts = time.perf_counter()
+        for k in range(args.warmup, args.warmup + args.steps):
+            step_host(k)
+        extras["host_buffer_frames_per_s"] = args.steps * B / (time.perf_counter() - ts)
         # dominant kernel: finest-level smoother passes, timed with HIP events
         lib.mmpbsa_profile_enable(1)
         step(args.warmup + args.steps)
@@ -134,7 +144,8 @@ def main():
     plain_lat = []
     for f in range(args.plain_frames):
         ts = time.perf_counter()
-        rst.compute_binding_energy_single_frame(top, frames_nm[f], rec, lig, pb_params=pb)
+        rst.compute_binding_energy_device_frames(top, dev[f:f + 1], rec, lig, pb_params=pb,
+                                                 batched=False)
         plain_lat.append(time.perf_counter() - ts)
     plain_fps = 1.0 / float(np.median(plain_lat)) if plain_lat else None
 

@@ -204,13 +204,25 @@ int mmpbsa_frame_energy(mmpbsa_split *s, const double *xyz, int32_t method,
                         const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
                         const mmpbsa_sa_params *sa, mmpbsa_frame_terms *out);
 
-/* Many frames of one split.  mode 0 = plain (host-synchronous, one frame at a
- * time), mode 1 = batched (frames overlapped on device streams, graph-replayed
- * solver cycles).  xyz_frames is [n_frames][n_atoms][3]. */
+/* Many frames of one split: the per-frame body of compute_binding_energy
+ * (binding.rs:563-689) over frames already read.  mode 0 = plain (one frame at
+ * a time, host-synchronous, no graphs), mode 1 = batched (the PB solves of
+ * groups of frames advance together in shared, graph-replayed launches).
+ * xyz_frames is [n_frames][n_atoms][3] in host memory. */
 int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_frames,
                          int32_t method, const mmpbsa_gb_params *gb,
                          const mmpbsa_pb_params *pb, const mmpbsa_sa_params *sa,
                          int32_t mode, mmpbsa_frame_terms *out);
+
+/* As mmpbsa_frames_energy with the frames already in device memory (HBM) of
+ * the context's device: d_xyz_frames is a device pointer, read in place on the
+ * library's stream, and must be fully written (caller's streams synchronized)
+ * before the call.  Results are identical to mmpbsa_frames_energy's.  No
+ * counterpart in the reference (its frames always come from files). */
+int mmpbsa_frames_energy_device(mmpbsa_split *s, const double *d_xyz_frames, int32_t n_frames,
+                                int32_t method, const mmpbsa_gb_params *gb,
+                                const mmpbsa_pb_params *pb, const mmpbsa_sa_params *sa,
+                                int32_t mode, mmpbsa_frame_terms *out);
 
 /* decompose_binding_energy (decomposition.rs:75).  rec_out/lig_out have one
  * entry per requested residue, in the order given at split creation. */

@@ -92,30 +92,40 @@ struct FrameCoords {
     const double *hcp = nullptr;
 };
 
-// Upload one frame and slice the complex, receptor and ligand coordinates
-// (binding.rs:213-222, :345-346) on the device.
-void stage_frame(Worker &w, mmpbsa_split *s, const double *xyz, FrameCoords &f)
+// Slice the complex, receptor and ligand coordinates of one frame
+// (binding.rs:213-222, :345-346) on the device.  Host frames are uploaded
+// first; device frames (on_device) are read in place and leave no host copy
+// (bounding boxes are then taken on the device).
+void stage_frame(Worker &w, mmpbsa_split *s, const double *xyz, FrameCoords &f, bool on_device)
 {
     const int n_all = s->ctx->top.h.n;
-    f.full = DBuf<double>(w.pool, 3 * (size_t)n_all);
-    HIP_OK(hipMemcpyAsync(f.full.get(), xyz, sizeof(double) * 3 * n_all, hipMemcpyHostToDevice,
-                          w.stream));
+    const double *full = xyz;
+    if (!on_device) {
+        f.full = DBuf<double>(w.pool, 3 * (size_t)n_all);
+        HIP_OK(hipMemcpyAsync(f.full.get(), xyz, sizeof(double) * 3 * n_all,
+                              hipMemcpyHostToDevice, w.stream));
+        full = f.full.get();
+    }
     if (s->identity) {
-        f.dc = f.full.get();
-        f.hcp = xyz;
+        f.dc = full;
+        f.hcp = on_device ? nullptr : xyz;
     } else {
         f.c = DBuf<double>(w.pool, 3 * s->csel.size());
-        launch_gather_xyz(f.full.get(), s->d_csel.p, (int)s->csel.size(), f.c.get(), w.stream);
+        launch_gather_xyz(full, s->d_csel.p, (int)s->csel.size(), f.c.get(), w.stream);
         f.dc = f.c.get();
-        f.hc = gather_host(xyz, s->csel);
-        f.hcp = f.hc.data();
+        if (!on_device) {
+            f.hc = gather_host(xyz, s->csel);
+            f.hcp = f.hc.data();
+        }
     }
     f.r = DBuf<double>(w.pool, 3 * s->ra.size() + 3);
     f.l = DBuf<double>(w.pool, 3 * s->la.size() + 3);
     launch_gather_xyz(f.dc, s->d_ra.p, (int)s->ra.size(), f.r.get(), w.stream);
     launch_gather_xyz(f.dc, s->d_la.p, (int)s->la.size(), f.l.get(), w.stream);
-    f.hr = gather_host(f.hcp, s->ra);
-    f.hl = gather_host(f.hcp, s->la);
+    if (!on_device) {
+        f.hr = gather_host(f.hcp, s->ra);
+        f.hl = gather_host(f.hcp, s->la);
+    }
 }
 
 double polar_of(Worker &w, const DevTopo &t, const double *d, const double *h, int method,
@@ -140,17 +150,19 @@ double mm_total(const mmpbsa_mm_terms &m)
 // graph-replayed cycles); otherwise each solve runs on its own.
 void frames_group(Worker &w, mmpbsa_split *s, const double *xyz, int nf, size_t stride,
                   int method, const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb,
-                  const mmpbsa_sa_params &sa, mmpbsa_frame_terms *out, bool batched)
+                  const mmpbsa_sa_params &sa, mmpbsa_frame_terms *out, bool batched,
+                  bool on_device = false)
 {
     std::vector<FrameCoords> fc(nf);
-    for (int f = 0; f < nf; ++f) stage_frame(w, s, xyz + stride * f, fc[f]);
+    for (int f = 0; f < nf; ++f) stage_frame(w, s, xyz + stride * f, fc[f], on_device);
     const DevTopo *tops[3] = {&s->ctop, &s->rtop, &s->ltop};
     std::vector<mmpbsa_pb_result> pr(3 * (size_t)nf);
     if (method == MMPBSA_METHOD_PB) {
         std::vector<PbJob> jobs(3 * (size_t)nf);
         for (int f = 0; f < nf; ++f) {
             const double *ds[3] = {fc[f].dc, fc[f].r.get(), fc[f].l.get()};
-            const double *hs[3] = {fc[f].hcp, fc[f].hr.data(), fc[f].hl.data()};
+            const double *hs[3] = {fc[f].hcp, on_device ? nullptr : fc[f].hr.data(),
+                                   on_device ? nullptr : fc[f].hl.data()};
             for (int k = 0; k < 3; ++k) {
                 PbJob &j = jobs[3 * (size_t)f + k];
                 j.t = tops[k];
@@ -171,7 +183,8 @@ void frames_group(Worker &w, mmpbsa_split *s, const double *xyz, int nf, size_t 
     }
     for (int f = 0; f < nf; ++f) {
         const double *ds[3] = {fc[f].dc, fc[f].r.get(), fc[f].l.get()};
-        const double *hs[3] = {fc[f].hcp, fc[f].hr.data(), fc[f].hl.data()};
+        const double *hs[3] = {fc[f].hcp, on_device ? nullptr : fc[f].hr.data(),
+                               on_device ? nullptr : fc[f].hl.data()};
         mmpbsa_frame_terms *o = out + f;
         mmpbsa_subsystem_energy *e[3] = {&o->complex_, &o->receptor, &o->ligand};
         for (int k = 0; k < 3; ++k) {
@@ -470,9 +483,13 @@ int mmpbsa_frame_energy(mmpbsa_split *s, const double *xyz, int32_t method,
     });
 }
 
-int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_frames,
-                         int32_t method, const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
-                         const mmpbsa_sa_params *sa, int32_t mode, mmpbsa_frame_terms *out)
+}  // extern "C"
+
+namespace {
+int frames_energy_impl(mmpbsa_split *s, const double *xyz_frames, int32_t n_frames,
+                       int32_t method, const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
+                       const mmpbsa_sa_params *sa, int32_t mode, mmpbsa_frame_terms *out,
+                       bool on_device)
 {
     if (!s || !out || n_frames < 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "invalid argument");
     mmpbsa_ctx *ctx = s->ctx;
@@ -491,7 +508,8 @@ int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_fr
         const size_t stride = 3 * (size_t)ctx->top.h.n;
         if (mode == 0) {
             for (int f = 0; f < n_frames; ++f)
-                frame_energy(*ctx->w, s, xyz_frames + stride * f, method, g, p, a, out + f);
+                frames_group(*ctx->w, s, xyz_frames + stride * f, 1, 0, method, g, p, a, out + f,
+                             false, on_device);
             return;
         }
         // batched: groups of frames whose PB solves advance together
@@ -499,9 +517,27 @@ int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_fr
         for (int f0 = 0; f0 < n_frames; f0 += group) {
             const int nf = std::min(group, n_frames - f0);
             frames_group(*ctx->w, s, xyz_frames + stride * f0, nf, stride, method, g, p, a,
-                         out + f0, true);
+                         out + f0, true, on_device);
         }
     });
+}
+}  // namespace
+
+extern "C" {
+
+int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_frames,
+                         int32_t method, const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
+                         const mmpbsa_sa_params *sa, int32_t mode, mmpbsa_frame_terms *out)
+{
+    return frames_energy_impl(s, xyz_frames, n_frames, method, gb, pb, sa, mode, out, false);
+}
+
+int mmpbsa_frames_energy_device(mmpbsa_split *s, const double *d_xyz_frames, int32_t n_frames,
+                                int32_t method, const mmpbsa_gb_params *gb,
+                                const mmpbsa_pb_params *pb, const mmpbsa_sa_params *sa,
+                                int32_t mode, mmpbsa_frame_terms *out)
+{
+    return frames_energy_impl(s, d_xyz_frames, n_frames, method, gb, pb, sa, mode, out, true);
 }
 
 // decompose_binding_energy_impl (decomposition.rs:210-410)
@@ -522,7 +558,7 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
         if (sap) sa = *sap;
         Worker &w = *ctx->w;
         FrameCoords f;
-        stage_frame(w, s, xyz, f);
+        stage_frame(w, s, xyz, f, false);
         const int nr = (int)s->ra.size(), nl = (int)s->la.size(), na = nr + nl;
         const PairTopo cv = s->ctop.view();
         // per-atom vdW / elec halves (receptor block, then ligand block)

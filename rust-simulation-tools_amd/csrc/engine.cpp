@@ -92,6 +92,18 @@ void bbox(int n, const double *xyz, double *mn, double *mx)
         }
 }
 
+void device_bbox(Worker &w, int n, const double *d_xyz, double *mn, double *mx)
+{
+    DBuf<double> d(w.pool, 6);
+    launch_bbox(d_xyz, n, d.get(), w.stream);
+    HIP_OK(hipMemcpyAsync(w.pin, d.get(), sizeof(double) * 6, hipMemcpyDeviceToHost, w.stream));
+    w.sync();
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = w.pin[k];
+        mx[k] = w.pin[3 + k];
+    }
+}
+
 // auto_grid (pb_grid.rs:188-215); the three spacings are equal.
 GridDesc auto_grid(const double *mn, const double *mx, double h, double buffer)
 {
@@ -211,7 +223,10 @@ void run_sa(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xy
     }
     const double max_ext = t.max_radius + p.probe_radius;
     double mn[3], mx[3];
-    bbox(n, h_xyz, mn, mx);
+    if (h_xyz)
+        bbox(n, h_xyz, mn, mx);
+    else
+        device_bbox(w, n, d_xyz, mn, mx);
     DevCells c;
     build_cells(w, d_xyz, n, mn, mx, max_ext, c);
     const std::vector<double> pts = fibonacci_points(p.n_sphere_points);

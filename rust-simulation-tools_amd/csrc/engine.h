@@ -142,6 +142,8 @@ struct SolveInfo {
 void run_mm(Worker &w, const DevTopo &t, const double *d_xyz, mmpbsa_mm_terms *out);
 double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_params &p,
               double *d_born /* nullable: pooled scratch used */, std::vector<double> *born_out);
+// h_xyz (host copy of d_xyz) may be null: the bounding box is then taken on
+// the device.
 void run_sa(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xyz,
             const mmpbsa_sa_params &p, double *d_per_atom_out, std::vector<double> *per_atom,
             double *total_sasa, double *total);
@@ -154,7 +156,7 @@ void run_pb(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xy
 struct PbJob {
     const DevTopo *t = nullptr;
     const double *d_xyz = nullptr;
-    const double *h_xyz = nullptr;
+    const double *h_xyz = nullptr;  // host copy, or null (bounding box on the device)
     mmpbsa_pb_params p{};
     mmpbsa_pb_result *out = nullptr;
     struct State;
@@ -172,6 +174,8 @@ struct DevCells {
     CellList cl;
 };
 void bbox(int n, const double *xyz, double *mn, double *mx);
+// the same from device coordinates (one small readback)
+void device_bbox(Worker &w, int n, const double *d_xyz, double *mn, double *mx);
 GridDesc auto_grid(const double *mn, const double *mx, double h, double buffer);
 GridDesc coarse_grid(const double *mn, const double *mx, double h, double fillratio,
                      int64_t fscale);

@@ -6,6 +6,8 @@
 // order, so boolean maps and face values match the CPU bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include <algorithm>
 #include <cmath>
 
@@ -654,6 +656,39 @@ void launch_gather_xyz(const double *src, const int *idx, int n, double *dst, hi
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(gather_xyz_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, src, idx, n, dst);
+    LAUNCH_OK();
+}
+
+// Bounding box of n AoS points (min then max per axis) by one workgroup.
+// Min and max are exact, so the result does not depend on the reduction order.
+__global__ __launch_bounds__(1024) void bbox_kernel(const double *__restrict__ xyz, int n,
+                                                    double *__restrict__ out)
+{
+    __shared__ double red[6][1024];
+    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        for (int k = 0; k < 3; ++k) {
+            const double c = xyz[3 * (int64_t)i + k];
+            if (c < v[k]) v[k] = c;
+            if (c > v[3 + k]) v[3 + k] = c;
+        }
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 6; ++k) {
+                const double o = red[k][threadIdx.x + off];
+                double &m = red[k][threadIdx.x];
+                if (k < 3 ? o < m : o > m) m = o;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+void launch_bbox(const double *xyz, int n, double *out6, hipStream_t s)
+{
+    hipLaunchKernelGGL(bbox_kernel, dim3(1), dim3(1024), 0, s, xyz, n, out6);
     LAUNCH_OK();
 }
 

@@ -675,6 +675,24 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
         if (j.t->h.n == 0) continue;
         act.push_back(&j);
     }
+    // bounding boxes of jobs without a host copy of their coordinates: one
+    // reduction per job, one readback for all
+    std::vector<double> dev_box;
+    {
+        std::vector<PbJob *> need;
+        for (PbJob *j : act)
+            if (!j->h_xyz) need.push_back(j);
+        if (!need.empty()) {
+            DBuf<double> d(w.pool, 6 * need.size());
+            for (size_t k = 0; k < need.size(); ++k)
+                launch_bbox(need[k]->d_xyz, need[k]->t->h.n, d.get() + 6 * k, w.stream);
+            dev_box.resize(6 * need.size());
+            HIP_OK(hipMemcpyAsync(dev_box.data(), d.get(), sizeof(double) * dev_box.size(),
+                                  hipMemcpyDeviceToHost, w.stream));
+            w.sync();
+        }
+    }
+    size_t dev_box_k = 0;
     // geometry, cell lists, charge maps (pb_energy.rs:332-372)
     for (PbJob *j : act) {
         j->st = std::make_shared<PbJob::State>();
@@ -682,7 +700,15 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
         const mmpbsa_pb_params &p = j->p;
         const int n = j->t->h.n;
         S.kappa = compute_kappa(p.salt_concentration, p.solvent_dielectric, p.temperature);
-        bbox(n, j->h_xyz, S.mn, S.mx);
+        if (j->h_xyz) {
+            bbox(n, j->h_xyz, S.mn, S.mx);
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                S.mn[k] = dev_box[6 * dev_box_k + k];
+                S.mx[k] = dev_box[6 * dev_box_k + 3 + k];
+            }
+            ++dev_box_k;
+        }
         S.fine = auto_grid(S.mn, S.mx, p.grid_spacing, p.grid_buffer);
         if (p.fine_dims[0] > 0) {
             S.fine.nx = p.fine_dims[0];

@@ -378,6 +378,37 @@ def compute_binding_energy_frames(topology: AmberTopology, frames, receptor_resi
     return BindingResult([FrameEnergy(t) for t in out])
 
 
+def compute_binding_energy_device_frames(topology: AmberTopology, frames_angstrom,
+                                         receptor_residues, ligand_residues, gb_params=None,
+                                         pb_params=None, sa_params=None,
+                                         batched=True) -> BindingResult:
+    """As compute_binding_energy_frames, with the frames already in HBM: a
+    contiguous float64 torch CUDA tensor [F, N, 3] in Angstrom (the library's
+    unit; no copy or conversion happens).  The caller's writes to it must be
+    complete (synchronize) before the call.  Extension: the reference reads
+    frames from files only."""
+    c = _ctx(topology)
+    t = frames_angstrom
+    if (not getattr(t, "is_cuda", False) or t.dtype != _torch_float64() or t.dim() != 3
+            or t.shape[1] != topology.n_atoms or t.shape[2] != 3 or not t.is_contiguous()):
+        raise ValueError("frames must be a contiguous float64 CUDA tensor of shape (F, N, 3)")
+    if t.shape[0] == 0:
+        raise ValueError("No frames found in trajectory")
+    s = c.split(receptor_residues, ligand_residues)
+    m, g, p = _method(gb_params, pb_params)
+    a = (sa_params or SaParams())._c
+    out = (FrameTermsC * t.shape[0])()
+    _check(c.lib.mmpbsa_frames_energy_device(s, C.c_void_p(t.data_ptr()), t.shape[0], m,
+                                             C.byref(g), C.byref(p), C.byref(a),
+                                             1 if batched else 0, out), c.lib, c.h)
+    return BindingResult([FrameEnergy(tt) for tt in out])
+
+
+def _torch_float64():
+    import torch
+    return torch.float64
+
+
 def decompose_binding_energy(topology: AmberTopology, coords, receptor_residues,
                              ligand_residues, gb_params=None, sa_params=None) -> DecompositionResult:
     c = _ctx(topology)

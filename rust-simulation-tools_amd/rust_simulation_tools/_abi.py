@@ -185,6 +185,10 @@ SIGNATURES = {
     "mmpbsa_frames_energy": (C.c_int, [P, c_double_p, C.c_int32, C.c_int32, C.POINTER(GbParamsC),
                                        C.POINTER(PbParamsC), C.POINTER(SaParamsC), C.c_int32,
                                        C.POINTER(FrameTermsC)]),
+    "mmpbsa_frames_energy_device": (C.c_int, [P, C.c_void_p, C.c_int32, C.c_int32,
+                                              C.POINTER(GbParamsC), C.POINTER(PbParamsC),
+                                              C.POINTER(SaParamsC), C.c_int32,
+                                              C.POINTER(FrameTermsC)]),
     "mmpbsa_decompose": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), C.POINTER(SaParamsC),
                                    C.POINTER(ResidueContribC), C.POINTER(ResidueContribC)]),
     "mmpbsa_profile_enable": (C.c_int, [C.c_int32]),

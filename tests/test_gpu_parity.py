@@ -124,3 +124,25 @@ def test_batched_frames_match_single(cfg0):
         for k in ("complex_polar", "receptor_polar", "ligand_polar", "delta_total", "delta_mm",
                   "delta_sa"):
             assert abs(getattr(fb, k) - getattr(fp, k)) <= 1e-9, k
+
+
+@pytest.mark.gpu
+def test_device_frames_match_host_frames(cfg0):
+    """Frames already in HBM give exactly the results of host frames (the
+    bounding boxes then come from a device reduction; min/max are exact)."""
+    import torch
+    from rust_simulation_tools import synthetic
+    top, _, rec, lig, seed = cfg0
+    base = synthetic.config_system(0)[1]
+    frames = np.stack([synthetic.noisy_frame(base, seed, f) for f in range(2)]) / 10.0
+    pb = rst.PbParams(**PB_CFG0)
+    dev = torch.from_numpy(frames * 10.0).to("cuda")
+    torch.cuda.synchronize()
+    for method in ("pb", "gb"):
+        kw = dict(pb_params=pb) if method == "pb" else {}
+        host = rst.compute_binding_energy_frames(top, frames, rec, lig, batched=True, **kw)
+        on_dev = rst.compute_binding_energy_device_frames(top, dev, rec, lig, batched=True, **kw)
+        for fh, fd in zip(host.frames(), on_dev.frames()):
+            for k in ("complex_polar", "receptor_polar", "ligand_polar", "complex_mm",
+                      "complex_sa", "delta_total"):
+                assert getattr(fh, k) == getattr(fd, k), (method, k)
