@@ -215,10 +215,11 @@ int mmpbsa_frames_energy(mmpbsa_split *s, const double *xyz_frames, int32_t n_fr
                          int32_t mode, mmpbsa_frame_terms *out);
 
 /* As mmpbsa_frames_energy with the frames already in device memory (HBM) of
- * the context's device: d_xyz_frames is a device pointer, read in place on the
- * library's stream, and must be fully written (caller's streams synchronized)
- * before the call.  Results are identical to mmpbsa_frames_energy's.  No
- * counterpart in the reference (its frames always come from files). */
+ * the context's device: d_xyz_frames is a device pointer (for example from
+ * mmpbsa_device_alloc), read in place on the library's stream, and must be
+ * fully written before the call.  Results are identical to
+ * mmpbsa_frames_energy's.  No counterpart in the reference (its frames always
+ * come from files). */
 int mmpbsa_frames_energy_device(mmpbsa_split *s, const double *d_xyz_frames, int32_t n_frames,
                                 int32_t method, const mmpbsa_gb_params *gb,
                                 const mmpbsa_pb_params *pb, const mmpbsa_sa_params *sa,
@@ -229,6 +230,17 @@ int mmpbsa_frames_energy_device(mmpbsa_split *s, const double *d_xyz_frames, int
 int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params *gb,
                      const mmpbsa_sa_params *sa, mmpbsa_residue_contrib *rec_out,
                      mmpbsa_residue_contrib *lig_out);
+
+/* ---- device memory --------------------------------------------------------
+ * For callers that stage inputs in HBM without another GPU runtime (no
+ * counterpart in the reference).  Allocation is on `device`; copies are
+ * synchronous.  mmpbsa_device_copy runs a plain copy kernel (bytes a multiple
+ * of 16), whose hardware counters calibrate traffic measurements.            */
+int mmpbsa_device_alloc(int32_t device, int64_t bytes, void **out);
+int mmpbsa_device_free(void *ptr);
+int mmpbsa_device_upload(void *dst, const void *src, int64_t bytes);
+int mmpbsa_device_download(void *dst, const void *src, int64_t bytes);
+int mmpbsa_device_copy(void *dst, const void *src, int64_t bytes);
 
 /* ---- measurement (no counterpart in the reference) ----------------------
  * While enabled, each launch of the finest-grid smoother half-sweep (the

@@ -622,6 +622,50 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
     });
 }
 
+int mmpbsa_device_alloc(int32_t device, int64_t bytes, void **out)
+{
+    if (!out || bytes < 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "invalid argument");
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        require_device();
+        HIP_OK(hipSetDevice(device));
+        HIP_OK(hipMalloc(out, (size_t)(bytes > 0 ? bytes : 1)));
+    });
+}
+
+int mmpbsa_device_free(void *ptr)
+{
+    return guarded(nullptr, [&] {
+        if (ptr) HIP_OK(hipFree(ptr));
+    });
+}
+
+int mmpbsa_device_upload(void *dst, const void *src, int64_t bytes)
+{
+    if ((!dst || !src) && bytes > 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null pointer");
+    return guarded(nullptr, [&] {
+        if (bytes > 0) HIP_OK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int mmpbsa_device_download(void *dst, const void *src, int64_t bytes)
+{
+    if ((!dst || !src) && bytes > 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null pointer");
+    return guarded(nullptr, [&] {
+        if (bytes > 0) HIP_OK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int mmpbsa_device_copy(void *dst, const void *src, int64_t bytes)
+{
+    if ((!dst || !src) && bytes > 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null pointer");
+    if (bytes % 16) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "bytes must be a multiple of 16");
+    return guarded(nullptr, [&] {
+        launch_copy16(src, dst, bytes / 16, nullptr);
+        HIP_OK(hipDeviceSynchronize());
+    });
+}
+
 int mmpbsa_profile_enable(int32_t on)
 {
     std::lock_guard<std::mutex> lk(profile_mutex());

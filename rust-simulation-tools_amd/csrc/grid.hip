@@ -686,6 +686,24 @@ __global__ __launch_bounds__(1024) void bbox_kernel(const double *__restrict__ x
     if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Plain copy of n 16-byte words (a reference point for traffic counters).
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t s)
+{
+    if (n16 <= 0) return;
+    const int blocks = (int)std::min<int64_t>(ceil_div(n16, 256), 8192);
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst,
+                       n16);
+    LAUNCH_OK();
+}
+
 void launch_bbox(const double *xyz, int n, double *out6, hipStream_t s)
 {
     hipLaunchKernelGGL(bbox_kernel, dim3(1), dim3(1024), 0, s, xyz, n, out6);

@@ -119,6 +119,8 @@ void launch_fill_u64(unsigned long long *p, unsigned long long v, int64_t n, hip
 void launch_gather_xyz(const double *src, const int *idx, int n, double *dst, hipStream_t s);
 // mn[3] then mx[3] of n AoS points into out6 (device)
 void launch_bbox(const double *xyz, int n, double *out6, hipStream_t s);
+// plain device copy of n16 16-byte words
+void launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t s);
 void launch_segment_sum(const double *v, const int *seg_atoms, const int *seg_ptr, int n_seg,
                         double *out, hipStream_t s);
 

@@ -378,35 +378,72 @@ def compute_binding_energy_frames(topology: AmberTopology, frames, receptor_resi
     return BindingResult([FrameEnergy(t) for t in out])
 
 
-def compute_binding_energy_device_frames(topology: AmberTopology, frames_angstrom,
+class DeviceArray:
+    """A float64 array in HBM, allocated and filled through the library (no
+    other GPU runtime involved).  Slicing the first axis gives a view.
+    Extension: the reference has no device-resident inputs."""
+
+    def __init__(self, host=None, *, _base=None, _ptr=None, _shape=None, device=0):
+        self._base = _base
+        if _base is not None:
+            self.ptr, self.shape = _ptr, _shape
+            return
+        a = np.ascontiguousarray(np.asarray(host, dtype=np.float64))
+        self.shape = a.shape
+        lib = _abi.load_library()
+        p = C.c_void_p()
+        _check(lib.mmpbsa_device_alloc(device, a.nbytes, C.byref(p)), lib, None)
+        self.ptr = p.value
+        self._lib = lib
+        _check(lib.mmpbsa_device_upload(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p),
+                                        a.nbytes), lib, None)
+
+    def __getitem__(self, sl):
+        if not isinstance(sl, slice) or sl.step not in (None, 1):
+            raise TypeError("DeviceArray supports contiguous slices of the first axis")
+        a, b, _ = sl.indices(self.shape[0])
+        row = int(np.prod(self.shape[1:])) * 8
+        return DeviceArray(_base=self, _ptr=self.ptr + a * row,
+                           _shape=(max(0, b - a),) + tuple(self.shape[1:]))
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=np.float64)
+        lib = _abi.load_library()
+        _check(lib.mmpbsa_device_download(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                          out.nbytes), lib, None)
+        return out
+
+    def __del__(self):
+        if getattr(self, "_base", None) is None and getattr(self, "ptr", None):
+            try:
+                self._lib.mmpbsa_device_free(C.c_void_p(self.ptr))
+            except Exception:
+                pass
+            self.ptr = None
+
+
+def compute_binding_energy_device_frames(topology: AmberTopology, frames_angstrom: DeviceArray,
                                          receptor_residues, ligand_residues, gb_params=None,
                                          pb_params=None, sa_params=None,
                                          batched=True) -> BindingResult:
     """As compute_binding_energy_frames, with the frames already in HBM: a
-    contiguous float64 torch CUDA tensor [F, N, 3] in Angstrom (the library's
-    unit; no copy or conversion happens).  The caller's writes to it must be
-    complete (synchronize) before the call.  Extension: the reference reads
-    frames from files only."""
+    DeviceArray [F, N, 3] in Angstrom (the library's unit; nothing is copied
+    or converted).  Extension: the reference reads frames from files only."""
     c = _ctx(topology)
     t = frames_angstrom
-    if (not getattr(t, "is_cuda", False) or t.dtype != _torch_float64() or t.dim() != 3
-            or t.shape[1] != topology.n_atoms or t.shape[2] != 3 or not t.is_contiguous()):
-        raise ValueError("frames must be a contiguous float64 CUDA tensor of shape (F, N, 3)")
+    if (not isinstance(t, DeviceArray) or len(t.shape) != 3 or t.shape[1] != topology.n_atoms
+            or t.shape[2] != 3):
+        raise ValueError("frames must be a DeviceArray of shape (F, N, 3)")
     if t.shape[0] == 0:
         raise ValueError("No frames found in trajectory")
     s = c.split(receptor_residues, ligand_residues)
     m, g, p = _method(gb_params, pb_params)
     a = (sa_params or SaParams())._c
     out = (FrameTermsC * t.shape[0])()
-    _check(c.lib.mmpbsa_frames_energy_device(s, C.c_void_p(t.data_ptr()), t.shape[0], m,
-                                             C.byref(g), C.byref(p), C.byref(a),
-                                             1 if batched else 0, out), c.lib, c.h)
+    _check(c.lib.mmpbsa_frames_energy_device(s, C.c_void_p(t.ptr), t.shape[0], m, C.byref(g),
+                                             C.byref(p), C.byref(a), 1 if batched else 0, out),
+           c.lib, c.h)
     return BindingResult([FrameEnergy(tt) for tt in out])
-
-
-def _torch_float64():
-    import torch
-    return torch.float64
 
 
 def decompose_binding_energy(topology: AmberTopology, coords, receptor_residues,

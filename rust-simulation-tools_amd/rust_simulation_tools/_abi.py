@@ -191,6 +191,11 @@ SIGNATURES = {
                                               C.POINTER(FrameTermsC)]),
     "mmpbsa_decompose": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), C.POINTER(SaParamsC),
                                    C.POINTER(ResidueContribC), C.POINTER(ResidueContribC)]),
+    "mmpbsa_device_alloc": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
+    "mmpbsa_device_free": (C.c_int, [C.c_void_p]),
+    "mmpbsa_device_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "mmpbsa_device_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "mmpbsa_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "mmpbsa_profile_enable": (C.c_int, [C.c_int32]),
     "mmpbsa_profile_read": (C.c_int, [c_double_p, c_int64_p, c_double_p]),
 }

@@ -126,23 +126,24 @@ def test_batched_frames_match_single(cfg0):
             assert abs(getattr(fb, k) - getattr(fp, k)) <= 1e-9, k
 
 
-@pytest.mark.gpu
 def test_device_frames_match_host_frames(cfg0):
     """Frames already in HBM give exactly the results of host frames (the
     bounding boxes then come from a device reduction; min/max are exact)."""
-    import torch
     from rust_simulation_tools import synthetic
     top, _, rec, lig, seed = cfg0
     base = synthetic.config_system(0)[1]
     frames = np.stack([synthetic.noisy_frame(base, seed, f) for f in range(2)]) / 10.0
     pb = rst.PbParams(**PB_CFG0)
-    dev = torch.from_numpy(frames * 10.0).to("cuda")
-    torch.cuda.synchronize()
+    dev = rst.DeviceArray(frames * 10.0)
+    assert np.array_equal(dev.numpy(), frames * 10.0)
     for method in ("pb", "gb"):
         kw = dict(pb_params=pb) if method == "pb" else {}
         host = rst.compute_binding_energy_frames(top, frames, rec, lig, batched=True, **kw)
         on_dev = rst.compute_binding_energy_device_frames(top, dev, rec, lig, batched=True, **kw)
+        one = rst.compute_binding_energy_device_frames(top, dev[1:2], rec, lig, batched=False,
+                                                       **kw)
         for fh, fd in zip(host.frames(), on_dev.frames()):
             for k in ("complex_polar", "receptor_polar", "ligand_polar", "complex_mm",
                       "complex_sa", "delta_total"):
                 assert getattr(fh, k) == getattr(fd, k), (method, k)
+        assert one.frames()[0].delta_total == host.frames()[1].delta_total
