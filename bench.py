@@ -13,9 +13,13 @@ roofline from HIP events in one extra profiled step of the same workload, the
 host cores.
 """
 import argparse
+import csv
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -35,6 +39,49 @@ CFG0_PB = dict(grid_spacing=0.6, solute_dielectric=4.0, solvent_dielectric=80.0,
                salt_concentration=0.15, fine_dims=(65, 65, 65))
 
 
+def measure_traffic(frames_per_step, grid_threads):
+    """HBM bytes per launch of the timed smoother passes from two rocprofv3
+    counter runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of one batched
+    step in a child process, each scaled by what the same counter reads for a
+    1 GiB plain copy kernel run first in that process.  The timed launches are
+    the smoother dispatches of that launch size."""
+    prof = shutil.which("rocprofv3")
+    if not prof or grid_threads <= 0:
+        return None
+    copy_mib = 1024
+    child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0",
+             "--frames-per-step", str(frames_per_step), "--no-cpu", "--no-extras",
+             "--plain-frames", "0", "--no-traffic", "--copy-calibration-mib", str(copy_mib)]
+    vals = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(tmp, counter)
+            cmd = [prof, "--pmc", counter, "--kernel-include-regex", "smooth_kernel|copy_kernel",
+                   "--output-format", "csv", "-d", out, "-o", "run", "--"] + child
+            r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               timeout=300, cwd=tmp, env=dict(os.environ, TMPDIR="/tmp"))
+            path = os.path.join(out, "run_counter_collection.csv")
+            if r.returncode != 0 or not os.path.exists(path):
+                return None
+            copy, smooth = [], []
+            with open(path) as fh:
+                for row in csv.DictReader(fh):
+                    v = float(row["Counter_Value"])
+                    if "copy_kernel" in row["Kernel_Name"]:
+                        copy.append(v)
+                    elif int(row["Grid_Size"]) == grid_threads:
+                        smooth.append(v)
+            if not copy or not smooth:
+                return None
+            scale = (copy_mib * 1024.0) / (sum(copy) / len(copy))  # KiB moved / KiB counted
+            vals[counter] = (sum(smooth) / len(smooth), scale, len(smooth))
+    fetch = vals["FETCH_SIZE"][0] * vals["FETCH_SIZE"][1] * 1024.0
+    write = vals["WRITE_SIZE"][0] * vals["WRITE_SIZE"][1] * 1024.0
+    return {"bytes_per_launch": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_scale": vals["FETCH_SIZE"][1], "write_scale": vals["WRITE_SIZE"][1],
+            "dispatches": vals["FETCH_SIZE"][2], "copy_calibration_bytes": copy_mib << 20}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +92,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip roofline and small-system legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter runs behind roofline.traffic")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy a buffer of this size on the GPU (a plain copy kernel of "
                          "known bytes, to scale hardware traffic counters)")
@@ -112,13 +161,16 @@ def main():
         # dominant kernel: finest-level smoother passes, timed with HIP events
         lib.mmpbsa_profile_enable(1)
         step(args.warmup + args.steps)
-        ms, n, by = C.c_double(), C.c_int64(), C.c_double()
-        lib.mmpbsa_profile_read(C.byref(ms), C.byref(n), C.byref(by))
+        ms, n, by, gt = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+        lib.mmpbsa_profile_read(C.byref(ms), C.byref(n), C.byref(by), C.byref(gt))
         lib.mmpbsa_profile_enable(0)
         if n.value > 0 and ms.value > 0:
             achieved = by.value / (ms.value * 1e-3) / 1e9
+            traffic = None if args.no_traffic else measure_traffic(B, gt.value)
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": traffic["bytes_per_launch"] if traffic else None,
+                        "traffic_detail": traffic,
                         "kernel": "smooth_kernel (finest level: one red-black sweep per "
                                   "launch, batched over solves)",
                         "launches": n.value, "avg_launch_us": ms.value * 1e3 / n.value,

@@ -243,14 +243,17 @@ int mmpbsa_device_download(void *dst, const void *src, int64_t bytes);
 int mmpbsa_device_copy(void *dst, const void *src, int64_t bytes);
 
 /* ---- measurement (no counterpart in the reference) ----------------------
- * While enabled, each launch of the finest-grid smoother half-sweep (the
- * dominant kernel) is bracketed by HIP events on its own stream and solver
- * cycles are launched directly rather than replayed from a graph.
- * mmpbsa_profile_read returns the summed event time, the number of launches
- * and their algorithmic bytes (14 B per interior node of a variable-
- * dielectric grid, 6 B for a uniform one: half of a full red-black sweep). */
+ * While enabled, each level-0 smoother pass of the fine focusing stage (the
+ * dominant kernel: one red-black sweep per launch) is bracketed by HIP events
+ * on its own stream, and solver cycles are launched directly rather than
+ * replayed from a graph.  mmpbsa_profile_read returns the summed event time,
+ * the number of launches, their algorithmic bytes (per node: correction read
+ * 4 unless the pass starts from zero, right-hand side 4, code byte 1 on
+ * non-uniform maps, correction write 4, plus 0.5 when the pass prolongates)
+ * and the size in threads of the last timed launch. */
 int mmpbsa_profile_enable(int32_t on);
-int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes);
+int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes,
+                        int64_t *grid_threads);
 
 #ifdef __cplusplus
 }

@@ -677,13 +677,15 @@ int mmpbsa_profile_enable(int32_t on)
     return MMPBSA_OK;
 }
 
-int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes)
+int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes,
+                        int64_t *grid_threads)
 {
     std::lock_guard<std::mutex> lk(profile_mutex());
     const ProfileStats &p = profile_stats();
     if (total_ms) *total_ms = p.total_ms;
     if (launches) *launches = p.launches;
     if (alg_bytes) *alg_bytes = p.alg_bytes;
+    if (grid_threads) *grid_threads = p.grid_threads;
     return MMPBSA_OK;
 }
 

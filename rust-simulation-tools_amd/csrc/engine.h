@@ -194,6 +194,7 @@ struct ProfileStats {
     double total_ms = 0.0;
     int64_t launches = 0;
     double alg_bytes = 0.0;
+    int64_t grid_threads = 0;  // launch size (threads) of the last timed pass
 };
 ProfileStats &profile_stats();
 std::mutex &profile_mutex();

@@ -261,12 +261,12 @@ __device__ __forceinline__ float face_sel(uint32_t w, int bit, uint32_t cin, uin
     return __uint_as_float((m & cin) | (~m & cout));
 }
 
-// Every global load of the kernel is unconditional (lanes without work load
-// from a harmless address and discard the value): loads under control flow
-// make the compiler copy their results at the join, and the wait for those
-// copies would also wait for the prefetch issued before the barrier.
-// PROLONG: the pass adds the coarse level's prolongated correction to every
-// node it loads (prolongate_add fused into the first post-smoothing sweep).
+// Loads are unconditional (lanes without work load from a harmless address and
+// discard the value): loads under control flow make the compiler copy their
+// results at the join and wait for every older load there.  The right-hand
+// side and code bytes are loaded once per plane for both columns of a pair,
+// coalesced, and kept in a three-plane register ring: plane z serves the red
+// half-sweep of step z and the black half-sweep of step z + 1.
 template <bool FIRST, bool PROLONG>
 __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restrict__ Ls,
                                                      const MgLevelDev *__restrict__ Cs, int dir,
@@ -294,8 +294,8 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     // 32-bit vector offset from a scalar base
     const uint32_t sxy = (uint32_t)nx * (uint32_t)ny;
     const bool uni = L.uni != 0;
-    // non-uniform levels always carry code bytes; uniform ones read no
-    // coefficients, and their dummy loads go to f
+    // non-uniform levels always carry code bytes; uniform ones read none, and
+    // their dummy loads go to f
     const auto src = to_global<const float>(dir ? L.e : L.e2);
     const auto dst = to_global<float>((FIRST || dir) ? L.e2 : L.e);
     const auto fg = to_global<const float>(L.f);
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     const auto exg = to_global<const float>(uni ? L.f : L.ex);
     const auto eyg = to_global<const float>(uni ? L.f : L.ey);
     const auto ezg = to_global<const float>(uni ? L.f : L.ez);
-    const float ih = L.ihx;                // the grids are cubic: ihx == ihy == ihz
+    const float ih = L.ihx;  // the grids are cubic: ihx == ihy == ihz
     const float omega = L.omega, ck2 = L.c_k2;
     const uint32_t cin = __float_as_uint(L.c_in), cout = __float_as_uint(L.c_out);
     const float uih = L.eps_u * ih, udg_inv = 1.0f / (6.0f * L.eps_u * ih);
@@ -314,24 +314,29 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     const int t0 = z0 - K + 1;
     const int t1 = z1 + K - 1;  // last step: writes plane z1 - 1
 
-    // per column pair and step parity q = (t - t0) & 1: region index and
-    // in-plane offset of the active node, and whether it is interior to both
-    // the grid and the shrunk region
-    int ri[2][PP], go[2][PP];
-    bool ok[2][PP];
+    // Column pair m: region row and first column, in-plane offset of the pair,
+    // which columns exist in the grid, and per step parity q = (t - t0) & 1 the
+    // active column (0 / 1) and whether it is interior to grid and region.
+    int pri[PP], pgo[PP];
+    bool pin0[PP], pin1[PP], act1[2][PP], ok[2][PP];
 #pragma unroll
     for (int m = 0; m < PP; ++m) {
         const int P = tid + 256 * m;
         const int ry = P / (RW / 2);
-        const int y = yl + ry;
+        const int rx0 = 2 * (P - ry * (RW / 2));
+        const int y = yl + ry, x = xl + rx0;
+        pri[m] = ry * RW + rx0;
+        pgo[m] = x + nx * y;
+        const bool yin = y >= 0 && y < ny;
+        pin0[m] = yin && x >= 0 && x < nx;
+        pin1[m] = yin && x + 1 >= 0 && x + 1 < nx;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const int rx = 2 * (P - ry * (RW / 2)) + ((y + t0 + q - xl) & 1);
-            const int x = xl + rx;
-            ri[q][m] = ry * RW + rx;
-            go[q][m] = x + nx * y;
-            ok[q][m] = ry >= 1 && ry <= RW - 2 && rx >= 1 && rx <= RW - 2 && x >= 1 &&
-                       x <= nx - 2 && y >= 1 && y <= ny - 2;
+            const int c = (y + t0 + q - xl) & 1;
+            const int rx = rx0 + c, xa = x + c;
+            act1[q][m] = c != 0;
+            ok[q][m] = ry >= 1 && ry <= RW - 2 && rx >= 1 && rx <= RW - 2 && xa >= 1 &&
+                       xa <= nx - 2 && y >= 1 && y <= ny - 2;
         }
     }
     // plane loads: region value n = tid + 256 j sits in region row
@@ -383,22 +388,27 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     auto zok = [&](int k, int z) {
         return z >= 1 && z <= nz - 2 && z >= z0 - (K - k) && z <= z1 - 1 + (K - k);
     };
-    // right-hand side and code bytes of both half-sweeps of step t (parity q)
-    auto prefetch = [&](int t, int q, float (&F)[K][PP], uint32_t (&W)[K][PP]) {
+    // f and code bytes of both columns of every pair at plane z
+    struct PlaneCoef {
+        float f0[PP], f1[PP];
+        uint32_t w0[PP], w1[PP];
+    };
+    auto load_coef = [&](int z, PlaneCoef &Q) {
+        const bool zin = z >= 1 && z <= nz - 2;
+        const uint32_t zo = sxy * (uint32_t)(zin ? z : 0);
 #pragma unroll
-        for (int k = 1; k <= K; ++k) {
-            const int z = t - k + 1;
-            const bool zk = zok(k, z);
-            const uint32_t zo = sxy * (uint32_t)(zk ? z : 0);
-#pragma unroll
-            for (int m = 0; m < PP; ++m) {
-                const uint32_t g = (zk && ok[q][m]) ? zo + (uint32_t)go[q][m] : 0u;
-                F[k - 1][m] = fg[g];
-                W[k - 1][m] = cwg[g];
-            }
+        for (int m = 0; m < PP; ++m) {
+            const uint32_t g0 = (zin && pin0[m]) ? zo + (uint32_t)pgo[m] : 0u;
+            const uint32_t g1 = (zin && pin1[m]) ? zo + (uint32_t)pgo[m] + 1u : 0u;
+            Q.f0[m] = fg[g0];
+            Q.f1[m] = fg[g1];
+            Q.w0[m] = cwg[g0];
+            Q.w1[m] = cwg[g1];
         }
     };
-    auto stage = [&](int k, int t, int q, const float (&F)[K][PP], const uint32_t (&W)[K][PP]) {
+    // half-sweep k of step t (parity q) on plane z = t - k + 1 with that
+    // plane's coefficients Q
+    auto stage = [&](int k, int t, int q, const PlaneCoef &Q) {
         const int z = t - k + 1;
         if (!zok(k, z)) return;
         const float *__restrict__ pm = slot(z - 1);
@@ -407,15 +417,17 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
 #pragma unroll
         for (int m = 0; m < PP; ++m) {
             if (!ok[q][m]) continue;
-            const int i = ri[q][m];
+            const bool c1 = act1[q][m];
+            const int i = pri[m] + (c1 ? 1 : 0);
+            const float fv = c1 ? Q.f1[m] : Q.f0[m];
             const float exx = pc[i + 1], emx = pc[i - 1], eyy = pc[i + RW], emy = pc[i - RW];
             const float ezz = pp[i], emz = pm[i], old = pc[i];
             float gs;
             if (uni) {
                 const float sum = ((exx + emx) + (eyy + emy)) + (ezz + emz);
-                gs = fmaf(sum, uih, F[k - 1][m]) * udg_inv;
+                gs = fmaf(sum, uih, fv) * udg_inv;
             } else {
-                const uint32_t w = W[k - 1][m];
+                const uint32_t w = c1 ? Q.w1[m] : Q.w0[m];
                 float a0, a1, a2, a3, a4, a5;
                 if (!(w & kCodeSpecial)) {
                     a0 = face_sel(w, 0, cin, cout);
@@ -425,7 +437,7 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
                     a4 = face_sel(w, 4, cin, cout);
                     a5 = face_sel(w, 5, cin, cout);
                 } else {  // rare: faces cut by the dielectric boundary
-                    const uint32_t g = sxy * (uint32_t)z + (uint32_t)go[q][m];
+                    const uint32_t g = sxy * (uint32_t)z + (uint32_t)(pgo[m] + (c1 ? 1 : 0));
                     a0 = exg[g];
                     a1 = exg[g - 1];
                     a2 = eyg[g];
@@ -437,37 +449,39 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
                 const float sum =
                     fmaf(a0, exx, fmaf(a1, emx, fmaf(a2, eyy, fmaf(a3, emy, fmaf(a4, ezz, a5 * emz)))));
                 const float dg = fmaf(((a0 + a1) + (a2 + a3)) + (a4 + a5), ih, k2);
-                gs = fmaf(sum, ih, F[k - 1][m]) * __builtin_amdgcn_rcpf(dg);
+                gs = fmaf(sum, ih, fv) * __builtin_amdgcn_rcpf(dg);
             }
             pc[i] = fmaf(omega, gs - old, old);
         }
     };
-    // one step: stage plane t+1, write plane t-2 back, start the loads of
-    // plane t+2 and of the next step's coefficients, then the two half-sweeps
-    auto step = [&](int t, int q, const float (&F)[K][PP], const uint32_t (&W)[K][PP],
-                    float (&Fn)[K][PP], uint32_t (&Wn)[K][PP]) {
+    // one step: stage plane t+1 of e, write plane t-2 back, start the loads of
+    // the coefficients of plane t+1 and of e at plane t+2, then the red
+    // half-sweep on plane t (coefficients Qb) and the black one on plane t-1
+    // (coefficients Qa)
+    PlaneCoef Qa, Qb, Qn;
+    auto step = [&](int t, int q) {
         store_plane(t + 1);
         write_back(t - K);
+        load_coef(t + 1, Qn);
         load_plane(t + 2);
-        prefetch(t + 1, q ^ 1, Fn, Wn);
         __syncthreads();
-#pragma unroll
-        for (int k = 1; k <= K; ++k) {
-            stage(k, t, q, F, W);
-            __syncthreads();
-        }
+        stage(1, t, q, Qb);
+        __syncthreads();
+        stage(2, t, q, Qa);
+        __syncthreads();
+        Qa = Qb;
+        Qb = Qn;
     };
-    float fa[K][PP], fb[K][PP];
-    uint32_t wa[K][PP], wb[K][PP];
     load_plane(t0 - 1);
     store_plane(t0 - 1);
     load_plane(t0);
     store_plane(t0);
+    load_coef(t0 - 1, Qa);
+    load_coef(t0, Qb);
     load_plane(t0 + 1);
-    prefetch(t0, 0, fa, wa);
     for (int t = t0; t <= t1; t += 2) {
-        step(t, 0, fa, wa, fb, wb);
-        if (t + 1 <= t1) step(t + 1, 1, fb, wb, fa, wa);
+        step(t, 0);
+        if (t + 1 <= t1) step(t + 1, 1);
     }
 }
 

@@ -278,18 +278,23 @@ MgLevelDev null_level()
 
 }  // namespace
 
-static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_graph);
+static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_graph,
+                           bool profile_level0);
 
-static void mg_solve_batch(Worker &w, std::vector<MgTask *> &tasks, bool use_graph)
+// profile_level0: when profiling is enabled, time this batch's level-0
+// smoother passes (the fine focusing stage: the dominant kernel launches).
+static void mg_solve_batch(Worker &w, std::vector<MgTask *> &tasks, bool use_graph,
+                           bool profile_level0)
 {
     const size_t g = (size_t)std::max(1, tuning().batch_solves);
     for (size_t a = 0; a < tasks.size(); a += g) {
         std::vector<MgTask *> part(tasks.begin() + a, tasks.begin() + std::min(tasks.size(), a + g));
-        mg_solve_group(w, part, use_graph);
+        mg_solve_group(w, part, use_graph, profile_level0);
     }
 }
 
-static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_graph)
+static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_graph,
+                           bool profile_level0)
 {
     std::vector<MgTask *> mgt;
     for (MgTask *t : tasks) {
@@ -302,7 +307,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     const int S = (int)mgt.size();
     if (S == 0) return;
     ProfileStats &prof = profile_stats();
-    const bool profiling = prof.enabled;
+    const bool profiling = prof.enabled && profile_level0;
     if (profiling) use_graph = false;
 
     std::vector<TaskWork> W(S);
@@ -622,6 +627,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         prof.total_ms += ms_tot;
         prof.launches += (int64_t)evs.size();
         prof.alg_bytes += bytes;
+        prof.grid_threads = (int64_t)sblocks[0] * S * 256;
     }
     for (int s = 0; s < S; ++s) {
         TaskWork &tw = W[s];
@@ -779,7 +785,7 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
             tasks.push_back(&sd.ct);
         }
     }
-    if (!tasks.empty()) mg_solve_batch(w, tasks, batched);
+    if (!tasks.empty()) mg_solve_batch(w, tasks, batched, false);
     // coarse validity (pb_energy.rs:188-194), then fine boundary values
     {
         std::vector<DBuf<double>> parts;
@@ -842,7 +848,7 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
             tasks.push_back(&sd.ft);
         }
     }
-    if (!tasks.empty()) mg_solve_batch(w, tasks, batched);
+    if (!tasks.empty()) mg_solve_batch(w, tasks, batched, true);
     // energies, compute_elec_energy (pb_solver.rs:1174-1186)
     {
         const size_t nres = 2 * act.size() + 1;
