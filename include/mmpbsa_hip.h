@@ -231,6 +231,47 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
                      const mmpbsa_sa_params *sa, mmpbsa_residue_contrib *rec_out,
                      mmpbsa_residue_contrib *lig_out);
 
+/* ---- serving (no counterpart in the reference) ---------------------------
+ * A server owns one worker thread and HIP stream.  Requests (frames of one
+ * split, copied at submission) wait in a queue; each scheduling round takes
+ * frames round-robin from the waiting requests (continuous batching: new
+ * requests join the next round) while the sum of their device-memory
+ * estimates fits hbm_budget_bytes, and runs them through the batched path of
+ * mmpbsa_frames_energy, frames of different complexes sharing solver
+ * launches.  A frame's result depends only on its own inputs: it is identical
+ * alone and in any mix.  Splits (and their contexts) must outlive their
+ * requests.                                                                 */
+typedef struct mmpbsa_server mmpbsa_server;
+
+typedef struct mmpbsa_server_params {
+    int64_t hbm_budget_bytes;   /* admission limit; 0 = 3/4 of free device memory */
+    int32_t max_round_frames;   /* frames per scheduling round (default 16)      */
+} mmpbsa_server_params;
+
+typedef struct mmpbsa_request_stats {
+    double queue_ms;            /* submission -> first frame scheduled           */
+    double total_ms;            /* submission -> last frame finished             */
+    int32_t n_frames;
+    int32_t status;             /* MMPBSA_OK or the error of the request         */
+} mmpbsa_request_stats;
+
+void mmpbsa_server_params_default(mmpbsa_server_params *p);
+int mmpbsa_server_create(const mmpbsa_server_params *p, int32_t device, mmpbsa_server **out);
+/* Finishes the requests already submitted, then stops. */
+int mmpbsa_server_destroy(mmpbsa_server *srv);
+/* MMPBSA_ERR_TOO_LARGE when one frame alone exceeds the budget. */
+int mmpbsa_server_submit(mmpbsa_server *srv, mmpbsa_split *split, const double *xyz_frames,
+                         int32_t n_frames, int32_t method, const mmpbsa_gb_params *gb,
+                         const mmpbsa_pb_params *pb, const mmpbsa_sa_params *sa,
+                         uint64_t *ticket);
+/* MMPBSA_ERR_NOT_READY until every frame is done; then copies n_frames
+ * results to out (may be null), fills stats (may be null) and releases the
+ * ticket.  wait blocks instead of returning NOT_READY. */
+int mmpbsa_server_poll(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
+                       mmpbsa_request_stats *stats);
+int mmpbsa_server_wait(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
+                       mmpbsa_request_stats *stats);
+
 /* ---- device memory --------------------------------------------------------
  * For callers that stage inputs in HBM without another GPU runtime (no
  * counterpart in the reference).  Allocation is on `device`; copies are

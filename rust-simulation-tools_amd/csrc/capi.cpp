@@ -2,10 +2,15 @@
 // receptor/ligand splits, frame energies, batched frames, decomposition.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <thread>
+#include <unordered_map>
 
 #include "decomp.h"
 #include "engine.h"
@@ -128,50 +133,58 @@ void stage_frame(Worker &w, mmpbsa_split *s, const double *xyz, FrameCoords &f, 
     }
 }
 
-double polar_of(Worker &w, const DevTopo &t, const double *d, const double *h, int method,
-                const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb)
-{
-    if (method == MMPBSA_METHOD_PB) {
-        mmpbsa_pb_result r;
-        run_pb(w, t, d, h, pb, &r);
-        return r.total;
-    }
-    return run_gb(w, t, d, gb, nullptr, nullptr);
-}
-
 double mm_total(const mmpbsa_mm_terms &m)
 {
     return m.bond + m.angle + m.dihedral + m.vdw + m.elec + m.vdw_14 + m.elec_14;
 }
 
-// compute_frame_energy (binding.rs:333-415)
-// compute_frame_energy (binding.rs:333-415) for nf frames.  With batched, the
-// PB solves of all frames and subsystems advance together (shared launches,
-// graph-replayed cycles); otherwise each solve runs on its own.
-void frames_group(Worker &w, mmpbsa_split *s, const double *xyz, int nf, size_t stride,
-                  int method, const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb,
-                  const mmpbsa_sa_params &sa, mmpbsa_frame_terms *out, bool batched,
-                  bool on_device = false)
+// One frame of one split with its method and parameters.
+struct FrameItem {
+    mmpbsa_split *s = nullptr;
+    const double *xyz = nullptr;  // all atoms of the caller's topology (host or device)
+    int method = MMPBSA_METHOD_GB;
+    mmpbsa_gb_params gb{};
+    mmpbsa_pb_params pb{};
+    mmpbsa_sa_params sa{};
+    mmpbsa_frame_terms *out = nullptr;
+};
+
+// compute_frame_energy (binding.rs:333-415) for a list of frames, possibly of
+// different splits, methods and parameters.  With batched, the PB solves of
+// all frames and subsystems advance together (shared launches, graph-replayed
+// cycles); otherwise each solve runs on its own.  Every frame's result depends
+// only on its own inputs, so it is the same alone and in any batch.
+void run_items(Worker &w, const std::vector<FrameItem> &items, bool batched, bool on_device)
 {
+    const size_t nf = items.size();
     std::vector<FrameCoords> fc(nf);
-    for (int f = 0; f < nf; ++f) stage_frame(w, s, xyz + stride * f, fc[f], on_device);
-    const DevTopo *tops[3] = {&s->ctop, &s->rtop, &s->ltop};
-    std::vector<mmpbsa_pb_result> pr(3 * (size_t)nf);
-    if (method == MMPBSA_METHOD_PB) {
-        std::vector<PbJob> jobs(3 * (size_t)nf);
-        for (int f = 0; f < nf; ++f) {
-            const double *ds[3] = {fc[f].dc, fc[f].r.get(), fc[f].l.get()};
-            const double *hs[3] = {fc[f].hcp, on_device ? nullptr : fc[f].hr.data(),
-                                   on_device ? nullptr : fc[f].hl.data()};
-            for (int k = 0; k < 3; ++k) {
-                PbJob &j = jobs[3 * (size_t)f + k];
-                j.t = tops[k];
-                j.d_xyz = ds[k];
-                j.h_xyz = hs[k];
-                j.p = pb;
-                j.out = &pr[3 * (size_t)f + k];
-            }
+    for (size_t f = 0; f < nf; ++f) stage_frame(w, items[f].s, items[f].xyz, fc[f], on_device);
+    auto tops = [&](size_t f, int k) -> const DevTopo * {
+        mmpbsa_split *s = items[f].s;
+        return k == 0 ? &s->ctop : (k == 1 ? &s->rtop : &s->ltop);
+    };
+    auto dptr = [&](size_t f, int k) -> const double * {
+        return k == 0 ? fc[f].dc : (k == 1 ? fc[f].r.get() : fc[f].l.get());
+    };
+    auto hptr = [&](size_t f, int k) -> const double * {
+        if (on_device) return nullptr;
+        return k == 0 ? fc[f].hcp : (k == 1 ? fc[f].hr.data() : fc[f].hl.data());
+    };
+    std::vector<mmpbsa_pb_result> pr(3 * nf);
+    std::vector<PbJob> jobs;
+    for (size_t f = 0; f < nf; ++f) {
+        if (items[f].method != MMPBSA_METHOD_PB) continue;
+        for (int k = 0; k < 3; ++k) {
+            PbJob j;
+            j.t = tops(f, k);
+            j.d_xyz = dptr(f, k);
+            j.h_xyz = hptr(f, k);
+            j.p = items[f].pb;
+            j.out = &pr[3 * f + k];
+            jobs.push_back(j);
         }
+    }
+    if (!jobs.empty()) {
         if (batched) {
             run_pb_jobs(w, jobs, true);
         } else {
@@ -181,20 +194,19 @@ void frames_group(Worker &w, mmpbsa_split *s, const double *xyz, int nf, size_t 
             }
         }
     }
-    for (int f = 0; f < nf; ++f) {
-        const double *ds[3] = {fc[f].dc, fc[f].r.get(), fc[f].l.get()};
-        const double *hs[3] = {fc[f].hcp, on_device ? nullptr : fc[f].hr.data(),
-                               on_device ? nullptr : fc[f].hl.data()};
-        mmpbsa_frame_terms *o = out + f;
+    for (size_t f = 0; f < nf; ++f) {
+        const FrameItem &it = items[f];
+        mmpbsa_frame_terms *o = it.out;
         mmpbsa_subsystem_energy *e[3] = {&o->complex_, &o->receptor, &o->ligand};
         for (int k = 0; k < 3; ++k) {
             mmpbsa_mm_terms mm;
-            run_mm(w, *tops[k], ds[k], &mm);
+            run_mm(w, *tops(f, k), dptr(f, k), &mm);
             e[k]->mm = mm_total(mm);
-            e[k]->polar = method == MMPBSA_METHOD_PB ? pr[3 * (size_t)f + k].total
-                                                     : run_gb(w, *tops[k], ds[k], gb, nullptr, nullptr);
+            e[k]->polar = it.method == MMPBSA_METHOD_PB
+                              ? pr[3 * f + k].total
+                              : run_gb(w, *tops(f, k), dptr(f, k), it.gb, nullptr, nullptr);
             double tsasa = 0.0, tsa = 0.0;
-            run_sa(w, *tops[k], ds[k], hs[k], sa, nullptr, nullptr, &tsasa, &tsa);
+            run_sa(w, *tops(f, k), dptr(f, k), hptr(f, k), it.sa, nullptr, nullptr, &tsasa, &tsa);
             e[k]->sa = tsa;
         }
         o->delta_mm = o->complex_.mm - o->receptor.mm - o->ligand.mm;
@@ -204,12 +216,174 @@ void frames_group(Worker &w, mmpbsa_split *s, const double *xyz, int nf, size_t 
     }
 }
 
+// nf frames of one split at xyz + stride * f
+void frames_group(Worker &w, mmpbsa_split *s, const double *xyz, int nf, size_t stride,
+                  int method, const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb,
+                  const mmpbsa_sa_params &sa, mmpbsa_frame_terms *out, bool batched,
+                  bool on_device = false)
+{
+    std::vector<FrameItem> items(nf);
+    for (int f = 0; f < nf; ++f) {
+        FrameItem &it = items[f];
+        it.s = s;
+        it.xyz = xyz + stride * f;
+        it.method = method;
+        it.gb = gb;
+        it.pb = pb;
+        it.sa = sa;
+        it.out = out + f;
+    }
+    run_items(w, items, batched, on_device);
+}
+
 void frame_energy(Worker &w, mmpbsa_split *s, const double *xyz, int method,
                   const mmpbsa_gb_params &gb, const mmpbsa_pb_params &pb,
                   const mmpbsa_sa_params &sa, mmpbsa_frame_terms *out)
 {
     frames_group(w, s, xyz, 1, 0, method, gb, pb, sa, out, false);
 }
+
+// ---- serving ------------------------------------------------------------------
+using Clock = std::chrono::steady_clock;
+
+struct ServerRequest {
+    uint64_t ticket = 0;
+    mmpbsa_split *split = nullptr;
+    std::vector<double> xyz;          // the caller's frames, copied at submission
+    int n_frames = 0;
+    int method = MMPBSA_METHOD_GB;
+    mmpbsa_gb_params gb{};
+    mmpbsa_pb_params pb{};
+    mmpbsa_sa_params sa{};
+    std::vector<mmpbsa_frame_terms> out;
+    int next = 0;                      // first frame not yet scheduled
+    int done = 0;                      // frames finished
+    double frame_bytes = 0.0;          // admission estimate of one frame in flight
+    Clock::time_point t_submit, t_start, t_end;
+    bool started = false;
+    int status = MMPBSA_OK;
+    std::string err;
+};
+
+// Device bytes one frame holds while its PB solves run: per grid node of each
+// subsystem's fine and coarse grids, the FP64 maps, charge density, two
+// potentials per medium, the FP32 multigrid arrays of both solves and the
+// geometry scratch (an upper estimate used for admission only).
+constexpr double kPbBytesPerNode = 320.0;
+
+double frame_bytes_estimate(const mmpbsa_split *s, const double *xyz, int method,
+                            const mmpbsa_pb_params &p)
+{
+    const size_t n = s->csel.size();
+    if (method != MMPBSA_METHOD_PB || n == 0) return 600.0 * (double)n * 3.0 + 4096.0;
+    std::vector<double> hc = s->identity ? std::vector<double>(xyz, xyz + 3 * n)
+                                         : gather_host(xyz, s->csel);
+    double mn[3], mx[3];
+    bbox((int)n, hc.data(), mn, mx);
+    GridDesc f = auto_grid(mn, mx, p.grid_spacing, p.grid_buffer);
+    double nodes = (double)f.size();
+    if (p.fine_dims[0] > 0) {
+        const double a = p.fine_dims[0], b = p.fine_dims[1] > 0 ? p.fine_dims[1] : a,
+                     c = p.fine_dims[2] > 0 ? p.fine_dims[2] : a;
+        nodes = a * b * c;
+    }
+    if (p.fillratio > 0.0 && p.fscale > 1)
+        nodes += (double)coarse_grid(mn, mx, p.grid_spacing, p.fillratio, p.fscale).size();
+    // the receptor and ligand grids are no larger than the complex's
+    return 3.0 * kPbBytesPerNode * nodes;
+}
+
+}  // namespace
+
+struct mmpbsa_server {
+    int device = 0;
+    double budget = 0.0;
+    int max_round = 16;
+    std::unique_ptr<Worker> w;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServerRequest *> pending;   // requests with frames not yet scheduled
+    std::unordered_map<uint64_t, std::unique_ptr<ServerRequest>> reqs;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    double in_flight = 0.0;
+    std::thread th;
+    void loop();
+};
+
+// Continuous batching: each round takes frames round-robin from the pending
+// requests (one frame per request per pass, new requests join at the next
+// round) while the round's admission estimate fits the HBM budget, then runs
+// them through the batched path.
+void mmpbsa_server::loop()
+{
+    (void)hipSetDevice(device);
+    for (;;) {
+        std::vector<std::pair<ServerRequest *, int>> picks;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+            if (pending.empty()) return;  // stop requested and nothing left
+            double bytes = 0.0;
+            bool full = false;
+            while (!full && !pending.empty() && (int)picks.size() < max_round) {
+                bool took = false;
+                for (auto it = pending.begin(); it != pending.end() && (int)picks.size() < max_round;) {
+                    ServerRequest *r = *it;
+                    if (!picks.empty() && bytes + r->frame_bytes > budget) {
+                        full = true;
+                        break;
+                    }
+                    if (!r->started) {
+                        r->started = true;
+                        r->t_start = Clock::now();
+                    }
+                    picks.push_back({r, r->next++});
+                    bytes += r->frame_bytes;
+                    took = true;
+                    if (r->next >= r->n_frames)
+                        it = pending.erase(it);
+                    else
+                        ++it;
+                }
+                if (!took) break;
+            }
+            in_flight = bytes;
+        }
+        std::vector<FrameItem> items(picks.size());
+        for (size_t k = 0; k < picks.size(); ++k) {
+            ServerRequest *r = picks[k].first;
+            const int f = picks[k].second;
+            const size_t stride = 3 * (size_t)r->split->ctx->top.h.n;
+            FrameItem &it = items[k];
+            it.s = r->split;
+            it.xyz = r->xyz.data() + stride * f;
+            it.method = r->method;
+            it.gb = r->gb;
+            it.pb = r->pb;
+            it.sa = r->sa;
+            it.out = &r->out[f];
+        }
+        std::string err;
+        const int st = guarded(nullptr, [&] { run_items(*w, items, true, false); });
+        if (st != MMPBSA_OK) err = g_err;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            in_flight = 0.0;
+            for (auto &pk : picks) {
+                ServerRequest *r = pk.first;
+                if (st != MMPBSA_OK && r->status == MMPBSA_OK) {
+                    r->status = st;
+                    r->err = err;
+                }
+                if (++r->done == r->n_frames) r->t_end = Clock::now();
+            }
+        }
+        cv_done.notify_all();
+    }
+}
+
+namespace {
 
 void check_xyz(const double *xyz) {
     if (!xyz) throw DeviceError(MMPBSA_ERR_INVALID_ARG, "null coordinates");
@@ -620,6 +794,143 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
         fill(s->rec_res, s->rec_loc, rec_out);
         fill(s->lig_res, s->lig_loc, lig_out);
     });
+}
+
+void mmpbsa_server_params_default(mmpbsa_server_params *p)
+{
+    if (!p) return;
+    p->hbm_budget_bytes = 0;
+    p->max_round_frames = 16;
+}
+
+int mmpbsa_server_create(const mmpbsa_server_params *params, int32_t device, mmpbsa_server **out)
+{
+    if (!out) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        require_device();
+        mmpbsa_server_params p;
+        mmpbsa_server_params_default(&p);
+        if (params) p = *params;
+        if (p.max_round_frames <= 0 || p.hbm_budget_bytes < 0)
+            throw DeviceError(MMPBSA_ERR_INVALID_ARG, "bad server parameters");
+        HIP_OK(hipSetDevice(device));
+        auto srv = std::make_unique<mmpbsa_server>();
+        srv->device = device;
+        srv->max_round = p.max_round_frames;
+        if (p.hbm_budget_bytes > 0) {
+            srv->budget = (double)p.hbm_budget_bytes;
+        } else {
+            size_t free_b = 0, total_b = 0;
+            HIP_OK(hipMemGetInfo(&free_b, &total_b));
+            srv->budget = 0.75 * (double)free_b;
+        }
+        srv->w = std::make_unique<Worker>();
+        srv->th = std::thread([s = srv.get()] { s->loop(); });
+        *out = srv.release();
+    });
+}
+
+int mmpbsa_server_destroy(mmpbsa_server *srv)
+{
+    if (!srv) return MMPBSA_OK;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+    }
+    srv->cv_work.notify_all();
+    if (srv->th.joinable()) srv->th.join();
+    delete srv;
+    return MMPBSA_OK;
+}
+
+int mmpbsa_server_submit(mmpbsa_server *srv, mmpbsa_split *split, const double *xyz_frames,
+                         int32_t n_frames, int32_t method, const mmpbsa_gb_params *gb,
+                         const mmpbsa_pb_params *pb, const mmpbsa_sa_params *sa, uint64_t *ticket)
+{
+    if (!srv || !split || !ticket || n_frames <= 0)
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "invalid argument");
+    return guarded(nullptr, [&] {
+        check_xyz(xyz_frames);
+        auto r = std::make_unique<ServerRequest>();
+        r->split = split;
+        r->n_frames = n_frames;
+        r->method = method;
+        mmpbsa_gb_params_default(&r->gb);
+        mmpbsa_pb_params_default(&r->pb);
+        mmpbsa_sa_params_default(&r->sa);
+        if (gb) r->gb = *gb;
+        if (pb) r->pb = *pb;
+        if (sa) r->sa = *sa;
+        const size_t stride = 3 * (size_t)split->ctx->top.h.n;
+        r->xyz.assign(xyz_frames, xyz_frames + stride * (size_t)n_frames);
+        r->out.assign(n_frames, mmpbsa_frame_terms{});
+        r->frame_bytes = frame_bytes_estimate(split, xyz_frames, method, r->pb);
+        if (r->frame_bytes > srv->budget)
+            throw DeviceError(MMPBSA_ERR_TOO_LARGE, "one frame needs more device memory (" +
+                                                        std::to_string((int64_t)r->frame_bytes) +
+                                                        " B) than the server's budget");
+        r->t_submit = Clock::now();
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (srv->stop) throw DeviceError(MMPBSA_ERR_INVALID_ARG, "server is shutting down");
+        r->ticket = srv->next_ticket++;
+        *ticket = r->ticket;
+        srv->pending.push_back(r.get());
+        srv->reqs[r->ticket] = std::move(r);
+        srv->cv_work.notify_one();
+    });
+}
+
+}  // extern "C"
+
+namespace {
+// Copy out a finished request and release its ticket (caller holds srv->mu).
+int finish_request(mmpbsa_server *srv, std::unordered_map<uint64_t,
+                   std::unique_ptr<ServerRequest>>::iterator it, mmpbsa_frame_terms *out,
+                   mmpbsa_request_stats *stats)
+{
+    ServerRequest &r = *it->second;
+    auto ms = [](Clock::time_point a, Clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    if (stats) {
+        stats->queue_ms = ms(r.t_submit, r.t_start);
+        stats->total_ms = ms(r.t_submit, r.t_end);
+        stats->n_frames = r.n_frames;
+        stats->status = r.status;
+    }
+    const int st = r.status;
+    const std::string err = r.err;
+    if (out && st == MMPBSA_OK) std::memcpy(out, r.out.data(), sizeof(mmpbsa_frame_terms) * r.n_frames);
+    srv->reqs.erase(it);
+    if (st != MMPBSA_OK) return fail(nullptr, st, err);
+    return MMPBSA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mmpbsa_server_poll(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
+                       mmpbsa_request_stats *stats)
+{
+    if (!srv) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null server");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(ticket);
+    if (it == srv->reqs.end()) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "unknown ticket");
+    if (it->second->done < it->second->n_frames) return MMPBSA_ERR_NOT_READY;
+    return finish_request(srv, it, out, stats);
+}
+
+int mmpbsa_server_wait(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
+                       mmpbsa_request_stats *stats)
+{
+    if (!srv) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null server");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(ticket);
+    if (it == srv->reqs.end()) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "unknown ticket");
+    ServerRequest *r = it->second.get();
+    srv->cv_done.wait(lk, [&] { return r->done >= r->n_frames; });
+    return finish_request(srv, srv->reqs.find(ticket), out, stats);
 }
 
 int mmpbsa_device_alloc(int32_t device, int64_t bytes, void **out)

@@ -15,7 +15,8 @@ import math
 import numpy as np
 
 from . import _abi
-from ._abi import (FrameTermsC, GbParamsC, MmTermsC, PbParamsC, PbResultC, ResidueContribC,
+from ._abi import (FrameTermsC, GbParamsC, MmTermsC, PbParamsC, PbResultC, RequestStatsC,
+                   ResidueContribC,
                    SaParamsC, load_library)
 from .topology import AmberTopology, extract_coords
 from .topology import extract_subtopology as _extract_subtopology
@@ -444,6 +445,81 @@ def compute_binding_energy_device_frames(topology: AmberTopology, frames_angstro
                                              C.byref(p), C.byref(a), 1 if batched else 0, out),
            c.lib, c.h)
     return BindingResult([FrameEnergy(tt) for tt in out])
+
+
+class RequestStats:
+    """Latency of one served request (milliseconds from submission)."""
+
+    def __init__(self, st: RequestStatsC):
+        self.queue_ms = st.queue_ms
+        self.total_ms = st.total_ms
+        self.n_frames = st.n_frames
+
+
+class Server:
+    """Continuous-batching MM-PBSA server (extension: the reference has no
+    served path).  submit() copies the frames ([F, N, 3] nm) and returns a
+    ticket at once; a worker thread schedules frames of all waiting requests
+    round-robin into shared batched rounds under an HBM budget; poll() / wait()
+    return (BindingResult, RequestStats).  A frame's result is identical alone
+    and in any mix of requests."""
+
+    def __init__(self, hbm_budget_bytes=0, max_round_frames=16, device=0):
+        self.lib = load_library()
+        p = _abi.ServerParamsC()
+        self.lib.mmpbsa_server_params_default(C.byref(p))
+        p.hbm_budget_bytes = int(hbm_budget_bytes)
+        p.max_round_frames = int(max_round_frames)
+        self.h = C.c_void_p()
+        _check(self.lib.mmpbsa_server_create(C.byref(p), device, C.byref(self.h)), self.lib)
+        self._keep = {}   # ticket -> (context, frame count): contexts outlive requests
+
+    def submit(self, topology: AmberTopology, frames, receptor_residues, ligand_residues,
+               gb_params=None, pb_params=None, sa_params=None) -> int:
+        c = _ctx(topology)
+        f = np.asarray(frames, dtype=np.float64)
+        if f.ndim == 2:
+            f = f[None]
+        if f.ndim != 3 or f.shape[1] != topology.n_atoms or f.shape[2] != 3 or f.shape[0] == 0:
+            raise ValueError("frames must be an (F, N, 3) array with F > 0")
+        x = np.ascontiguousarray(f * 10.0)
+        s = c.split(receptor_residues, ligand_residues)
+        m, g, p = _method(gb_params, pb_params)
+        a = (sa_params or SaParams())._c
+        t = C.c_uint64()
+        _check(self.lib.mmpbsa_server_submit(self.h, s, _dp(x), f.shape[0], m, C.byref(g),
+                                             C.byref(p), C.byref(a), C.byref(t)), self.lib)
+        self._keep[t.value] = (c, f.shape[0])
+        return t.value
+
+    def _finish(self, fn, ticket):
+        c, n = self._keep[ticket]
+        out = (FrameTermsC * n)()
+        st = RequestStatsC()
+        rc = fn(self.h, C.c_uint64(ticket), out, C.byref(st))
+        if rc == _abi.MMPBSA_ERR_NOT_READY:
+            return None
+        del self._keep[ticket]
+        _check(rc, self.lib)
+        return BindingResult([FrameEnergy(t) for t in out]), RequestStats(st)
+
+    def poll(self, ticket: int):
+        """(BindingResult, RequestStats), or None while frames are pending."""
+        return self._finish(self.lib.mmpbsa_server_poll, ticket)
+
+    def wait(self, ticket: int):
+        return self._finish(self.lib.mmpbsa_server_wait, ticket)
+
+    def close(self):
+        if self.h:
+            self.lib.mmpbsa_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def decompose_binding_energy(topology: AmberTopology, coords, receptor_residues,

@@ -17,6 +17,7 @@ c_int32_p = C.POINTER(C.c_int32)
 c_int64_p = C.POINTER(C.c_int64)
 
 MMPBSA_OK = 0
+MMPBSA_ERR_NOT_READY = 6
 ERRORS = {
     1: "invalid argument",
     2: "residue index out of range",
@@ -159,6 +160,15 @@ class ResidueContribC(C.Structure):
     ]
 
 
+class ServerParamsC(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_round_frames", C.c_int32)]
+
+
+class RequestStatsC(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("total_ms", C.c_double), ("n_frames", C.c_int32),
+                ("status", C.c_int32)]
+
+
 # Every symbol include/mmpbsa_hip.h declares, with its ctypes signature.
 P = C.c_void_p
 SIGNATURES = {
@@ -191,6 +201,16 @@ SIGNATURES = {
                                               C.POINTER(FrameTermsC)]),
     "mmpbsa_decompose": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), C.POINTER(SaParamsC),
                                    C.POINTER(ResidueContribC), C.POINTER(ResidueContribC)]),
+    "mmpbsa_server_params_default": (None, [C.POINTER(ServerParamsC)]),
+    "mmpbsa_server_create": (C.c_int, [C.POINTER(ServerParamsC), C.c_int32, C.POINTER(P)]),
+    "mmpbsa_server_destroy": (C.c_int, [P]),
+    "mmpbsa_server_submit": (C.c_int, [P, P, c_double_p, C.c_int32, C.c_int32,
+                                       C.POINTER(GbParamsC), C.POINTER(PbParamsC),
+                                       C.POINTER(SaParamsC), C.POINTER(C.c_uint64)]),
+    "mmpbsa_server_poll": (C.c_int, [P, C.c_uint64, C.POINTER(FrameTermsC),
+                                     C.POINTER(RequestStatsC)]),
+    "mmpbsa_server_wait": (C.c_int, [P, C.c_uint64, C.POINTER(FrameTermsC),
+                                     C.POINTER(RequestStatsC)]),
     "mmpbsa_device_alloc": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
     "mmpbsa_device_free": (C.c_int, [C.c_void_p]),
     "mmpbsa_device_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),

@@ -1,0 +1,85 @@
+"""Serving loop (continuous batching under an HBM budget): a request's
+results are identical alone and mixed with other complexes and methods, the
+budget is enforced at admission, and every ticket completes."""
+import time
+
+import numpy as np
+import pytest
+
+import rust_simulation_tools as rst
+from rust_simulation_tools import synthetic
+
+pytestmark = pytest.mark.gpu
+
+PB_SMALL = dict(grid_spacing=0.8, grid_buffer=4.0, solute_dielectric=4.0,
+                solvent_dielectric=80.0, salt_concentration=0.15)
+KEYS = ("complex_mm", "complex_polar", "receptor_polar", "ligand_polar", "complex_sa",
+        "receptor_sa", "ligand_sa", "delta_total")
+
+
+def _system(n, n_lig, idx, frames):
+    seed = synthetic.seed_for(3, idx)
+    top, base, rec, lig = synthetic.build_complex(n, n_lig, None, seed)
+    f = np.stack([synthetic.noisy_frame(base, seed, k) for k in range(frames)]) / 10.0
+    return top, f, rec, lig
+
+
+@pytest.fixture(scope="module")
+def systems():
+    return [_system(900, 80, 0, 3), _system(1400, 120, 1, 2), _system(600, 60, 2, 2)]
+
+
+def _same(a, b):
+    for fa, fb in zip(a.frames(), b.frames()):
+        for k in KEYS:
+            assert getattr(fa, k) == getattr(fb, k), k
+
+
+def test_alone_vs_mix_identity(systems):
+    (ta, fa, ra, la), (tb, fb, rb, lb), (tc, fc, rc, lc) = systems
+    pb = rst.PbParams(**PB_SMALL)
+    srv = rst.Server(max_round_frames=16)
+    try:
+        alone, st = srv.wait(srv.submit(ta, fa, ra, la, pb_params=pb))
+        assert st.n_frames == 3 and st.total_ms >= st.queue_ms >= 0.0
+        # the same request among a PB request of another complex and a GB one
+        t1 = srv.submit(tb, fb, rb, lb, pb_params=pb)
+        t2 = srv.submit(ta, fa, ra, la, pb_params=pb)
+        t3 = srv.submit(tc, fc, rc, lc)
+        mixed, _ = srv.wait(t2)
+        _same(alone, mixed)
+        # and both equal the direct batched path
+        direct = rst.compute_binding_energy_frames(ta, fa, ra, la, pb_params=pb, batched=True)
+        _same(alone, direct)
+        other, _ = srv.wait(t1)
+        _same(other, rst.compute_binding_energy_frames(tb, fb, rb, lb, pb_params=pb))
+        gb, _ = srv.wait(t3)
+        _same(gb, rst.compute_binding_energy_frames(tc, fc, rc, lc))
+    finally:
+        srv.close()
+
+
+def test_budget_admission(systems):
+    ta, fa, ra, la = systems[0]
+    pb = rst.PbParams(**PB_SMALL)
+    tiny = rst.Server(hbm_budget_bytes=1024)
+    try:
+        with pytest.raises(RuntimeError, match="budget"):
+            tiny.submit(ta, fa, ra, la, pb_params=pb)
+    finally:
+        tiny.close()
+    # a budget of a few frames (each ~0.1 GB by the admission estimate) still
+    # completes every request, with the same results
+    one = rst.Server(hbm_budget_bytes=250 << 20)
+    ref = rst.compute_binding_energy_frames(ta, fa, ra, la, pb_params=pb)
+    try:
+        tickets = [one.submit(ta, fa, ra, la, pb_params=pb) for _ in range(2)]
+        for t in tickets:
+            res = None
+            while res is None:
+                res = one.poll(t)
+                if res is None:
+                    time.sleep(0.01)
+            _same(res[0], ref)
+    finally:
+        one.close()
