@@ -15,6 +15,7 @@ host cores.
 import argparse
 import csv
 import json
+import math
 import os
 import shutil
 import subprocess
@@ -37,6 +38,100 @@ CFG1_PB = dict(grid_spacing=0.5, solute_dielectric=4.0, solvent_dielectric=80.0,
                salt_concentration=0.15, fine_dims=(161, 161, 161))
 CFG0_PB = dict(grid_spacing=0.6, solute_dielectric=4.0, solvent_dielectric=80.0,
                salt_concentration=0.15, fine_dims=(65, 65, 65))
+# many-complex workloads: auto_grid at 0.5 A with a 10 A buffer per subsystem
+AUTO_PB = dict(grid_spacing=0.5, grid_buffer=10.0, solute_dielectric=4.0,
+               solvent_dielectric=80.0, salt_concentration=0.15)
+
+
+def _complexes(rst, synthetic, config, sizes, frames, rng):
+    """(topology, frames nm, receptor residues, ligand residues) per size, with
+    a ligand fraction drawn from U(3%, 10%)."""
+    out = []
+    for i, (n, nf) in enumerate(zip(sizes, frames)):
+        seed = synthetic.seed_for(config, i)
+        n_lig = max(16, int(round(n * rng.uniform(0.03, 0.10) / 16.0)) * 16)
+        top, base, rec, lig = synthetic.build_complex(int(n), n_lig, None, seed)
+        f = np.stack([synthetic.noisy_frame(base, seed, k) for k in range(nf)]) / 10.0
+        out.append((top, f, rec, lig))
+    return out
+
+
+def uneven_leg(rst, synthetic):
+    """Eight complexes, N log-spaced over [3k, 60k], two frames each: all in one
+    server (their solves share rounds) against each complex batched alone."""
+    rng = np.random.Generator(np.random.PCG64(synthetic.seed_for(2, 900)))
+    sizes = np.round(np.exp(np.linspace(math.log(3000), math.log(60000), 8))).astype(int)
+    cx = _complexes(rst, synthetic, 2, sizes, [2] * 8, rng)
+    pb = rst.PbParams(**AUTO_PB)
+    for top, f, rec, lig in cx:  # contexts, splits, first-use costs
+        rst.compute_binding_energy_frames(top, f[:1], rec, lig, pb_params=pb)
+    ts = time.perf_counter()
+    for top, f, rec, lig in cx:
+        rst.compute_binding_energy_frames(top, f, rec, lig, pb_params=pb, batched=True)
+    alone = time.perf_counter() - ts
+    srv = rst.Server(hbm_budget_bytes=200 << 30, max_round_frames=16)
+    try:
+        top, f, rec, lig = cx[-1]  # warm the server's device pool with the largest frame
+        srv.wait(srv.submit(top, f[:1], rec, lig, pb_params=pb))
+        ts = time.perf_counter()
+        tickets = [srv.submit(top, f, rec, lig, pb_params=pb) for top, f, rec, lig in cx]
+        for t in tickets:
+            srv.wait(t)
+        mixed = time.perf_counter() - ts
+    finally:
+        srv.close()
+    nfr = 2 * len(cx)
+    return {"workload": "8 complexes, N = " + ", ".join(str(int(n)) for n in sizes) +
+                        "; 2 frames each; auto_grid h 0.5 A, buffer 10 A",
+            "mixed_frames_per_s": nfr / mixed, "one_complex_at_a_time_frames_per_s": nfr / alone}
+
+
+def serving_leg(rst, synthetic, check_oracle):
+    """Poisson arrivals (50/s) of 24 requests, N log-uniform over [5k, 25k],
+    1-8 frames each, into one server under a 200 GB budget: request latency
+    (submission to last frame) and throughput; one sampled frame is checked
+    against the FP64 oracle."""
+    rng = np.random.Generator(np.random.PCG64(synthetic.seed_for(3, 900)))
+    n_req = 24
+    sizes = np.round(np.exp(rng.uniform(math.log(5000), math.log(25000), n_req))).astype(int)
+    nframes = rng.integers(1, 9, n_req)
+    arrivals = np.cumsum(rng.exponential(1.0 / 50.0, n_req))
+    cx = _complexes(rst, synthetic, 3, sizes, nframes, rng)
+    pb = rst.PbParams(**AUTO_PB)
+    for top, f, rec, lig in cx:  # register topologies (contexts, splits) up front
+        rst.compute_binding_energy_frames(top, f[:1], rec, lig)
+    srv = rst.Server(hbm_budget_bytes=200 << 30, max_round_frames=16)
+    lat, results = [], []
+    try:
+        big = int(np.argmax(sizes))  # warm the server's device pool
+        srv.wait(srv.submit(cx[big][0], cx[big][1][:1], cx[big][2], cx[big][3], pb_params=pb))
+        t0 = time.perf_counter()
+        tickets = []
+        for k, (top, f, rec, lig) in enumerate(cx):
+            dt = arrivals[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(top, f, rec, lig, pb_params=pb))
+        for t in tickets:
+            r, st = srv.wait(t)
+            lat.append(st.total_ms)
+            results.append(r)
+        span = time.perf_counter() - t0
+    finally:
+        srv.close()
+    out = {"workload": f"{n_req} requests, Poisson 50/s, N log-uniform [5k, 25k], 1-8 frames, "
+                       "auto_grid h 0.5 A buffer 10 A, HBM budget 200 GB",
+           "frames": int(nframes.sum()), "frames_per_s": float(nframes.sum()) / span,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(np.max(lat))}}
+    if check_oracle:
+        from oracle import oracle as O
+        k = int(np.argmin(sizes))  # a sampled request: the smallest system, its first frame
+        top, f, rec, lig = cx[k]
+        o = O.frame_energy(top, f[0] * 10.0, rec, lig, method=1, pb=O.pb_params(**AUTO_PB))
+        out["sampled_dG_err_vs_fp64_oracle"] = abs(results[k].frames()[0].delta_total -
+                                                   o["delta_total"])
+    return out
 
 
 def measure_traffic(frames_per_step, grid_threads):
@@ -92,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip roofline and small-system legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-many", action="store_true",
+                    help="skip the many-complex legs (uneven batch, serving)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs behind roofline.traffic")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -189,6 +286,10 @@ def main():
         extras["small_complex_2500_atoms"] = {"batched_frames_per_s": fb,
                                               "plain_frames_per_s": fp,
                                               "batched_over_plain": fb / fp}
+
+    if not args.no_extras and not args.no_many:
+        extras["uneven_batch"] = uneven_leg(rst, synthetic)
+        extras["serving"] = serving_leg(rst, synthetic, not args.no_cpu)
 
     plain_lat = []
     for f in range(args.plain_frames):

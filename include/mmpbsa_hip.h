@@ -234,9 +234,10 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
 /* ---- serving (no counterpart in the reference) ---------------------------
  * A server owns one worker thread and HIP stream.  Requests (frames of one
  * split, copied at submission) wait in a queue; each scheduling round takes
- * frames round-robin from the waiting requests (continuous batching: new
- * requests join the next round) while the sum of their device-memory
- * estimates fits hbm_budget_bytes, and runs them through the batched path of
+ * frames from the waiting requests oldest first (continuous batching: new
+ * requests join the next round) while the round holds at most
+ * max_round_frames frames and the sum of their device-memory estimates fits
+ * hbm_budget_bytes, and runs them through the batched path of
  * mmpbsa_frames_energy, frames of different complexes sharing solver
  * launches.  A frame's result depends only on its own inputs: it is identical
  * alone and in any mix.  Splits (and their contexts) must outlive their

@@ -311,10 +311,12 @@ struct mmpbsa_server {
     void loop();
 };
 
-// Continuous batching: each round takes frames round-robin from the pending
-// requests (one frame per request per pass, new requests join at the next
-// round) while the round's admission estimate fits the HBM budget, then runs
-// them through the batched path.
+// Continuous batching: each round takes frames from the waiting requests,
+// oldest first (a request's remaining frames, then the next request's), while
+// the round holds at most max_round frames and their admission estimate fits
+// the HBM budget; requests submitted meanwhile join the next round.  Oldest
+// first finishes requests in arrival order, so a request's latency is its
+// queueing plus its own frames rather than the whole backlog.
 void mmpbsa_server::loop()
 {
     (void)hipSetDevice(device);
@@ -325,28 +327,16 @@ void mmpbsa_server::loop()
             cv_work.wait(lk, [&] { return stop || !pending.empty(); });
             if (pending.empty()) return;  // stop requested and nothing left
             double bytes = 0.0;
-            bool full = false;
-            while (!full && !pending.empty() && (int)picks.size() < max_round) {
-                bool took = false;
-                for (auto it = pending.begin(); it != pending.end() && (int)picks.size() < max_round;) {
-                    ServerRequest *r = *it;
-                    if (!picks.empty() && bytes + r->frame_bytes > budget) {
-                        full = true;
-                        break;
-                    }
-                    if (!r->started) {
-                        r->started = true;
-                        r->t_start = Clock::now();
-                    }
-                    picks.push_back({r, r->next++});
-                    bytes += r->frame_bytes;
-                    took = true;
-                    if (r->next >= r->n_frames)
-                        it = pending.erase(it);
-                    else
-                        ++it;
+            while (!pending.empty() && (int)picks.size() < max_round) {
+                ServerRequest *r = pending.front();
+                if (!picks.empty() && bytes + r->frame_bytes > budget) break;
+                if (!r->started) {
+                    r->started = true;
+                    r->t_start = Clock::now();
                 }
-                if (!took) break;
+                picks.push_back({r, r->next++});
+                bytes += r->frame_bytes;
+                if (r->next >= r->n_frames) pending.pop_front();
             }
             in_flight = bytes;
         }

@@ -44,6 +44,7 @@ struct MgLevelDev {
     float *e2;         // correction after pre-smoothing (batched levels)
     float *f;          // right-hand side
     float *res;        // residual scratch
+    int zc;            // smoother: planes per workgroup (z-chunk length)
     // index helpers: (nx-1)/2 slots per row, ny-2 interior rows, nx, nx*ny
     FastDiv fd_half, fd_row, fd_nx, fd_nxy;
 };
@@ -81,13 +82,15 @@ struct MgOuterDev {
 // Tiled smoother: one red-black Gauss-Seidel sweep over the level, out of
 // place.  dir 0 reads e2 and writes e, dir 1 reads e and writes e2; first = 1
 // starts from a zero correction (nothing is read) and writes e2.  Each
-// workgroup owns an (x, y) tile and a range of zc planes, streams it along z
+// workgroup owns an (x, y) tile and a range of L.zc planes, streams it along z
 // through an LDS ring and recomputes a halo, so the result equals that of two
 // in-place half-sweeps over the whole level.
 int smooth_tiles(int nx, int ny);  // (x, y) tiles of one plane
+int smooth_zc(int nz);             // z-chunk length for a level of nz planes
+// max_blocks: the largest smooth_tiles * ceil(nz / zc) among the solves
 // C (one coarse level per solve, or null): add the coarse correction's
 // prolongation to the source while loading it (first post-smoothing pass).
-void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir, int zc,
+void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
                int max_blocks, const MgStatus *st, hipStream_t s);
 void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
                    hipStream_t s);

@@ -16,6 +16,7 @@
 // roughly a hundred tiny launches per cycle.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "common.h"
@@ -254,6 +255,15 @@ static_assert(RN % 256 == 0 && NP % 256 == 0, "region must fill the workgroup");
 
 int smooth_tiles(int nx, int ny) { return ceil_div(nx, sm::TILE) * ceil_div(ny, sm::TILE); }
 
+// About 48 planes per workgroup: long enough that the two extra planes a
+// chunk loads stay cheap, short enough that large grids split into many
+// workgroups whatever else shares the launch.
+int smooth_zc(int nz)
+{
+    const int nch = std::max(1, (nz + 24) / 48);
+    return ceil_div(nz, nch);
+}
+
 // face value: bit set -> cin, clear -> cout (a bitfield select on the bits)
 __device__ __forceinline__ float face_sel(uint32_t w, int bit, uint32_t cin, uint32_t cout)
 {
@@ -270,7 +280,7 @@ __device__ __forceinline__ float face_sel(uint32_t w, int bit, uint32_t cin, uin
 template <bool FIRST, bool PROLONG>
 __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restrict__ Ls,
                                                      const MgLevelDev *__restrict__ Cs, int dir,
-                                                     int zc, const MgStatus *__restrict__ st)
+                                                     const MgStatus *__restrict__ st)
 {
     using namespace sm;
     __shared__ float ring[RING * RN];
@@ -280,7 +290,7 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     if (stopped(st + s)) return;
     const MgLevelDev L = Ls[s];
     if (L.nx == 0) return;
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int nx = L.nx, ny = L.ny, nz = L.nz, zc = L.zc;
     const int ntx = (nx + TILE - 1) / TILE, nty = (ny + TILE - 1) / TILE;
     const int nch = (nz + zc - 1) / zc;
     if (b >= ntx * nty * nch) return;
@@ -485,17 +495,17 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     }
 }
 
-void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir, int zc,
+void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
                int max_blocks, const MgStatus *st, hipStream_t s)
 {
     if (n_solves <= 0 || max_blocks <= 0) return;
     const dim3 grid(max_blocks, n_solves);
     if (first)
-        hipLaunchKernelGGL((smooth_kernel<true, false>), grid, dim3(256), 0, s, L, C, dir, zc, st);
+        hipLaunchKernelGGL((smooth_kernel<true, false>), grid, dim3(256), 0, s, L, C, dir, st);
     else if (C)
-        hipLaunchKernelGGL((smooth_kernel<false, true>), grid, dim3(256), 0, s, L, C, dir, zc, st);
+        hipLaunchKernelGGL((smooth_kernel<false, true>), grid, dim3(256), 0, s, L, C, dir, st);
     else
-        hipLaunchKernelGGL((smooth_kernel<false, false>), grid, dim3(256), 0, s, L, C, dir, zc, st);
+        hipLaunchKernelGGL((smooth_kernel<false, false>), grid, dim3(256), 0, s, L, C, dir, st);
     LAUNCH_OK();
 }
 

@@ -357,6 +357,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const float ih = (float)(1.0 / (lv.g.h * lv.g.h));
             dv.ihx = dv.ihy = dv.ihz = ih;
             dv.omega = (float)estimate_omega(lv.g.nx, lv.g.ny, lv.g.nz);
+            dv.zc = smooth_zc(lv.g.nz);
             dv.e = lv.e.get();
             dv.e2 = lv.e2.get();
             dv.f = lv.f.get();
@@ -424,7 +425,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         std::vector<MgOuterDev> O;
     } H[2];
     std::vector<int64_t> maxnodes(maxT, 0), maxcoarse(maxT, 0);
-    std::vector<int> maxtiles(maxT, 0), maxnz(maxT, 0);
+    std::vector<int> sblocks(maxT, 0);  // smoother launch width per level
     int64_t maxN0 = 0;
     for (int p = 0; p < 2; ++p) {
         H[p].F.assign(nF, null_level());
@@ -444,8 +445,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 H[p].F[(size_t)l * S + s] = lev(l);
                 H[p].C[(size_t)l * S + s] = lev(l + 1);
                 const GridDesc &g = tw.L[l].g;
-                maxtiles[l] = std::max(maxtiles[l], smooth_tiles(g.nx, g.ny));
-                maxnz[l] = std::max(maxnz[l], g.nz);
+                sblocks[l] = std::max(sblocks[l], smooth_tiles(g.nx, g.ny) *
+                                                      ceil_div(g.nz, smooth_zc(g.nz)));
                 maxnodes[l] = std::max(maxnodes[l], lv_nodes(tw.L[l].g));
                 maxcoarse[l] = std::max(maxcoarse[l], lv_nodes(tw.L[l + 1].g));
             }
@@ -460,14 +461,6 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     }
     std::vector<MgOuterDev> O0(S);
     for (int s = 0; s < S; ++s) O0[s] = W[s].O0;
-    // smoother launch shape per level: enough z-chunks for ~8 workgroups per CU
-    std::vector<int> zc(maxT, 1), sblocks(maxT, 0);
-    for (int l = 0; l < maxT; ++l) {
-        const int want = ceil_div(2048, std::max<int64_t>(1, (int64_t)maxtiles[l] * S));
-        const int nch = std::max(1, std::min(want, maxnz[l] / 16));
-        zc[l] = ceil_div(maxnz[l], nch);
-        sblocks[l] = maxtiles[l] * ceil_div(maxnz[l], zc[l]);
-    }
     // one device blob holding every table
     auto bytes_of = [](size_t n, size_t sz) { return (n * sz + 255) & ~(size_t)255; };
     const size_t szF = bytes_of(nF ? nF : 1, sizeof(MgLevelDev));
@@ -539,8 +532,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 HIP_OK(hipEventCreate(&ev.b));
                 HIP_OK(hipEventRecord(ev.a, w.stream));
             }
-            mg_smooth(F, prolong ? C : nullptr, S, first ? 1 : 0, dir, zc[l], sblocks[l], dst,
-                      w.stream);
+            mg_smooth(F, prolong ? C : nullptr, S, first ? 1 : 0, dir, sblocks[l], dst, w.stream);
             if (profiling && l == 0) {
                 HIP_OK(hipEventRecord(ev.b, w.stream));
                 evs.push_back(ev);

@@ -459,8 +459,8 @@ class RequestStats:
 class Server:
     """Continuous-batching MM-PBSA server (extension: the reference has no
     served path).  submit() copies the frames ([F, N, 3] nm) and returns a
-    ticket at once; a worker thread schedules frames of all waiting requests
-    round-robin into shared batched rounds under an HBM budget; poll() / wait()
+    ticket at once; a worker thread schedules frames of the waiting requests,
+    oldest first, into shared batched rounds under an HBM budget; poll() / wait()
     return (BindingResult, RequestStats).  A frame's result is identical alone
     and in any mix of requests."""
 
