@@ -231,6 +231,25 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
                      const mmpbsa_sa_params *sa, mmpbsa_residue_contrib *rec_out,
                      mmpbsa_residue_contrib *lig_out);
 
+/* ---- statistics and entropy over frames (host; no device needed) ---------- */
+typedef struct mmpbsa_binding_stats {    /* BindingResult (binding.rs:704-743) */
+    double mean_delta_mm, mean_delta_polar, mean_delta_sa, mean_delta_total;
+    double std_delta_total, std_delta_mm, std_delta_polar, std_delta_sa;  /* sample std */
+    double sem_delta_total;                                                /* std / sqrt(n) */
+} mmpbsa_binding_stats;
+
+/* binding.rs:704-743: means in frame order, sample standard deviations (0 for
+ * one frame), SEM of delta_total.  n == 0 is MMPBSA_ERR_INVALID_ARG ("No
+ * frames found in trajectory", binding.rs:700-702). */
+int mmpbsa_binding_stats_compute(const mmpbsa_frame_terms *frames, int32_t n,
+                                 mmpbsa_binding_stats *out);
+
+/* interaction_entropy (entropy.rs:53-78): -T dS = kT ln <exp((dMM - <dMM>)/kT)>
+ * by log-sum-exp, kT = 0.00198688 T.  Fewer than two frames (the reference's
+ * None) is MMPBSA_ERR_INVALID_ARG. */
+int mmpbsa_interaction_entropy(const mmpbsa_frame_terms *frames, int32_t n, double temperature,
+                               double *minus_tds);
+
 /* ---- serving (no counterpart in the reference) ---------------------------
  * A server owns one worker thread and HIP stream.  Requests (frames of one
  * split, copied at submission) wait in a queue; each scheduling round takes

@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
+#include <limits>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -784,6 +786,59 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
         fill(s->rec_res, s->rec_loc, rec_out);
         fill(s->lig_res, s->lig_loc, lig_out);
     });
+}
+
+int mmpbsa_binding_stats_compute(const mmpbsa_frame_terms *frames, int32_t n,
+                                 mmpbsa_binding_stats *out)
+{
+    if (!out || (n > 0 && !frames)) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    if (n <= 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "No frames found in trajectory");
+    auto mean = [&](double mmpbsa_frame_terms::*m) {
+        double acc = 0.0;
+        for (int32_t k = 0; k < n; ++k) acc += frames[k].*m;
+        return acc / (double)n;
+    };
+    auto sample_std = [&](double mmpbsa_frame_terms::*m, double mu) {
+        if (n <= 1) return 0.0;
+        double acc = 0.0;
+        for (int32_t k = 0; k < n; ++k) {
+            const double d = frames[k].*m - mu;
+            acc += d * d;
+        }
+        return std::sqrt(acc / (double)(n - 1));
+    };
+    out->mean_delta_mm = mean(&mmpbsa_frame_terms::delta_mm);
+    out->mean_delta_polar = mean(&mmpbsa_frame_terms::delta_polar);
+    out->mean_delta_sa = mean(&mmpbsa_frame_terms::delta_sa);
+    out->mean_delta_total = mean(&mmpbsa_frame_terms::delta_total);
+    out->std_delta_total = sample_std(&mmpbsa_frame_terms::delta_total, out->mean_delta_total);
+    out->std_delta_mm = sample_std(&mmpbsa_frame_terms::delta_mm, out->mean_delta_mm);
+    out->std_delta_polar = sample_std(&mmpbsa_frame_terms::delta_polar, out->mean_delta_polar);
+    out->std_delta_sa = sample_std(&mmpbsa_frame_terms::delta_sa, out->mean_delta_sa);
+    out->sem_delta_total = out->std_delta_total / std::sqrt((double)n);
+    return MMPBSA_OK;
+}
+
+int mmpbsa_interaction_entropy(const mmpbsa_frame_terms *frames, int32_t n, double temperature,
+                               double *minus_tds)
+{
+    if (!minus_tds || (n > 0 && !frames))
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    if (n < 2) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "interaction entropy needs >= 2 frames");
+    const double kt = 0.00198688 * temperature;
+    double mean_de = 0.0;
+    for (int32_t k = 0; k < n; ++k) mean_de += frames[k].delta_mm;
+    mean_de /= (double)n;
+    std::vector<double> ex(n);
+    double mx = -std::numeric_limits<double>::infinity();
+    for (int32_t k = 0; k < n; ++k) {
+        ex[k] = (frames[k].delta_mm - mean_de) / kt;
+        mx = std::max(mx, ex[k]);
+    }
+    double acc = 0.0;
+    for (int32_t k = 0; k < n; ++k) acc += std::exp(ex[k] - mx);
+    *minus_tds = kt * (mx + std::log(acc / (double)n));
+    return MMPBSA_OK;
 }
 
 void mmpbsa_server_params_default(mmpbsa_server_params *p)

@@ -10,7 +10,6 @@ without a GPU, calling a function without one raises RuntimeError.
 from __future__ import annotations
 
 import ctypes as C
-import math
 
 import numpy as np
 
@@ -155,6 +154,7 @@ class FrameEnergy:
     """FrameEnergy getters (lib.rs:1760-1850), *_gb aliases of *_polar."""
 
     def __init__(self, t: FrameTermsC):
+        self._c = FrameTermsC.from_buffer_copy(t)
         for part, src in (("complex", t.complex_), ("receptor", t.receptor), ("ligand", t.ligand)):
             setattr(self, part + "_mm", src.mm)
             setattr(self, part + "_polar", src.polar)
@@ -168,35 +168,52 @@ class FrameEnergy:
         self.delta_total = t.delta_total
 
 
+def _terms_array(frames):
+    arr = (FrameTermsC * max(len(frames), 1))()
+    for k, f in enumerate(frames):
+        arr[k] = f._c
+    return arr
+
+
 class BindingResult:
-    """Statistics of binding.rs:704-743 over in-memory frames."""
+    """BindingResult (binding.rs:704-743): per-frame energies with means,
+    sample standard deviations and the SEM of delta_total, computed by the
+    library (mmpbsa_binding_stats_compute) in the reference's order."""
 
     def __init__(self, frames):
         self._frames = frames
-        n = len(frames)
-
-        def mean(key):
-            return sum(getattr(f, key) for f in frames) / n
-
-        def std(key, m):
-            if n <= 1:
-                return 0.0
-            return math.sqrt(sum((getattr(f, key) - m) ** 2 for f in frames) / (n - 1))
-
-        self.mean_delta_mm = mean("delta_mm")
-        self.mean_delta_polar = mean("delta_polar")
+        lib = load_library()
+        st = _abi.BindingStatsC()
+        _check(lib.mmpbsa_binding_stats_compute(_terms_array(frames), len(frames), C.byref(st)),
+               lib)
+        for k, _ in _abi.BindingStatsC._fields_:
+            setattr(self, k, getattr(st, k))
         self.mean_delta_gb = self.mean_delta_polar
-        self.mean_delta_sa = mean("delta_sa")
-        self.mean_delta_total = mean("delta_total")
-        self.std_delta_total = std("delta_total", self.mean_delta_total)
-        self.std_delta_mm = std("delta_mm", self.mean_delta_mm)
-        self.std_delta_polar = std("delta_polar", self.mean_delta_polar)
         self.std_delta_gb = self.std_delta_polar
-        self.std_delta_sa = std("delta_sa", self.mean_delta_sa)
-        self.sem_delta_total = self.std_delta_total / math.sqrt(n)
 
     def frames(self):
         return list(self._frames)
+
+
+class EntropyEstimate:
+    """EntropyEstimate getters (lib.rs:1993-2011)."""
+
+    def __init__(self, minus_tds, method):
+        self.minus_tds = minus_tds
+        self.method = method
+
+
+def interaction_entropy(frames, temperature=298.15):
+    """interaction_entropy (entropy.rs:53-78; lib.rs:2216-2224): -T dS from the
+    fluctuation of delta_mm over frames, or None with fewer than two frames."""
+    frames = list(frames)
+    if len(frames) < 2:
+        return None
+    lib = load_library()
+    out = C.c_double()
+    _check(lib.mmpbsa_interaction_entropy(_terms_array(frames), len(frames), float(temperature),
+                                          C.byref(out)), lib)
+    return EntropyEstimate(out.value, "interaction_entropy")
 
 
 class ResidueContribution:

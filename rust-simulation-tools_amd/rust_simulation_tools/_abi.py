@@ -160,6 +160,13 @@ class ResidueContribC(C.Structure):
     ]
 
 
+class BindingStatsC(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("mean_delta_mm", "mean_delta_polar", "mean_delta_sa", "mean_delta_total",
+                 "std_delta_total", "std_delta_mm", "std_delta_polar", "std_delta_sa",
+                 "sem_delta_total")]
+
+
 class ServerParamsC(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_round_frames", C.c_int32)]
 
@@ -201,6 +208,10 @@ SIGNATURES = {
                                               C.POINTER(FrameTermsC)]),
     "mmpbsa_decompose": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), C.POINTER(SaParamsC),
                                    C.POINTER(ResidueContribC), C.POINTER(ResidueContribC)]),
+    "mmpbsa_binding_stats_compute": (C.c_int, [C.POINTER(FrameTermsC), C.c_int32,
+                                               C.POINTER(BindingStatsC)]),
+    "mmpbsa_interaction_entropy": (C.c_int, [C.POINTER(FrameTermsC), C.c_int32, C.c_double,
+                                             c_double_p]),
     "mmpbsa_server_params_default": (None, [C.POINTER(ServerParamsC)]),
     "mmpbsa_server_create": (C.c_int, [C.POINTER(ServerParamsC), C.c_int32, C.POINTER(P)]),
     "mmpbsa_server_destroy": (C.c_int, [P]),
