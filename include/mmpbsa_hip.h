@@ -36,6 +36,7 @@ extern "C" {
 #define MMPBSA_ERR_NO_DEVICE 5     /* no gfx950 device visible                 */
 #define MMPBSA_ERR_NOT_READY 6     /* server ticket not finished yet           */
 #define MMPBSA_ERR_TOO_LARGE 7     /* request exceeds a fixed capacity         */
+#define MMPBSA_ERR_IO 8            /* file missing, unreadable or malformed    */
 
 /* ---- topology view: caller-owned, read-only; copied at context creation --
  * Mirrors AmberTopology (crates/core/src/amber/prmtop.rs:34-105).
@@ -230,6 +231,32 @@ int mmpbsa_frames_energy_device(mmpbsa_split *s, const double *d_xyz_frames, int
 int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params *gb,
                      const mmpbsa_sa_params *sa, mmpbsa_residue_contrib *rec_out,
                      mmpbsa_residue_contrib *lig_out);
+
+/* ---- topology ingest (host; no device needed) ----------------------------
+ * parse_prmtop (crates/core/src/amber/prmtop.rs:436-733): %FLAG sections,
+ * indices made 0-based, bond / angle / dihedral entries divided by 3,
+ * charges = CHARGE / 18.2223 (charges_amber keeps CHARGE), ignore_14 from a
+ * negative third or fourth dihedral index, EXCLUDED_ATOMS_LIST 0 -> -1,
+ * SCEE / SCNB_SCALE_FACTOR first value (defaults 1.2 / 2.0), missing MASS /
+ * RADII / SCREEN read as zeros.  Errors are MMPBSA_ERR_IO with the
+ * reference's messages (mmpbsa_prmtop_last_error); a per-atom section shorter
+ * than n_atoms is also an error here.                                        */
+typedef struct mmpbsa_prmtop mmpbsa_prmtop;
+
+typedef struct mmpbsa_prmtop_info {
+    int32_t n_atoms, n_residues, n_types;
+    const double *masses;          /* [n_atoms] amu                              */
+    const char *atom_names;        /* [n_atoms][5]: up to 4 characters, NUL-ended */
+    const char *residue_labels;    /* [n_residues][5]                            */
+} mmpbsa_prmtop_info;
+
+int mmpbsa_prmtop_read(const char *path, mmpbsa_prmtop **out);
+const char *mmpbsa_prmtop_last_error(void);
+int mmpbsa_prmtop_destroy(mmpbsa_prmtop *t);
+/* A topology view of the parsed file (pointers valid until destroy), ready
+ * for mmpbsa_ctx_create. */
+int mmpbsa_prmtop_view(const mmpbsa_prmtop *t, mmpbsa_topology_view *view);
+int mmpbsa_prmtop_info_get(const mmpbsa_prmtop *t, mmpbsa_prmtop_info *info);
 
 /* ---- statistics and entropy over frames (host; no device needed) ---------- */
 typedef struct mmpbsa_binding_stats {    /* BindingResult (binding.rs:704-743) */

@@ -17,7 +17,7 @@ from . import _abi
 from ._abi import (FrameTermsC, GbParamsC, MmTermsC, PbParamsC, PbResultC, RequestStatsC,
                    ResidueContribC,
                    SaParamsC, load_library)
-from .topology import AmberTopology, extract_coords
+from .topology import AmberTopology, extract_coords, read_prmtop
 from .topology import extract_subtopology as _extract_subtopology
 
 __all__ = [

@@ -26,7 +26,9 @@ ERRORS = {
     5: "no gfx950 device",
     6: "not ready",
     7: "request too large",
+    8: "file missing, unreadable or malformed",
 }
+MMPBSA_ERR_IO = 8
 
 
 class TopologyView(C.Structure):
@@ -160,6 +162,12 @@ class ResidueContribC(C.Structure):
     ]
 
 
+class PrmtopInfoC(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("n_residues", C.c_int32), ("n_types", C.c_int32),
+                ("masses", c_double_p), ("atom_names", C.c_void_p),
+                ("residue_labels", C.c_void_p)]
+
+
 class BindingStatsC(C.Structure):
     _fields_ = [(k, C.c_double) for k in
                 ("mean_delta_mm", "mean_delta_polar", "mean_delta_sa", "mean_delta_total",
@@ -208,6 +216,11 @@ SIGNATURES = {
                                               C.POINTER(FrameTermsC)]),
     "mmpbsa_decompose": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), C.POINTER(SaParamsC),
                                    C.POINTER(ResidueContribC), C.POINTER(ResidueContribC)]),
+    "mmpbsa_prmtop_read": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "mmpbsa_prmtop_last_error": (C.c_char_p, []),
+    "mmpbsa_prmtop_destroy": (C.c_int, [P]),
+    "mmpbsa_prmtop_view": (C.c_int, [P, C.POINTER(TopologyView)]),
+    "mmpbsa_prmtop_info_get": (C.c_int, [P, C.POINTER(PrmtopInfoC)]),
     "mmpbsa_binding_stats_compute": (C.c_int, [C.POINTER(FrameTermsC), C.c_int32,
                                                C.POINTER(BindingStatsC)]),
     "mmpbsa_interaction_entropy": (C.c_int, [C.POINTER(FrameTermsC), C.c_int32, C.c_double,

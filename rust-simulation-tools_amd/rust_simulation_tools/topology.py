@@ -15,6 +15,19 @@ import numpy as np
 from ._abi import TopologyView, as_ptr
 
 
+class _Getter(np.ndarray):
+    """An array field that can also be called, so both the dataclass field
+    (topo.charges) and the reference's PyO3 getter method (topo.charges(),
+    lib.rs getters returning a fresh numpy array) read the same data."""
+
+    def __call__(self):
+        return np.array(self)
+
+
+def _getter(a, dtype):
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype)).view(_Getter)
+
+
 def _f64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
@@ -51,9 +64,10 @@ class AmberTopology:
     scee_scale_factor: float = 1.2
     scnb_scale_factor: float = 2.0
     atom_names: list = None
+    masses: np.ndarray = None                # amu (prmtop MASS)
 
     def __post_init__(self):
-        self.charges = _f64(self.charges)
+        self.charges = _getter(self.charges, np.float64)
         self.charges_amber = _f64(self.charges_amber)
         self.radii = _f64(self.radii)
         self.screen = _f64(self.screen)
@@ -61,7 +75,7 @@ class AmberTopology:
         self.nb_parm_index = np.ascontiguousarray(np.asarray(self.nb_parm_index, dtype=np.int64))
         self.lj_acoef = _f64(self.lj_acoef)
         self.lj_bcoef = _f64(self.lj_bcoef)
-        self.residue_pointers = _i32(self.residue_pointers)
+        self.residue_pointers = _getter(self.residue_pointers, np.int32)
         self.bonds = _i32(np.zeros((0, 3)) if self.bonds is None else self.bonds).reshape(-1, 3)
         self.angles = _i32(np.zeros((0, 4)) if self.angles is None else self.angles).reshape(-1, 4)
         self.dihedrals = _i32(np.zeros((0, 6)) if self.dihedrals is None else self.dihedrals).reshape(-1, 6)
@@ -81,6 +95,7 @@ class AmberTopology:
             self.residue_labels = ["RES"] * len(self.residue_pointers)
         if self.atom_names is None:
             self.atom_names = ["X"] * n
+        self.masses = _f64(np.zeros(n) if self.masses is None else self.masses)
 
     # ---- reference accessors -------------------------------------------
     @property
@@ -98,6 +113,31 @@ class AmberTopology:
         for r in range(self.n_residues):
             out[ptr[r]:ptr[r + 1]] = r
         return out
+
+    def _self_lj(self):
+        """Per-type sigma (nm) and epsilon (kJ/mol) from the self terms
+        (prmtop.rs:530-556)."""
+        nt = int(self.n_types)
+        sig, eps = np.zeros(nt), np.zeros(nt)
+        for i in range(nt):
+            k = nt * i + i
+            if k >= self.nb_parm_index.shape[0] or self.nb_parm_index[k] < 1:
+                continue
+            a, b = self.lj_acoef[self.nb_parm_index[k] - 1], self.lj_bcoef[self.nb_parm_index[k] - 1]
+            if abs(b) > 1e-20 and abs(a) > 1e-20:
+                sig[i] = (a / b) ** (1.0 / 6.0) * 0.1
+                eps[i] = b * b / (4.0 * a) * 4.184
+        return sig, eps
+
+    def sigmas(self) -> np.ndarray:
+        return self._self_lj()[0][self.atom_type_indices]
+
+    def epsilons(self) -> np.ndarray:
+        return self._self_lj()[1][self.atom_type_indices]
+
+    def bonds_list(self):
+        """The reference's bonds() getter: (i, j) pairs."""
+        return [(int(b[0]), int(b[1])) for b in self.bonds]
 
     def build_selection(self, residue_indices) -> np.ndarray:
         """prmtop.rs:165-204: atom indices of the residues, in the given order."""
@@ -152,6 +192,58 @@ class AmberTopology:
         v.scee = float(self.scee_scale_factor)
         v.scnb = float(self.scnb_scale_factor)
         return v
+
+
+def read_prmtop(path) -> AmberTopology:
+    """read_prmtop (lib.rs:979-984 over prmtop.rs:436-733), parsed by the
+    library; failures raise OSError with the reference's messages."""
+    import os
+
+    from ._abi import PrmtopInfoC, load_library
+    lib = load_library()
+    h = C.c_void_p()
+    rc = lib.mmpbsa_prmtop_read(os.fsencode(str(path)), C.byref(h))
+    if rc != 0:
+        raise OSError(lib.mmpbsa_prmtop_last_error().decode())
+    try:
+        v = TopologyView()
+        info = PrmtopInfoC()
+        lib.mmpbsa_prmtop_view(h, C.byref(v))
+        lib.mmpbsa_prmtop_info_get(h, C.byref(info))
+
+        def arr(ptr, count, dtype):
+            if count <= 0 or not ptr:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+        def names(base, count):
+            return [C.string_at(base + 5 * k).decode("ascii", "replace") for k in range(count)]
+
+        n, nr, nt = v.n_atoms, v.n_residues, v.n_types
+        return AmberTopology(
+            charges=arr(v.charges, n, np.float64), charges_amber=arr(v.charges_amber, n, np.float64),
+            radii=arr(v.radii, n, np.float64), screen=arr(v.screen, n, np.float64),
+            atom_type_indices=arr(v.atom_type, n, np.int32), n_types=nt,
+            nb_parm_index=arr(v.nb_parm_index, nt * nt, np.int64),
+            lj_acoef=arr(v.lj_acoef, v.n_lj, np.float64), lj_bcoef=arr(v.lj_bcoef, v.n_lj, np.float64),
+            residue_pointers=arr(v.residue_pointers, nr, np.int32),
+            residue_labels=names(info.residue_labels, nr),
+            bonds=arr(v.bonds, 3 * v.n_bonds, np.int32).reshape(-1, 3),
+            bond_force_constants=arr(v.bond_k, v.n_bond_types, np.float64),
+            bond_equil_values=arr(v.bond_r0, v.n_bond_types, np.float64),
+            angles=arr(v.angles, 4 * v.n_angles, np.int32).reshape(-1, 4),
+            angle_force_constants=arr(v.angle_k, v.n_angle_types, np.float64),
+            angle_equil_values=arr(v.angle_theta0, v.n_angle_types, np.float64),
+            dihedrals=arr(v.dihedrals, 6 * v.n_dihedrals, np.int32).reshape(-1, 6),
+            dihedral_force_constants=arr(v.dihedral_pk, v.n_dihedral_types, np.float64),
+            dihedral_periodicities=arr(v.dihedral_pn, v.n_dihedral_types, np.float64),
+            dihedral_phases=arr(v.dihedral_phase, v.n_dihedral_types, np.float64),
+            num_excluded_atoms=arr(v.num_excluded, n, np.int32),
+            excluded_atoms_list=arr(v.excluded_atoms, v.n_excluded, np.int32),
+            scee_scale_factor=v.scee, scnb_scale_factor=v.scnb,
+            atom_names=names(info.atom_names, n), masses=arr(info.masses, n, np.float64))
+    finally:
+        lib.mmpbsa_prmtop_destroy(h)
 
 
 def extract_subtopology(top: AmberTopology, atom_indices) -> AmberTopology:
