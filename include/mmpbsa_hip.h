@@ -258,6 +258,64 @@ int mmpbsa_prmtop_destroy(mmpbsa_prmtop *t);
 int mmpbsa_prmtop_view(const mmpbsa_prmtop *t, mmpbsa_topology_view *view);
 int mmpbsa_prmtop_info_get(const mmpbsa_prmtop *t, mmpbsa_prmtop_info *info);
 
+/* ---- trajectory ingest (host; no device needed) ---------------------------
+ * DCD: crates/core/src/trajectory/dcd.rs (header :197-348, frames :351-416);
+ * positions are returned in nm as there.  mdcrd: crates/mmpbsa/src/mdcrd.rs
+ * (positions in A).  Errors are MMPBSA_ERR_IO with the reference's messages
+ * (mmpbsa_traj_last_error).                                                 */
+typedef struct mmpbsa_dcd_header {
+    int64_t n_frames, n_atoms;
+    int32_t start_timestep, timestep_interval;
+    float timestep;
+    int32_t has_unit_cell, is_charmm, is_64bit, big_endian;
+    int64_t first_frame_offset, frame_size;
+} mmpbsa_dcd_header;
+
+typedef struct mmpbsa_dcd mmpbsa_dcd;
+typedef struct mmpbsa_mdcrd mmpbsa_mdcrd;
+
+const char *mmpbsa_traj_last_error(void);
+int mmpbsa_dcd_open(const char *path, mmpbsa_dcd **out);
+int mmpbsa_dcd_close(mmpbsa_dcd *d);
+/* header, the number of title lines, and title k (NUL-terminated into buf) */
+int mmpbsa_dcd_get_header(const mmpbsa_dcd *d, mmpbsa_dcd_header *out, int32_t *n_titles);
+int mmpbsa_dcd_title(const mmpbsa_dcd *d, int32_t k, char *buf, int32_t cap);
+int mmpbsa_dcd_seek(mmpbsa_dcd *d, int64_t frame);
+int64_t mmpbsa_dcd_current(const mmpbsa_dcd *d);
+/* next frame: xyz_nm [n_atoms][3]; box6 (nullable) a, b, c in nm and alpha,
+ * beta, gamma in degrees, set when has_unit_cell; *got = 0 at the end. */
+int mmpbsa_dcd_read_frame(mmpbsa_dcd *d, double *xyz_nm, double *box6, int32_t *got);
+
+int mmpbsa_mdcrd_open(const char *path, int64_t n_atoms, int32_t has_box, mmpbsa_mdcrd **out);
+int mmpbsa_mdcrd_close(mmpbsa_mdcrd *m);
+int64_t mmpbsa_mdcrd_current(const mmpbsa_mdcrd *m);
+/* next frame: xyz_A [n_atoms][3]; box3 (nullable) X, Y, Z in A; *got = 0 at the end */
+int mmpbsa_mdcrd_read_frame(mmpbsa_mdcrd *m, double *xyz_A, double *box3, int32_t *got);
+
+/* reimage_complex (binding.rs:234-308): unwrap each bonded component by
+ * breadth-first minimum-image steps, then move components next to the first
+ * one by the minimum image of their geometric centres.  bonds: [n_bonds][2]. */
+int mmpbsa_reimage(double *xyz, int32_t n_atoms, const int32_t *bonds, int32_t n_bonds,
+                   const double box[3]);
+
+/* compute_binding_energy (binding.rs:461-744) over a trajectory file: frames
+ * start_frame, start_frame + stride, ... below end_frame (end_frame < 0: all)
+ * are read (format 0 = mdcrd in A, has_box; 1 = DCD in nm, converted),
+ * sliced to the split's complex, reimaged when the file carries a box, and
+ * processed in batches of 64 frames through the batched path.  The results
+ * are kept in the split until mmpbsa_trajectory_frames copies them out.
+ * No frames is MMPBSA_ERR_INVALID_ARG ("No frames found in trajectory"). */
+enum { MMPBSA_TRAJ_MDCRD = 0, MMPBSA_TRAJ_DCD = 1 };
+int mmpbsa_binding_energy_trajectory(mmpbsa_split *s, const char *path, int32_t format,
+                                     int32_t has_box, int64_t stride, int64_t start_frame,
+                                     int64_t end_frame, int32_t method,
+                                     const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
+                                     const mmpbsa_sa_params *sa, int32_t *n_frames);
+/* Copies the last trajectory call's n frames (and the complex coordinates of
+ * the last frame processed, [n_complex_atoms][3] A, nullable). */
+int mmpbsa_trajectory_frames(const mmpbsa_split *s, mmpbsa_frame_terms *out, int32_t n,
+                             double *last_frame_xyz);
+
 /* ---- statistics and entropy over frames (host; no device needed) ---------- */
 typedef struct mmpbsa_binding_stats {    /* BindingResult (binding.rs:704-743) */
     double mean_delta_mm, mean_delta_polar, mean_delta_sa, mean_delta_total;

@@ -16,6 +16,7 @@
 
 #include "decomp.h"
 #include "engine.h"
+#include "trajectory.h"
 
 using namespace mmpbsa;
 
@@ -38,6 +39,9 @@ struct mmpbsa_split {
     DArr<int> d_seg_pos, d_seg_ptr;
     int cross_excl = 0;
     std::vector<std::unique_ptr<Worker>> workers;
+    // results of the last trajectory call
+    std::vector<mmpbsa_frame_terms> traj_out;
+    std::vector<double> traj_last;
 };
 
 namespace {
@@ -103,8 +107,24 @@ struct FrameCoords {
 // (binding.rs:213-222, :345-346) on the device.  Host frames are uploaded
 // first; device frames (on_device) are read in place and leave no host copy
 // (bounding boxes are then taken on the device).
-void stage_frame(Worker &w, mmpbsa_split *s, const double *xyz, FrameCoords &f, bool on_device)
+void stage_frame(Worker &w, mmpbsa_split *s, const double *xyz, FrameCoords &f, bool on_device,
+                 bool complex_local = false)
 {
+    if (complex_local) {  // host coordinates of the complex atoms only
+        const size_t nc = s->csel.size();
+        f.c = DBuf<double>(w.pool, 3 * nc + 3);
+        HIP_OK(hipMemcpyAsync(f.c.get(), xyz, sizeof(double) * 3 * nc, hipMemcpyHostToDevice,
+                              w.stream));
+        f.dc = f.c.get();
+        f.hcp = xyz;
+        f.r = DBuf<double>(w.pool, 3 * s->ra.size() + 3);
+        f.l = DBuf<double>(w.pool, 3 * s->la.size() + 3);
+        launch_gather_xyz(f.dc, s->d_ra.p, (int)s->ra.size(), f.r.get(), w.stream);
+        launch_gather_xyz(f.dc, s->d_la.p, (int)s->la.size(), f.l.get(), w.stream);
+        f.hr = gather_host(f.hcp, s->ra);
+        f.hl = gather_host(f.hcp, s->la);
+        return;
+    }
     const int n_all = s->ctx->top.h.n;
     const double *full = xyz;
     if (!on_device) {
@@ -149,6 +169,7 @@ struct FrameItem {
     mmpbsa_pb_params pb{};
     mmpbsa_sa_params sa{};
     mmpbsa_frame_terms *out = nullptr;
+    bool complex_local = false;   // xyz holds only the split's complex atoms
 };
 
 // compute_frame_energy (binding.rs:333-415) for a list of frames, possibly of
@@ -160,7 +181,8 @@ void run_items(Worker &w, const std::vector<FrameItem> &items, bool batched, boo
 {
     const size_t nf = items.size();
     std::vector<FrameCoords> fc(nf);
-    for (size_t f = 0; f < nf; ++f) stage_frame(w, items[f].s, items[f].xyz, fc[f], on_device);
+    for (size_t f = 0; f < nf; ++f)
+        stage_frame(w, items[f].s, items[f].xyz, fc[f], on_device, items[f].complex_local);
     auto tops = [&](size_t f, int k) -> const DevTopo * {
         mmpbsa_split *s = items[f].s;
         return k == 0 ? &s->ctop : (k == 1 ? &s->rtop : &s->ltop);
@@ -786,6 +808,119 @@ int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params 
         fill(s->rec_res, s->rec_loc, rec_out);
         fill(s->lig_res, s->lig_loc, lig_out);
     });
+}
+
+int mmpbsa_binding_energy_trajectory(mmpbsa_split *s, const char *path, int32_t format,
+                                     int32_t has_box, int64_t stride, int64_t start_frame,
+                                     int64_t end_frame, int32_t method,
+                                     const mmpbsa_gb_params *gb, const mmpbsa_pb_params *pb,
+                                     const mmpbsa_sa_params *sa, int32_t *n_frames)
+{
+    if (!s || !path || !n_frames) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    *n_frames = 0;
+    mmpbsa_ctx *ctx = s->ctx;
+    return guarded(ctx, [&] {
+        mmpbsa_gb_params g;
+        mmpbsa_pb_params p;
+        mmpbsa_sa_params a;
+        mmpbsa_gb_params_default(&g);
+        mmpbsa_pb_params_default(&p);
+        mmpbsa_sa_params_default(&a);
+        if (gb) g = *gb;
+        if (pb) p = *pb;
+        if (sa) a = *sa;
+        const int64_t st = std::max<int64_t>(1, stride);  // binding.rs:525
+        const int64_t end = end_frame < 0 ? INT64_MAX : end_frame;
+        const int n_all = ctx->top.h.n;
+        const size_t nc = s->csel.size();
+        const std::vector<int> &cb = s->ctop.h.bonds;  // (i, j, type), complex-local
+        std::vector<double> full(3 * (size_t)n_all), box(6, 0.0);
+        std::unique_ptr<DcdFile> dcd;
+        std::unique_ptr<MdcrdFile> mdcrd;
+        try {
+            if (format == MMPBSA_TRAJ_DCD) {
+                dcd = std::make_unique<DcdFile>(path);
+                if (dcd->hd.n_atoms < n_all)
+                    throw TrajError("DCD has " + std::to_string(dcd->hd.n_atoms) +
+                                    " atoms, the topology " + std::to_string(n_all));
+                full.resize(3 * (size_t)dcd->hd.n_atoms);
+            } else {
+                mdcrd = std::make_unique<MdcrdFile>(path, n_all, has_box != 0);
+            }
+        } catch (const TrajError &e) {
+            throw DeviceError(MMPBSA_ERR_IO, e.what());
+        }
+        const int kBatch = 64;  // binding.rs:527
+        std::vector<std::vector<double>> batch;
+        std::vector<mmpbsa_frame_terms> out;
+        auto flush = [&] {
+            if (batch.empty()) return;
+            const size_t base = out.size();
+            out.resize(base + batch.size());
+            std::vector<FrameItem> items(batch.size());
+            for (size_t k = 0; k < batch.size(); ++k) {
+                FrameItem &it = items[k];
+                it.s = s;
+                it.xyz = batch[k].data();
+                it.method = method;
+                it.gb = g;
+                it.pb = p;
+                it.sa = a;
+                it.out = &out[base + k];
+                it.complex_local = true;
+            }
+            run_items(*ctx->w, items, true, false);
+            batch.clear();
+        };
+        std::vector<double> last;
+        int64_t idx = 0;
+        for (;;) {
+            bool got;
+            try {
+                got = dcd ? dcd->read_frame(full.data(), box.data())
+                          : mdcrd->read_frame(full.data(), box.data());
+            } catch (const TrajError &e) {
+                throw DeviceError(MMPBSA_ERR_IO, e.what());
+            }
+            if (!got || idx >= end) break;
+            if (idx >= start_frame && (idx - start_frame) % st == 0) {
+                std::vector<double> c(3 * nc);
+                const bool nm = dcd != nullptr;  // DCD positions are nm: back to A
+                for (size_t k = 0; k < nc; ++k) {
+                    const size_t src = s->identity ? k : (size_t)s->csel[k];
+                    for (int d = 0; d < 3; ++d)
+                        c[3 * k + d] = nm ? full[3 * src + d] * 10.0 : full[3 * src + d];
+                }
+                const bool boxed = dcd ? dcd->hd.has_unit_cell != 0 : has_box != 0;
+                if (boxed) {
+                    const double bx[3] = {nm ? box[0] * 10.0 : box[0], nm ? box[1] * 10.0 : box[1],
+                                          nm ? box[2] * 10.0 : box[2]};
+                    reimage_complex(c.data(), (int)nc, cb.data(), (int)(cb.size() / 3), 3, bx);
+                }
+                last = c;
+                batch.push_back(std::move(c));
+                if ((int)batch.size() >= kBatch) flush();
+            }
+            ++idx;
+        }
+        flush();
+        if (out.empty()) throw DeviceError(MMPBSA_ERR_INVALID_ARG, "No frames found in trajectory");
+        s->traj_out = std::move(out);
+        s->traj_last = std::move(last);
+        *n_frames = (int32_t)s->traj_out.size();
+    });
+}
+
+int mmpbsa_trajectory_frames(const mmpbsa_split *s, mmpbsa_frame_terms *out, int32_t n,
+                             double *last_frame_xyz)
+{
+    if (!s || (!out && n > 0)) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    if (n > (int32_t)s->traj_out.size())
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "more frames requested than computed");
+    if (n > 0) std::memcpy(out, s->traj_out.data(), sizeof(mmpbsa_frame_terms) * n);
+    if (last_frame_xyz && !s->traj_last.empty())
+        std::memcpy(last_frame_xyz, s->traj_last.data(), sizeof(double) * s->traj_last.size());
+    return MMPBSA_OK;
 }
 
 int mmpbsa_binding_stats_compute(const mmpbsa_frame_terms *frames, int32_t n,

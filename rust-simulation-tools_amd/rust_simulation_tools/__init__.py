@@ -10,6 +10,7 @@ without a GPU, calling a function without one raises RuntimeError.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -18,6 +19,7 @@ from ._abi import (FrameTermsC, GbParamsC, MmTermsC, PbParamsC, PbResultC, Reque
                    ResidueContribC,
                    SaParamsC, load_library)
 from .topology import AmberTopology, extract_coords, read_prmtop
+from .trajectory import DcdReader, MdcrdReader, read_dcd_header, reimage
 from .topology import extract_subtopology as _extract_subtopology
 
 __all__ = [
@@ -438,6 +440,36 @@ class DeviceArray:
             except Exception:
                 pass
             self.ptr = None
+
+
+def compute_binding_energy(topology, trajectory_path, receptor_residues, ligand_residues,
+                           gb_params=None, pb_params=None, sa_params=None, trajectory_format=None,
+                           has_box=False, stride=1, start_frame=0,
+                           end_frame=None) -> BindingResult:
+    """compute_binding_energy (lib.rs:2117-2160; binding.rs:461-744): topology
+    is an AmberTopology or a prmtop path (OSError if it cannot be read); the
+    trajectory is mdcrd (A, has_box) unless trajectory_format == "dcd" (nm);
+    frames start_frame, start_frame + stride, ... below end_frame, sliced to
+    the complex, reimaged when the file has a box, in batches of 64 through
+    the batched device path.  Other failures raise ValueError, as there."""
+    top = topology if isinstance(topology, AmberTopology) else read_prmtop(topology)
+    c = _ctx(top)
+    s = c.split(receptor_residues, ligand_residues)
+    m, g, p = _method(gb_params, pb_params)
+    a = (sa_params or SaParams())._c
+    fmt = 1 if trajectory_format == "dcd" else 0
+    n = C.c_int32()
+    rc = c.lib.mmpbsa_binding_energy_trajectory(
+        s, os.fsencode(str(trajectory_path)), fmt, 1 if has_box else 0, max(1, int(stride)),
+        int(start_frame), -1 if end_frame is None else int(end_frame), m, C.byref(g), C.byref(p),
+        C.byref(a), C.byref(n))
+    if rc == _abi.MMPBSA_ERR_IO:
+        msg = c.lib.mmpbsa_last_error(c.h)
+        raise ValueError(msg.decode() if msg else "trajectory error")
+    _check(rc, c.lib, c.h)
+    out = (FrameTermsC * max(n.value, 1))()
+    _check(c.lib.mmpbsa_trajectory_frames(s, out, n.value, None), c.lib, c.h)
+    return BindingResult([FrameEnergy(out[k]) for k in range(n.value)])
 
 
 def compute_binding_energy_device_frames(topology: AmberTopology, frames_angstrom: DeviceArray,

@@ -168,6 +168,14 @@ class PrmtopInfoC(C.Structure):
                 ("residue_labels", C.c_void_p)]
 
 
+class DcdHeaderC(C.Structure):
+    _fields_ = [("n_frames", C.c_int64), ("n_atoms", C.c_int64), ("start_timestep", C.c_int32),
+                ("timestep_interval", C.c_int32), ("timestep", C.c_float),
+                ("has_unit_cell", C.c_int32), ("is_charmm", C.c_int32), ("is_64bit", C.c_int32),
+                ("big_endian", C.c_int32), ("first_frame_offset", C.c_int64),
+                ("frame_size", C.c_int64)]
+
+
 class BindingStatsC(C.Structure):
     _fields_ = [(k, C.c_double) for k in
                 ("mean_delta_mm", "mean_delta_polar", "mean_delta_sa", "mean_delta_total",
@@ -221,6 +229,24 @@ SIGNATURES = {
     "mmpbsa_prmtop_destroy": (C.c_int, [P]),
     "mmpbsa_prmtop_view": (C.c_int, [P, C.POINTER(TopologyView)]),
     "mmpbsa_prmtop_info_get": (C.c_int, [P, C.POINTER(PrmtopInfoC)]),
+    "mmpbsa_traj_last_error": (C.c_char_p, []),
+    "mmpbsa_dcd_open": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "mmpbsa_dcd_close": (C.c_int, [P]),
+    "mmpbsa_dcd_get_header": (C.c_int, [P, C.POINTER(DcdHeaderC), c_int32_p]),
+    "mmpbsa_dcd_title": (C.c_int, [P, C.c_int32, C.c_char_p, C.c_int32]),
+    "mmpbsa_dcd_seek": (C.c_int, [P, C.c_int64]),
+    "mmpbsa_dcd_current": (C.c_int64, [P]),
+    "mmpbsa_dcd_read_frame": (C.c_int, [P, c_double_p, c_double_p, c_int32_p]),
+    "mmpbsa_mdcrd_open": (C.c_int, [C.c_char_p, C.c_int64, C.c_int32, C.POINTER(P)]),
+    "mmpbsa_mdcrd_close": (C.c_int, [P]),
+    "mmpbsa_mdcrd_current": (C.c_int64, [P]),
+    "mmpbsa_mdcrd_read_frame": (C.c_int, [P, c_double_p, c_double_p, c_int32_p]),
+    "mmpbsa_reimage": (C.c_int, [c_double_p, C.c_int32, c_int32_p, C.c_int32, c_double_p]),
+    "mmpbsa_binding_energy_trajectory": (C.c_int, [P, C.c_char_p, C.c_int32, C.c_int32,
+                                                   C.c_int64, C.c_int64, C.c_int64, C.c_int32,
+                                                   C.POINTER(GbParamsC), C.POINTER(PbParamsC),
+                                                   C.POINTER(SaParamsC), c_int32_p]),
+    "mmpbsa_trajectory_frames": (C.c_int, [P, C.POINTER(FrameTermsC), C.c_int32, c_double_p]),
     "mmpbsa_binding_stats_compute": (C.c_int, [C.POINTER(FrameTermsC), C.c_int32,
                                                C.POINTER(BindingStatsC)]),
     "mmpbsa_interaction_entropy": (C.c_int, [C.POINTER(FrameTermsC), C.c_int32, C.c_double,

@@ -1,0 +1,64 @@
+"""compute_binding_energy over trajectory files (DCD, mdcrd) equals the
+in-memory batched path on the frames the readers return; stride / start / end
+select frames as the reference does; a periodic box triggers reimaging."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import rust_simulation_tools as rst
+from rust_simulation_tools import synthetic
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from test_trajectory import _write_dcd, _write_mdcrd  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+PB = dict(grid_spacing=0.8, grid_buffer=4.0, solute_dielectric=4.0, solvent_dielectric=80.0,
+          salt_concentration=0.15)
+KEYS = ("complex_mm", "complex_polar", "receptor_polar", "ligand_polar", "complex_sa",
+        "delta_total")
+
+
+@pytest.fixture(scope="module")
+def system():
+    seed = synthetic.seed_for(5, 0)
+    top, base, rec, lig = synthetic.build_complex(700, 64, None, seed)
+    frames = np.stack([synthetic.noisy_frame(base, seed, k) for k in range(5)])  # A
+    return top, frames, rec, lig
+
+
+def test_dcd_trajectory(system, tmp_path):
+    top, frames, rec, lig = system
+    p = tmp_path / "t.dcd"
+    _write_dcd(p, frames, cells=[[500.0, 500.0, 500.0, 90.0, 90.0, 90.0]] * 5)
+    pb = rst.PbParams(**PB)
+    res = rst.compute_binding_energy(top, p, rec, lig, pb_params=pb, trajectory_format="dcd",
+                                     stride=2, start_frame=1)
+    traj, _ = rst.DcdReader(p).read_all()  # nm, as the reader returns them
+    ref = rst.compute_binding_energy_frames(top, traj[[1, 3]], rec, lig, pb_params=pb)
+    assert len(res.frames()) == 2
+    for a, b in zip(res.frames(), ref.frames()):
+        for k in KEYS:
+            assert getattr(a, k) == getattr(b, k), k
+    assert res.mean_delta_total == ref.mean_delta_total
+    gb = rst.compute_binding_energy(top, p, rec, lig, trajectory_format="dcd", end_frame=2)
+    assert len(gb.frames()) == 2
+
+
+def test_mdcrd_trajectory(system, tmp_path):
+    top, frames, rec, lig = system
+    p = tmp_path / "t.mdcrd"
+    _write_mdcrd(p, np.round(frames[:3], 3))
+    res = rst.compute_binding_energy(top, p, rec, lig)          # GB, every frame
+    rd = rst.MdcrdReader(p, top.n_atoms, False)
+    fr = np.stack([rd.read_frame() for _ in range(3)]) / 10.0
+    ref = rst.compute_binding_energy_frames(top, fr, rec, lig)
+    for a, b in zip(res.frames(), ref.frames()):
+        for k in KEYS:
+            assert getattr(a, k) == pytest.approx(getattr(b, k), abs=1e-9), k
+    with pytest.raises(ValueError, match="No frames found in trajectory"):
+        rst.compute_binding_energy(top, p, rec, lig, start_frame=10)
+    with pytest.raises(ValueError, match="Failed to open mdcrd"):
+        rst.compute_binding_energy(top, tmp_path / "none.mdcrd", rec, lig)
