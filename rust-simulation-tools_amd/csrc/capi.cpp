@@ -88,6 +88,12 @@ void require_device()
     g_device_ok = true;
 }
 
+// atoms of the split's complex (an identity split keeps no selection list)
+size_t complex_atoms(const mmpbsa_split *s)
+{
+    return s->identity ? (size_t)s->ctx->top.h.n : s->csel.size();
+}
+
 std::vector<double> gather_host(const double *xyz, const std::vector<int> &idx)
 {
     std::vector<double> out(idx.size() * 3);
@@ -111,7 +117,7 @@ void stage_frame(Worker &w, mmpbsa_split *s, const double *xyz, FrameCoords &f, 
                  bool complex_local = false)
 {
     if (complex_local) {  // host coordinates of the complex atoms only
-        const size_t nc = s->csel.size();
+        const size_t nc = complex_atoms(s);
         f.c = DBuf<double>(w.pool, 3 * nc + 3);
         HIP_OK(hipMemcpyAsync(f.c.get(), xyz, sizeof(double) * 3 * nc, hipMemcpyHostToDevice,
                               w.stream));
@@ -298,7 +304,7 @@ constexpr double kPbBytesPerNode = 320.0;
 double frame_bytes_estimate(const mmpbsa_split *s, const double *xyz, int method,
                             const mmpbsa_pb_params &p)
 {
-    const size_t n = s->csel.size();
+    const size_t n = complex_atoms(s);
     if (method != MMPBSA_METHOD_PB || n == 0) return 600.0 * (double)n * 3.0 + 4096.0;
     std::vector<double> hc = s->identity ? std::vector<double>(xyz, xyz + 3 * n)
                                          : gather_host(xyz, s->csel);
@@ -832,7 +838,7 @@ int mmpbsa_binding_energy_trajectory(mmpbsa_split *s, const char *path, int32_t 
         const int64_t st = std::max<int64_t>(1, stride);  // binding.rs:525
         const int64_t end = end_frame < 0 ? INT64_MAX : end_frame;
         const int n_all = ctx->top.h.n;
-        const size_t nc = s->csel.size();
+        const size_t nc = complex_atoms(s);
         const std::vector<int> &cb = s->ctop.h.bonds;  // (i, j, type), complex-local
         std::vector<double> full(3 * (size_t)n_all), box(6, 0.0);
         std::unique_ptr<DcdFile> dcd;
