@@ -111,6 +111,19 @@ typedef struct mmpbsa_pb_params {   /* pb_energy.rs:21-66 */
      * subsystem's bounding-box centre exactly as auto_grid centres
      * (pb_grid.rs:188-215).  Zero keeps the reference behaviour.           */
     int32_t fine_dims[3];
+    /* Extension: explicit focusing grids shared by all subsystems (a fixed
+     * frame, as a coarse-to-fine setup around an interface needs).  When
+     * coarse_dims[0] > 0 the coarse grid of pb_solve_with_focusing
+     * (pb_energy.rs:134-322) has these dimensions and coarse_spacing and is
+     * centred at coarse_center (origin = centre - (n-1) h / 2 per axis), in
+     * place of build_coarse_grid's; when fine_centered != 0 the fine grid
+     * (auto_grid's or fine_dims) is centred at fine_center in place of the
+     * subsystem's bounding-box centre.  Zeros keep the reference behaviour.  */
+    int32_t coarse_dims[3];
+    double coarse_spacing;
+    double coarse_center[3];
+    int32_t fine_centered;
+    double fine_center[3];
 } mmpbsa_pb_params;
 
 typedef struct mmpbsa_sa_params {   /* sa_energy.rs:12-32 */
@@ -231,6 +244,17 @@ int mmpbsa_frames_energy_device(mmpbsa_split *s, const double *d_xyz_frames, int
 int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params *gb,
                      const mmpbsa_sa_params *sa, mmpbsa_residue_contrib *rec_out,
                      mmpbsa_residue_contrib *lig_out);
+
+/* decompose_binding_energy (decomposition.rs:75) with a PB polar term in place
+ * of the GB-weighted share (decomposition.rs:311-352).  vdw, elec and sa are
+ * as in mmpbsa_decompose; polar is, per residue, the sum over its atoms of
+ * 0.5 q_i (dphi_cplx - dphi_iso)(r_i), dphi = phi_solv - phi_ref on each
+ * subsystem's fine grid (compute_pb_energy, pb_energy.rs:332-438).  The sum
+ * of polar over all residues is dG_PB, returned in *delta_pb (nullable).
+ * Extension: the reference has no PB decomposition. */
+int mmpbsa_decompose_pb(mmpbsa_split *s, const double *xyz, const mmpbsa_pb_params *pb,
+                        const mmpbsa_sa_params *sa, mmpbsa_residue_contrib *rec_out,
+                        mmpbsa_residue_contrib *lig_out, double *delta_pb);
 
 /* ---- topology ingest (host; no device needed) ----------------------------
  * parse_prmtop (crates/core/src/amber/prmtop.rs:436-733): %FLAG sections,

@@ -1562,7 +1562,7 @@ double orc_elec_energy(const orc_grid *g, const double *phi, int32_t n, const do
 static void pb_solve_focus(const orc_grid *fine, const double *fine_rho, const orc_grid *cg,
                            int32_t n, const double *xyz, const double *q, const double *radii,
                            const mmpbsa_pb_params *p, double ein, double eout, double kappa,
-                           int ionic, mmpbsa_pb_solve_info *info)
+                           int ionic, mmpbsa_pb_solve_info *info, double *phi_out)
 {
     const double bck = ionic ? kappa : 0.0;
     const int64_t NF = glen(fine);
@@ -1611,12 +1611,15 @@ static void pb_solve_focus(const orc_grid *fine, const double *fine_rho, const o
     info->iterations = rs.iterations;
     info->converged = rs.converged;
     info->final_residual = rs.final_residual;
+    if (phi_out) memcpy(phi_out, phi, sizeof(double) * NF);
     free(phi); free(ex); free(ey); free(ez); free(k2);
 }
 
-/* compute_pb_energy, pb_energy.rs:332-438 */
-void orc_pb_energy(const mmpbsa_topology_view *t, const double *xyz, const mmpbsa_pb_params *p,
-                   mmpbsa_pb_result *out)
+/* compute_pb_energy, pb_energy.rs:332-438, with the fine_dims and explicit
+ * focusing-grid extensions of mmpbsa_pb_params; atom_terms (nullable) gets
+ * 0.5 q_i (phi_solv - phi_ref)(r_i) per atom on the fine grid. */
+static void pb_energy_impl(const mmpbsa_topology_view *t, const double *xyz,
+                           const mmpbsa_pb_params *p, mmpbsa_pb_result *out, double *atom_terms)
 {
     memset(out, 0, sizeof(*out));
     const int32_t n = t->n_atoms;
@@ -1632,20 +1635,53 @@ void orc_pb_energy(const mmpbsa_topology_view *t, const double *xyz, const mmpbs
             fine.o[k] = 0.5 * (mn[k] + mx[k]) - 0.5 * ((double)(fine.n[k] - 1) * p->grid_spacing);
         }
     }
-    const int focus = p->fillratio > 0.0 && p->fscale > 1;
-    if (focus) orc_coarse_grid(n, xyz, p->grid_spacing, p->fillratio, p->fscale, &cg);
+    if (p->fine_centered)
+        for (int k = 0; k < 3; k++)
+            fine.o[k] = p->fine_center[k] - 0.5 * ((double)(fine.n[k] - 1) * p->grid_spacing);
+    const int explicit_cg = p->coarse_dims[0] > 0;
+    const int focus = explicit_cg || (p->fillratio > 0.0 && p->fscale > 1);
+    if (explicit_cg) {
+        for (int k = 0; k < 3; k++) {
+            cg.n[k] = p->coarse_dims[k];
+            cg.h[k] = p->coarse_spacing;
+            cg.o[k] = p->coarse_center[k] - 0.5 * ((double)(cg.n[k] - 1) * p->coarse_spacing);
+        }
+    } else if (focus) {
+        orc_coarse_grid(n, xyz, p->grid_spacing, p->fillratio, p->fscale, &cg);
+    }
     for (int k = 0; k < 3; k++) {
         out->fine_dims[k] = (int32_t)fine.n[k];
         out->coarse_dims[k] = focus ? (int32_t)cg.n[k] : 0;
     }
-    double *rho = (double *)malloc(sizeof(double) * glen(&fine));
+    const int64_t NF = glen(&fine);
+    double *rho = (double *)malloc(sizeof(double) * NF);
+    double *ps = atom_terms ? (double *)malloc(sizeof(double) * NF) : NULL;
+    double *pr = atom_terms ? (double *)malloc(sizeof(double) * NF) : NULL;
     orc_map_charges(&fine, n, xyz, t->charges, rho);
     pb_solve_focus(&fine, rho, focus ? &cg : NULL, n, xyz, t->charges, t->radii, p,
-                   p->solute_dielectric, p->solvent_dielectric, kappa, 1, &out->solv);
+                   p->solute_dielectric, p->solvent_dielectric, kappa, 1, &out->solv, ps);
     pb_solve_focus(&fine, rho, focus ? &cg : NULL, n, xyz, t->charges, t->radii, p,
-                   p->solute_dielectric, p->solute_dielectric, kappa, 0, &out->ref);
+                   p->solute_dielectric, p->solute_dielectric, kappa, 0, &out->ref, pr);
     out->total = out->solv.energy - out->ref.energy;
+    if (atom_terms)
+        for (int32_t a = 0; a < n; a++)
+            atom_terms[a] = 0.5 * t->charges[a] *
+                            (interp(&fine, ps, xyz + 3 * a) - interp(&fine, pr, xyz + 3 * a));
     free(rho);
+    free(ps);
+    free(pr);
+}
+
+void orc_pb_energy(const mmpbsa_topology_view *t, const double *xyz, const mmpbsa_pb_params *p,
+                   mmpbsa_pb_result *out)
+{
+    pb_energy_impl(t, xyz, p, out, NULL);
+}
+
+void orc_pb_atom_terms(const mmpbsa_topology_view *t, const double *xyz,
+                       const mmpbsa_pb_params *p, mmpbsa_pb_result *out, double *atom_terms)
+{
+    pb_energy_impl(t, xyz, p, out, atom_terms);
 }
 
 /* decompose_binding_energy_impl pair loop, decomposition.rs:240-283 */

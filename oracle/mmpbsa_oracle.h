@@ -96,9 +96,14 @@ void orc_restrict(const double *fine, const int64_t *fdims, double *coarse,
                   const int64_t *cdims, double scale);
 void orc_prolongate_add(const double *coarse, const int64_t *cdims, double *fine,
                         const int64_t *fdims);
-/* pb_energy.rs:332-438 (with the fine_dims extension) */
+/* pb_energy.rs:332-438 (with the fine_dims and explicit focusing-grid
+ * extensions) */
 void orc_pb_energy(const mmpbsa_topology_view *t, const double *xyz, const mmpbsa_pb_params *p,
                    mmpbsa_pb_result *out);
+/* the same, and per atom 0.5 q_i (phi_solv - phi_ref)(r_i) on the fine grid
+ * (the per-residue PB terms are sums of these) */
+void orc_pb_atom_terms(const mmpbsa_topology_view *t, const double *xyz,
+                       const mmpbsa_pb_params *p, mmpbsa_pb_result *out, double *atom_terms);
 /* decomposition.rs:240-283: receptor x ligand pairs into per-residue sums */
 void orc_decomp_pairs(const mmpbsa_topology_view *t, const double *xyz, int32_t n_rec,
                       const int32_t *rec_atoms, int32_t n_lig, const int32_t *lig_atoms,

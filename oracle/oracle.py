@@ -85,6 +85,7 @@ def lib():
                                     C.c_double]),
             "orc_prolongate_add": (None, [dp, C.POINTER(C.c_int64), dp, C.POINTER(C.c_int64)]),
             "orc_pb_energy": (None, [TV, dp, C.POINTER(PbParamsC), C.POINTER(PbResultC)]),
+            "orc_pb_atom_terms": (None, [TV, dp, C.POINTER(PbParamsC), C.POINTER(PbResultC), dp]),
             "orc_decomp_pairs": (None, [TV, dp, C.c_int32, ip, C.c_int32, ip, ip, dp, dp]),
             "orc_set_face_scan": (None, [C.c_int]),
             "orc_set_threads": (None, [C.c_int]),
@@ -113,9 +114,14 @@ def gb_params(**kw) -> GbParamsC:
 def pb_params(**kw) -> PbParamsC:
     p = PbParamsC(0.5, 10.0, 1.0, 80.0, 0.15, 300.0, 1.4, 2.0, 1e-6, 10000, 4.0, 8)
     for k, v in kw.items():
-        if k == "fine_dims":
+        if k in ("fine_dims", "coarse_dims"):
             for i in range(3):
-                p.fine_dims[i] = int(v[i])
+                getattr(p, k)[i] = int(v[i])
+        elif k in ("fine_center", "coarse_center"):
+            for i in range(3):
+                getattr(p, k)[i] = float(v[i])
+            if k == "fine_center":
+                p.fine_centered = 1
         else:
             setattr(p, k, v)
     return p
@@ -179,6 +185,47 @@ def pb_energy(top: AmberTopology, xyz, pb: PbParamsC) -> PbResultC:
     v = top.view()
     lib().orc_pb_energy(C.byref(v), xp, C.byref(pb), C.byref(out))
     return out
+
+
+def pb_atom_terms(top: AmberTopology, xyz, pb: PbParamsC):
+    """(PbResultC, per-atom 0.5 q (phi_solv - phi_ref)(r) on the fine grid)"""
+    x, xp = _d(xyz)
+    out = PbResultC()
+    terms = np.zeros(top.n_atoms)
+    v = top.view()
+    lib().orc_pb_atom_terms(C.byref(v), xp, C.byref(pb), C.byref(out),
+                            terms.ctypes.data_as(C.POINTER(C.c_double)))
+    return out, terms
+
+
+def pb_residue_terms(top: AmberTopology, xyz, rec_res, lig_res, pb: PbParamsC):
+    """Per-residue PB terms: for each requested residue, the sum over its atoms
+    of 0.5 q (dphi_complex - dphi_isolated) at the atom, dphi = phi_solv -
+    phi_ref, each subsystem on its own fine grid (compute_pb_energy,
+    pb_energy.rs:332-438, per atom).  Returns (receptor terms, ligand terms,
+    dG_PB) with the terms in the order of rec_res / lig_res."""
+    ctop, csel, ra, la = build_split(top, rec_res, lig_res)
+    cx = np.asarray(xyz, np.float64) if csel is None else extract_coords(xyz, csel)
+    rtop, ltop = extract_subtopology(ctop, ra), extract_subtopology(ctop, la)
+    oc, tc = pb_atom_terms(ctop, cx, pb)
+    orr, tr = pb_atom_terms(rtop, extract_coords(cx, ra), pb)
+    ol, tl = pb_atom_terms(ltop, extract_coords(cx, la), pb)
+    # per complex atom: complex term minus the same atom's term in its partner
+    d = np.zeros(ctop.n_atoms)
+    d[np.asarray(ra, np.int64)] += tc[np.asarray(ra, np.int64)] - tr
+    d[np.asarray(la, np.int64)] += tc[np.asarray(la, np.int64)] - tl
+    cptr = np.append(np.asarray(ctop.residue_pointers, np.int64), ctop.n_atoms)
+    per_res = np.array([d[cptr[r]:cptr[r + 1]].sum() for r in range(len(cptr) - 1)])
+    # requested residue -> complex residue: the complex keeps the selected
+    # residues in ascending original order (or all of them, unchanged)
+    if csel is None:
+        loc = {r: r for r in range(len(cptr) - 1)}
+    else:
+        cres = sorted(set(int(r) for r in rec_res) | set(int(r) for r in lig_res))
+        loc = {r: k for k, r in enumerate(cres)}
+    return (np.array([per_res[loc[int(r)]] for r in rec_res]),
+            np.array([per_res[loc[int(r)]] for r in lig_res]),
+            oc.total - orr.total - ol.total)
 
 
 # ---- frame orchestration -------------------------------------------------

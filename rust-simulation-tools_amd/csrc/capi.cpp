@@ -317,7 +317,10 @@ double frame_bytes_estimate(const mmpbsa_split *s, const double *xyz, int method
                      c = p.fine_dims[2] > 0 ? p.fine_dims[2] : a;
         nodes = a * b * c;
     }
-    if (p.fillratio > 0.0 && p.fscale > 1)
+    if (p.coarse_dims[0] > 0)
+        nodes += (double)p.coarse_dims[0] * std::max(p.coarse_dims[1], 1) *
+                 std::max(p.coarse_dims[2], 1);
+    else if (p.fillratio > 0.0 && p.fscale > 1)
         nodes += (double)coarse_grid(mn, mx, p.grid_spacing, p.fillratio, p.fscale).size();
     // the receptor and ligand grids are no larger than the complex's
     return 3.0 * kPbBytesPerNode * nodes;
@@ -735,84 +738,141 @@ int mmpbsa_frames_energy_device(mmpbsa_split *s, const double *d_xyz_frames, int
 }
 
 // decompose_binding_energy_impl (decomposition.rs:210-410)
-int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params *gbp,
-                     const mmpbsa_sa_params *sap, mmpbsa_residue_contrib *rec_out,
-                     mmpbsa_residue_contrib *lig_out)
+namespace {
+
+// decompose_binding_energy_impl (decomposition.rs:210-410).  The polar share
+// comes from GB weights (gbp != null, the reference's rule) or, as an
+// extension, from per-atom PB terms (pbp != null): 0.5 q_i (dphi_cplx -
+// dphi_iso)(r_i) with dphi = phi_solv - phi_ref, whose sum over all residues
+// is dG_PB (returned in *delta_pb).
+void decompose_impl(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params *gbp,
+                    const mmpbsa_pb_params *pbp, const mmpbsa_sa_params &sa,
+                    mmpbsa_residue_contrib *rec_out, mmpbsa_residue_contrib *lig_out,
+                    double *delta_pb)
 {
-    if (!s || (!rec_out && !s->rec_res.empty()) || (!lig_out && !s->lig_res.empty()))
-        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
-    mmpbsa_ctx *ctx = s->ctx;
-    return guarded(ctx, [&] {
-        check_xyz(xyz);
-        mmpbsa_gb_params gb;
-        mmpbsa_sa_params sa;
-        mmpbsa_gb_params_default(&gb);
-        mmpbsa_sa_params_default(&sa);
-        if (gbp) gb = *gbp;
-        if (sap) sa = *sap;
-        Worker &w = *ctx->w;
-        FrameCoords f;
-        stage_frame(w, s, xyz, f, false);
-        const int nr = (int)s->ra.size(), nl = (int)s->la.size(), na = nr + nl;
-        const PairTopo cv = s->ctop.view();
-        // per-atom vdW / elec halves (receptor block, then ligand block)
-        DBuf<double> vdw(w.pool, na + 1), elec(w.pool, na + 1);
-        {
-            const size_t wr = (size_t)decomp_chunks(nl) * std::max(nr, 1) * 4;
-            const size_t wl = (size_t)decomp_chunks(nr) * std::max(nl, 1) * 4;
-            DBuf<double> work(w.pool, std::max(wr, wl));
-            launch_decomp_pairs(f.dc, cv, s->d_ra.p, nr, s->d_la.p, nl, 1, s->cross_excl,
-                                work.get(), vdw.get(), elec.get(), w.stream);
-            launch_decomp_pairs(f.dc, cv, s->d_la.p, nl, s->d_ra.p, nr, 0, s->cross_excl,
-                                work.get(), vdw.get() + nr, elec.get() + nr, w.stream);
-        }
-        // GB shares
+    check_xyz(xyz);
+    Worker &w = *s->ctx->w;
+    FrameCoords f;
+    stage_frame(w, s, xyz, f, false);
+    const int nr = (int)s->ra.size(), nl = (int)s->la.size(), na = nr + nl;
+    const PairTopo cv = s->ctop.view();
+    // per-atom vdW / elec halves (receptor block, then ligand block)
+    DBuf<double> vdw(w.pool, na + 1), elec(w.pool, na + 1);
+    {
+        const size_t wr = (size_t)decomp_chunks(nl) * std::max(nr, 1) * 4;
+        const size_t wl = (size_t)decomp_chunks(nr) * std::max(nl, 1) * 4;
+        DBuf<double> work(w.pool, std::max(wr, wl));
+        launch_decomp_pairs(f.dc, cv, s->d_ra.p, nr, s->d_la.p, nl, 1, s->cross_excl,
+                            work.get(), vdw.get(), elec.get(), w.stream);
+        launch_decomp_pairs(f.dc, cv, s->d_la.p, nl, s->d_ra.p, nr, 0, s->cross_excl,
+                            work.get(), vdw.get() + nr, elec.get() + nr, w.stream);
+    }
+    DBuf<double> pol(w.pool, na + 1);
+    if (gbp) {
+        const mmpbsa_gb_params &gb = *gbp;
         DBuf<double> Rc(w.pool, s->ctop.h.n + 1), Rr(w.pool, nr + 1), Rl(w.pool, nl + 1);
         const double ec = run_gb(w, s->ctop, f.dc, gb, Rc.get(), nullptr);
         const double er = run_gb(w, s->rtop, f.r.get(), gb, Rr.get(), nullptr);
         const double el = run_gb(w, s->ltop, f.l.get(), gb, Rl.get(), nullptr);
         const double fac = -0.5 * (1.0 / gb.solute_dielectric - 1.0 / gb.solvent_dielectric);
-        DBuf<double> wts(w.pool, na + 1), scal(w.pool, 2), pol(w.pool, na + 1);
+        DBuf<double> wts(w.pool, na + 1), scal(w.pool, 2);
         launch_gb_weight(s->d_ra.p, nr, s->ctop.qa.p, Rc.get(), Rr.get(), fac, wts.get(), w.stream);
         launch_gb_weight(s->d_la.p, nl, s->ctop.qa.p, Rc.get(), Rl.get(), fac, wts.get() + nr,
                          w.stream);
         launch_fill_f64(scal.get(), ec - er - el, 1, w.stream);
         launch_sum_partials(wts.get(), na, 1, 1, scal.get() + 1, w.stream);
         launch_gb_share(wts.get(), na, scal.get(), pol.get(), w.stream);
-        // SA shares
-        DBuf<double> Sc(w.pool, s->ctop.h.n + 1), Sr(w.pool, nr + 1), Sl(w.pool, nl + 1);
-        double ts, tt;
-        run_sa(w, s->ctop, f.dc, f.hcp, sa, Sc.get(), nullptr, &ts, &tt);
-        run_sa(w, s->rtop, f.r.get(), f.hr.data(), sa, Sr.get(), nullptr, &ts, &tt);
-        run_sa(w, s->ltop, f.l.get(), f.hl.data(), sa, Sl.get(), nullptr, &ts, &tt);
-        DBuf<double> sav(w.pool, na + 1);
-        launch_sa_share(s->d_ra.p, nr, Sc.get(), Sr.get(), sa.surface_tension, sav.get(), w.stream);
-        launch_sa_share(s->d_la.p, nl, Sc.get(), Sl.get(), sa.surface_tension, sav.get() + nr,
-                        w.stream);
-        // per-residue sums (complex numbering)
-        const int ncr = s->ctop.h.n_res;
-        DBuf<double> res(w.pool, 4 * (size_t)std::max(ncr, 1));
-        launch_segment_sum(vdw.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get(), w.stream);
-        launch_segment_sum(elec.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + ncr, w.stream);
-        launch_segment_sum(pol.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + 2 * ncr, w.stream);
-        launch_segment_sum(sav.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + 3 * ncr, w.stream);
-        std::vector<double> h(4 * (size_t)std::max(ncr, 1));
-        HIP_OK(hipMemcpyAsync(h.data(), res.get(), sizeof(double) * 4 * ncr, hipMemcpyDeviceToHost,
-                              w.stream));
-        w.sync();
-        auto fill = [&](const std::vector<int> &orig, const std::vector<int> &loc,
-                        mmpbsa_residue_contrib *o) {
-            for (size_t k = 0; k < orig.size(); ++k) {
-                const int r = loc[k];
-                o[k].residue_index = orig[k];
-                o[k].vdw = h[r];
-                o[k].elec = h[ncr + r];
-                o[k].polar = h[2 * ncr + r];
-                o[k].sa = h[3 * ncr + r];
-            }
-        };
-        fill(s->rec_res, s->rec_loc, rec_out);
-        fill(s->lig_res, s->lig_loc, lig_out);
+    } else {
+        // the three subsystems' PB solves in one batch, with per-atom terms
+        DBuf<double> Tc(w.pool, s->ctop.h.n + 1), Tr(w.pool, nr + 1), Tl(w.pool, nl + 1);
+        mmpbsa_pb_result res[3];
+        std::vector<PbJob> jobs(3);
+        const DevTopo *tops[3] = {&s->ctop, &s->rtop, &s->ltop};
+        const double *dx[3] = {f.dc, f.r.get(), f.l.get()};
+        const double *hx[3] = {f.hcp, f.hr.data(), f.hl.data()};
+        double *tt[3] = {Tc.get(), Tr.get(), Tl.get()};
+        for (int k = 0; k < 3; ++k) {
+            jobs[k].t = tops[k];
+            jobs[k].d_xyz = dx[k];
+            jobs[k].h_xyz = hx[k];
+            jobs[k].p = *pbp;
+            jobs[k].out = &res[k];
+            jobs[k].d_atom_terms = tt[k];
+        }
+        run_pb_jobs(w, jobs, true);
+        launch_sa_share(s->d_ra.p, nr, Tc.get(), Tr.get(), 1.0, pol.get(), w.stream);
+        launch_sa_share(s->d_la.p, nl, Tc.get(), Tl.get(), 1.0, pol.get() + nr, w.stream);
+        if (delta_pb) *delta_pb = res[0].total - res[1].total - res[2].total;
+    }
+    // SA shares
+    DBuf<double> Sc(w.pool, s->ctop.h.n + 1), Sr(w.pool, nr + 1), Sl(w.pool, nl + 1);
+    double ts, tt;
+    run_sa(w, s->ctop, f.dc, f.hcp, sa, Sc.get(), nullptr, &ts, &tt);
+    run_sa(w, s->rtop, f.r.get(), f.hr.data(), sa, Sr.get(), nullptr, &ts, &tt);
+    run_sa(w, s->ltop, f.l.get(), f.hl.data(), sa, Sl.get(), nullptr, &ts, &tt);
+    DBuf<double> sav(w.pool, na + 1);
+    launch_sa_share(s->d_ra.p, nr, Sc.get(), Sr.get(), sa.surface_tension, sav.get(), w.stream);
+    launch_sa_share(s->d_la.p, nl, Sc.get(), Sl.get(), sa.surface_tension, sav.get() + nr,
+                    w.stream);
+    // per-residue sums (complex numbering)
+    const int ncr = s->ctop.h.n_res;
+    DBuf<double> res(w.pool, 4 * (size_t)std::max(ncr, 1));
+    launch_segment_sum(vdw.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get(), w.stream);
+    launch_segment_sum(elec.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + ncr, w.stream);
+    launch_segment_sum(pol.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + 2 * ncr, w.stream);
+    launch_segment_sum(sav.get(), s->d_seg_pos.p, s->d_seg_ptr.p, ncr, res.get() + 3 * ncr, w.stream);
+    std::vector<double> h(4 * (size_t)std::max(ncr, 1));
+    HIP_OK(hipMemcpyAsync(h.data(), res.get(), sizeof(double) * 4 * ncr, hipMemcpyDeviceToHost,
+                          w.stream));
+    w.sync();
+    auto fill = [&](const std::vector<int> &orig, const std::vector<int> &loc,
+                    mmpbsa_residue_contrib *o) {
+        for (size_t k = 0; k < orig.size(); ++k) {
+            const int r = loc[k];
+            o[k].residue_index = orig[k];
+            o[k].vdw = h[r];
+            o[k].elec = h[ncr + r];
+            o[k].polar = h[2 * ncr + r];
+            o[k].sa = h[3 * ncr + r];
+        }
+    };
+    fill(s->rec_res, s->rec_loc, rec_out);
+    fill(s->lig_res, s->lig_loc, lig_out);
+}
+
+}  // namespace
+
+int mmpbsa_decompose(mmpbsa_split *s, const double *xyz, const mmpbsa_gb_params *gbp,
+                     const mmpbsa_sa_params *sap, mmpbsa_residue_contrib *rec_out,
+                     mmpbsa_residue_contrib *lig_out)
+{
+    if (!s || (!rec_out && !s->rec_res.empty()) || (!lig_out && !s->lig_res.empty()))
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(s->ctx, [&] {
+        mmpbsa_gb_params gb;
+        mmpbsa_sa_params sa;
+        mmpbsa_gb_params_default(&gb);
+        mmpbsa_sa_params_default(&sa);
+        if (gbp) gb = *gbp;
+        if (sap) sa = *sap;
+        decompose_impl(s, xyz, &gb, nullptr, sa, rec_out, lig_out, nullptr);
+    });
+}
+
+int mmpbsa_decompose_pb(mmpbsa_split *s, const double *xyz, const mmpbsa_pb_params *pbp,
+                        const mmpbsa_sa_params *sap, mmpbsa_residue_contrib *rec_out,
+                        mmpbsa_residue_contrib *lig_out, double *delta_pb)
+{
+    if (!s || (!rec_out && !s->rec_res.empty()) || (!lig_out && !s->lig_res.empty()))
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(s->ctx, [&] {
+        mmpbsa_pb_params pb;
+        mmpbsa_sa_params sa;
+        mmpbsa_pb_params_default(&pb);
+        mmpbsa_sa_params_default(&sa);
+        if (pbp) pb = *pbp;
+        if (sap) sa = *sap;
+        decompose_impl(s, xyz, nullptr, &pb, sa, rec_out, lig_out, delta_pb);
     });
 }
 

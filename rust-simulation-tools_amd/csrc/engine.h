@@ -159,6 +159,7 @@ struct PbJob {
     const double *h_xyz = nullptr;  // host copy, or null (bounding box on the device)
     mmpbsa_pb_params p{};
     mmpbsa_pb_result *out = nullptr;
+    double *d_atom_terms = nullptr;  // device [n] 0.5 q (phi_solv - phi_ref)(r), or null
     struct State;
     std::shared_ptr<State> st;
 };

@@ -563,6 +563,30 @@ void launch_energy_gather(const GridDesc &g, const double *phi, int n, const dou
     LAUNCH_OK();
 }
 
+// Per-atom PB terms 0.5 q_i (phi_solv - phi_ref)(r_i); their sum over the
+// atoms is the subsystem's total of compute_pb_energy (pb_energy.rs:423-430)
+__global__ __launch_bounds__(256) void atom_terms_kernel(GridDesc g,
+                                                         const double *__restrict__ phi_s,
+                                                         const double *__restrict__ phi_r, int n,
+                                                         const double *__restrict__ xyz,
+                                                         const double *__restrict__ q,
+                                                         double *__restrict__ out)
+{
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= n) return;
+    const double x = xyz[3 * a], y = xyz[3 * a + 1], z = xyz[3 * a + 2];
+    out[a] = 0.5 * q[a] * (interp3(g, phi_s, x, y, z) - interp3(g, phi_r, x, y, z));
+}
+
+void launch_pb_atom_terms(const GridDesc &g, const double *phi_solv, const double *phi_ref, int n,
+                          const double *xyz, const double *q, double *out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(atom_terms_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, g, phi_solv,
+                       phi_ref, n, xyz, q, out);
+    LAUNCH_OK();
+}
+
 // max |v| ignoring NaN (as f64::max does), pb_energy.rs:188-194
 __global__ __launch_bounds__(256) void maxabs_kernel(const double *__restrict__ v, int64_t n,
                                                      double *__restrict__ partials)

@@ -109,6 +109,9 @@ void launch_interp_boundary(const GridDesc &fine, const GridDesc &coarse, const 
                             double *phi, hipStream_t s);
 void launch_energy_gather(const GridDesc &g, const double *phi, int n, const double *xyz,
                           const double *q, double *partials, int *n_blocks, hipStream_t s);
+// out[a] = 0.5 q_a (phi_solv - phi_ref)(r_a)
+void launch_pb_atom_terms(const GridDesc &g, const double *phi_solv, const double *phi_ref, int n,
+                          const double *xyz, const double *q, double *out, hipStream_t s);
 void launch_maxabs(const double *v, int64_t n, double *partials, int *n_blocks, hipStream_t s);
 void launch_max_partials(const double *partials, int n_blocks, double *out, hipStream_t s);
 void launch_fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);

@@ -720,8 +720,28 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
             S.fine.oy = o[1];
             S.fine.oz = o[2];
         }
-        S.focus = p.fillratio > 0.0 && p.fscale > 1;
-        if (S.focus) S.cg = coarse_grid(S.mn, S.mx, p.grid_spacing, p.fillratio, p.fscale);
+        // explicit frames (fine_center, coarse_dims): origin = centre - (n-1) h / 2
+        if (p.fine_centered) {
+            S.fine.ox = p.fine_center[0] - 0.5 * ((double)(S.fine.nx - 1) * p.grid_spacing);
+            S.fine.oy = p.fine_center[1] - 0.5 * ((double)(S.fine.ny - 1) * p.grid_spacing);
+            S.fine.oz = p.fine_center[2] - 0.5 * ((double)(S.fine.nz - 1) * p.grid_spacing);
+        }
+        const bool explicit_cg = p.coarse_dims[0] > 0;
+        S.focus = explicit_cg || (p.fillratio > 0.0 && p.fscale > 1);
+        if (explicit_cg) {
+            if (p.coarse_dims[1] <= 0 || p.coarse_dims[2] <= 0 || !(p.coarse_spacing > 0.0))
+                throw DeviceError(MMPBSA_ERR_INVALID_ARG,
+                                  "coarse_dims and coarse_spacing must be positive");
+            S.cg.nx = p.coarse_dims[0];
+            S.cg.ny = p.coarse_dims[1];
+            S.cg.nz = p.coarse_dims[2];
+            S.cg.h = p.coarse_spacing;
+            S.cg.ox = p.coarse_center[0] - 0.5 * ((double)(S.cg.nx - 1) * p.coarse_spacing);
+            S.cg.oy = p.coarse_center[1] - 0.5 * ((double)(S.cg.ny - 1) * p.coarse_spacing);
+            S.cg.oz = p.coarse_center[2] - 0.5 * ((double)(S.cg.nz - 1) * p.coarse_spacing);
+        } else if (S.focus) {
+            S.cg = coarse_grid(S.mn, S.mx, p.grid_spacing, p.fillratio, p.fscale);
+        }
         if (S.fine.size() > (int64_t)1 << 31 || (S.focus && S.cg.size() > (int64_t)1 << 31))
             throw DeviceError(MMPBSA_ERR_TOO_LARGE, "PB grid above 2^31 nodes");
         j->out->fine_dims[0] = S.fine.nx;
@@ -849,6 +869,9 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
         int k = 0;
         for (PbJob *j : act) {
             PbJob::State &S = *j->st;
+            if (j->d_atom_terms)
+                launch_pb_atom_terms(S.fine, S.side[0].phi.get(), S.side[1].phi.get(), j->t->h.n,
+                                     j->d_xyz, j->t->q.p, j->d_atom_terms, w.stream);
             for (auto &sd : S.side) {
                 sd.fm = MapSet();
                 parts.emplace_back(w.pool, std::max(1, ceil_div(j->t->h.n, 256)));

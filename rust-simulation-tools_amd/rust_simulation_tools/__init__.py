@@ -27,7 +27,7 @@ __all__ = [
     "SaEnergy", "PbEnergy", "FrameEnergy", "BindingResult", "ResidueContribution",
     "DecompositionResult", "compute_mm_energy", "compute_gb_energy", "compute_sa_energy",
     "compute_pb_energy", "compute_binding_energy_single_frame", "compute_binding_energy_frames",
-    "decompose_binding_energy", "extract_subtopology",
+    "decompose_binding_energy", "decompose_binding_energy_pb", "extract_subtopology",
 ]
 
 
@@ -66,14 +66,17 @@ class GbParams:
 
 
 class PbParams:
-    """lib.rs:1542-1626; defaults pb_energy.rs:49-66.  ``fine_dims`` (no
-    counterpart in the reference) fixes the fine grid dimensions, centred as
-    auto_grid centres."""
+    """lib.rs:1542-1626; defaults pb_energy.rs:49-66.  Extensions (no
+    counterpart in the reference): ``fine_dims`` fixes the fine grid
+    dimensions, centred as auto_grid centres or at ``fine_center``;
+    ``coarse_dims`` / ``coarse_spacing`` / ``coarse_center`` give an explicit
+    focusing coarse grid.  Explicit grids serve all three subsystems."""
 
     def __init__(self, grid_spacing=None, grid_buffer=None, solute_dielectric=None,
                  solvent_dielectric=None, salt_concentration=None, temperature=None,
                  probe_radius=None, ion_radius=None, tolerance=None, max_iterations=None,
-                 fillratio=None, fscale=None, fine_dims=None):
+                 fillratio=None, fscale=None, fine_dims=None, fine_center=None,
+                 coarse_dims=None, coarse_spacing=None, coarse_center=None):
         self._c = PbParamsC()
         load_library().mmpbsa_pb_params_default(C.byref(self._c))
         _opt(self._c, grid_spacing=grid_spacing, grid_buffer=grid_buffer,
@@ -85,6 +88,17 @@ class PbParams:
         if fine_dims is not None:
             for i in range(3):
                 self._c.fine_dims[i] = int(fine_dims[i])
+        if fine_center is not None:
+            self._c.fine_centered = 1
+            for i in range(3):
+                self._c.fine_center[i] = float(fine_center[i])
+        if coarse_dims is not None:
+            if coarse_spacing is None or coarse_center is None:
+                raise ValueError("coarse_dims needs coarse_spacing and coarse_center")
+            self._c.coarse_spacing = float(coarse_spacing)
+            for i in range(3):
+                self._c.coarse_dims[i] = int(coarse_dims[i])
+                self._c.coarse_center[i] = float(coarse_center[i])
 
     grid_spacing = property(lambda s: s._c.grid_spacing)
     solute_dielectric = property(lambda s: s._c.solute_dielectric)
@@ -587,6 +601,31 @@ def decompose_binding_energy(topology: AmberTopology, coords, receptor_residues,
          for k in range(len(receptor_residues))],
         [ResidueContribution(lig[k], lab[int(lig[k].residue_index)])
          for k in range(len(ligand_residues))])
+
+
+def decompose_binding_energy_pb(topology: AmberTopology, coords, receptor_residues,
+                                ligand_residues, pb_params=None, sa_params=None):
+    """decompose_binding_energy with per-residue PB polar terms (an extension:
+    the reference decomposes the GB term only).  Each residue's polar is the
+    sum over its atoms of 0.5 q (dphi_complex - dphi_isolated) at the atom,
+    dphi = phi_solvated - phi_reference; the polars of all residues sum to
+    dG_PB.  Returns (DecompositionResult, dG_PB)."""
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    s = c.split(receptor_residues, ligand_residues)
+    p = (pb_params or PbParams())._c
+    a = (sa_params or SaParams())._c
+    rec = (ResidueContribC * max(len(receptor_residues), 1))()
+    lig = (ResidueContribC * max(len(ligand_residues), 1))()
+    d = C.c_double(0.0)
+    _check(c.lib.mmpbsa_decompose_pb(s, _dp(x), C.byref(p), C.byref(a), rec, lig, C.byref(d)),
+           c.lib, c.h)
+    lab = topology.residue_labels
+    return DecompositionResult(
+        [ResidueContribution(rec[k], lab[int(rec[k].residue_index)])
+         for k in range(len(receptor_residues))],
+        [ResidueContribution(lig[k], lab[int(lig[k].residue_index)])
+         for k in range(len(ligand_residues))]), d.value
 
 
 def extract_subtopology(topology: AmberTopology, atom_indices) -> AmberTopology:

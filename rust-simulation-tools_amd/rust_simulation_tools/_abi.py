@@ -98,6 +98,11 @@ class PbParamsC(C.Structure):
         ("fillratio", C.c_double),
         ("fscale", C.c_int64),
         ("fine_dims", C.c_int32 * 3),
+        ("coarse_dims", C.c_int32 * 3),
+        ("coarse_spacing", C.c_double),
+        ("coarse_center", C.c_double * 3),
+        ("fine_centered", C.c_int32),
+        ("fine_center", C.c_double * 3),
     ]
 
 
@@ -224,6 +229,9 @@ SIGNATURES = {
                                               C.POINTER(FrameTermsC)]),
     "mmpbsa_decompose": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), C.POINTER(SaParamsC),
                                    C.POINTER(ResidueContribC), C.POINTER(ResidueContribC)]),
+    "mmpbsa_decompose_pb": (C.c_int, [P, c_double_p, C.POINTER(PbParamsC), C.POINTER(SaParamsC),
+                                      C.POINTER(ResidueContribC), C.POINTER(ResidueContribC),
+                                      c_double_p]),
     "mmpbsa_prmtop_read": (C.c_int, [C.c_char_p, C.POINTER(P)]),
     "mmpbsa_prmtop_last_error": (C.c_char_p, []),
     "mmpbsa_prmtop_destroy": (C.c_int, [P]),

@@ -225,10 +225,48 @@ def noisy_frame(base: np.ndarray, seed: int, frame: int, sigma: float = 0.3) -> 
 def config_system(config: int, complex_index: int = 0):
     """The receptor-ligand complex of a benchmark configuration.
     config 0: 2,500 atoms (2,300 + 200) in a 22 A sphere.
-    config 1: 25,000 atoms (23,500 + 1,500) in a 45 A sphere."""
+    config 1: 25,000 atoms (23,500 + 1,500) in a 45 A sphere.
+    config 4: 150,000 atoms (142,500 + 7,500), ~9,400 residues of 16 atoms,
+    in the 16 A^3-per-atom sphere (R ~ 82 A)."""
     s = seed_for(config, complex_index)
     if config == 0:
         return build_complex(2500, 200, 22.0, s) + (s,)
     if config == 1:
         return build_complex(25000, 1500, 45.0, s) + (s,)
+    if config == 4:
+        return build_complex(150000, 7500, None, s) + (s,)
     raise ValueError(f"no fixed system for configuration {config}")
+
+
+def interface_center(xyz, rec_atoms, lig_atoms, cutoff: float = 6.0) -> np.ndarray:
+    """Centroid of the interface: the receptor atoms within ``cutoff`` of a
+    ligand atom and the ligand atoms within ``cutoff`` of a receptor atom.
+    Falls back to the ligand centroid when the partners do not touch."""
+    from scipy.spatial import cKDTree
+
+    x = np.asarray(xyz, np.float64)
+    r, l = x[np.asarray(rec_atoms)], x[np.asarray(lig_atoms)]
+    if r.shape[0] == 0 or l.shape[0] == 0:
+        return (l if l.shape[0] else r).mean(axis=0)
+    near_r = np.isfinite(cKDTree(l).query(r, k=1, distance_upper_bound=cutoff)[0])
+    near_l = np.isfinite(cKDTree(r).query(l, k=1, distance_upper_bound=cutoff)[0])
+    pts = np.concatenate([r[near_r], l[near_l]])
+    return (pts if pts.shape[0] else l).mean(axis=0)
+
+
+def focusing_grids(xyz, rec_atoms, lig_atoms, coarse_dims=193, coarse_spacing=1.0,
+                   fine_dims=321, fine_spacing=0.4, cutoff: float = 6.0) -> dict:
+    """Explicit coarse-to-fine PB grids shared by complex, receptor and
+    ligand: a coarse grid centred on the complex's bounding box and a fine
+    grid centred on the interface centroid (PbParams keyword arguments; the
+    large-decomposition workload uses the defaults, 193^3 at 1.0 A then
+    321^3 at 0.4 A)."""
+    x = np.asarray(xyz, np.float64)
+    sel = np.concatenate([np.asarray(rec_atoms), np.asarray(lig_atoms)])
+    mn, mx = x[sel].min(axis=0), x[sel].max(axis=0)
+    cd = (coarse_dims,) * 3 if np.isscalar(coarse_dims) else tuple(coarse_dims)
+    fd = (fine_dims,) * 3 if np.isscalar(fine_dims) else tuple(fine_dims)
+    return dict(coarse_dims=cd, coarse_spacing=float(coarse_spacing),
+                coarse_center=tuple(0.5 * (mn + mx)), fine_dims=fd,
+                grid_spacing=float(fine_spacing),
+                fine_center=tuple(interface_center(x, rec_atoms, lig_atoms, cutoff)))

@@ -428,3 +428,85 @@ def test_decomposition_sums(oracle):
     assert abs(pol - dgb) < 1e-8
     sa = sum(d["sa"] for d in allr)
     assert abs(sa - fe["delta_sa"]) < 1e-9
+
+
+def _pb_small(**kw):
+    base = dict(grid_spacing=1.0, grid_buffer=4.0, solute_dielectric=4.0,
+                solvent_dielectric=80.0, salt_concentration=0.15, tolerance=1e-6)
+    base.update(kw)
+    return base
+
+
+def test_pb_explicit_grids_match_auto(oracle):
+    """Explicit focusing grids set to the geometry auto_grid and
+    build_coarse_grid (pb_grid.rs:188-215, pb_energy.rs:134-148) would pick
+    reproduce the reference path bit for bit; a shifted centre does not."""
+    from rust_simulation_tools import synthetic
+    top, x, _, _ = synthetic.build_complex(300, 40, seed=91)
+    p0 = oracle.pb_params(**_pb_small())
+    r0 = oracle.pb_energy(top, x, p0)
+    mn, mx = x.min(axis=0), x.max(axis=0)
+    c = 0.5 * (mn + mx)
+    cs = 1.0 * 8
+    cd = [int(math.ceil((mx[k] - mn[k]) * 4.0 / cs)) + 1 for k in range(3)]
+    cd = [m + 1 if m % 2 == 0 else m for m in cd]
+    assert list(r0.coarse_dims) == cd
+    p1 = oracle.pb_params(**_pb_small(fine_dims=tuple(r0.fine_dims), fine_center=tuple(c),
+                                      coarse_dims=tuple(cd), coarse_spacing=cs,
+                                      coarse_center=tuple(c)))
+    r1 = oracle.pb_energy(top, x, p1)
+    assert r1.total == r0.total and r1.solv.energy == r0.solv.energy
+    assert list(r1.coarse_dims) == cd and list(r1.fine_dims) == list(r0.fine_dims)
+    p2 = oracle.pb_params(**_pb_small(fine_dims=tuple(r0.fine_dims),
+                                      fine_center=tuple(c + 0.37), coarse_dims=tuple(cd),
+                                      coarse_spacing=cs, coarse_center=tuple(c)))
+    assert oracle.pb_energy(top, x, p2).total != r0.total
+
+
+def test_pb_residue_terms_sum_to_delta_pb(oracle):
+    """The per-atom terms 0.5 q (phi_solv - phi_ref)(r) sum, over all residues
+    of both partners, to dG_PB = G_complex - G_receptor - G_ligand, each G from
+    compute_pb_energy's own gather (pb_solver.rs:1174-1186); the order of the
+    requested residues only permutes the output (to rounding)."""
+    from rust_simulation_tools import synthetic
+    top, x, rec, lig = synthetic.build_complex(400, 48, seed=92)
+    p = oracle.pb_params(**_pb_small())
+    tr, tl, d = oracle.pb_residue_terms(top, x, rec, lig, p)
+    assert abs(tr.sum() + tl.sum() - d) < 1e-9 * max(1.0, abs(d))
+    assert np.abs(tr).max() > 1e-3 and d != 0.0
+    tr2, tl2, d2 = oracle.pb_residue_terms(top, x, list(rec)[::-1], lig, p)
+    # (the receptor's atoms are taken in the requested residue order, so the
+    # sums round differently)
+    assert abs(d2 - d) < 1e-10 * abs(d)
+    assert np.allclose(tr2, tr[::-1], rtol=0, atol=1e-9) and np.allclose(tl2, tl, rtol=0, atol=1e-9)
+
+
+def test_pb_residue_terms_far_apart_closed_form(oracle):
+    """Partners 40 A apart on one shared explicit grid, at 0.15 M salt (Debye
+    length ~8 A): in the solvated medium the far partner's field is screened
+    away, while the reference medium (eps_out = eps_in, no ions) keeps its
+    unscreened Coulomb field.  So each atom's term tends to
+    -0.5 q_i sum_j C q_j / (eps_in r_ij) over the other partner's atoms, and
+    every residue's PB term to the sum of those (within ~1%, the remainder
+    being the boundary and cavity effects that fall off with distance)."""
+    from rust_simulation_tools import synthetic
+    top, x, rec, lig = synthetic.build_complex(160, 32, seed=93)
+    ra, la = top.build_selection(rec), top.build_selection(lig)
+    x = x.copy()
+    x[la] += np.array([40.0, 0.0, 0.0]) - (x[la].mean(axis=0) - x[ra].mean(axis=0))
+    c = 0.5 * (x.min(axis=0) + x.max(axis=0))
+    p = oracle.pb_params(**_pb_small(fine_dims=(73, 33, 33), fine_center=tuple(c),
+                                     coarse_dims=(37, 17, 17), coarse_spacing=4.0,
+                                     coarse_center=tuple(c)))
+    tr, tl, d = oracle.pb_residue_terms(top, x, rec, lig, p)
+    q = np.asarray(top.charges)
+    r = np.linalg.norm(x[ra][:, None, :] - x[la][None, :, :], axis=2)
+    pair = 332.0522 * q[ra][:, None] * q[la][None, :] / (4.0 * r)
+    ptr = np.append(np.asarray(top.residue_pointers), top.n_atoms)
+    ares = np.searchsorted(ptr, np.arange(top.n_atoms), side="right") - 1
+    er = np.array([-0.5 * pair.sum(axis=1)[ares[ra] == k].sum() for k in rec])
+    el = np.array([-0.5 * pair.sum(axis=0)[ares[la] == k].sum() for k in lig])
+    scale = max(np.abs(er).max(), np.abs(el).max())
+    assert scale > 0.1
+    assert np.abs(tr - er).max() < 0.02 * scale and np.abs(tl - el).max() < 0.02 * scale
+    assert abs(tr.sum() + tl.sum() - d) < 1e-9 * max(1.0, abs(d))
