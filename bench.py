@@ -43,6 +43,10 @@ AUTO_PB = dict(grid_spacing=0.5, grid_buffer=10.0, solute_dielectric=4.0,
                solvent_dielectric=80.0, salt_concentration=0.15)
 
 
+def _note(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def _complexes(rst, synthetic, config, sizes, frames, rng):
     """(topology, frames nm, receptor residues, ligand residues) per size, with
     a ligand fraction drawn from U(3%, 10%)."""
@@ -134,6 +138,44 @@ def serving_leg(rst, synthetic, check_oracle):
     return out
 
 
+def large_decomposition_leg(rst, synthetic):
+    """The 150,000-atom complex (142,500 + 7,500 atoms; 8,907 + 469 residues of
+    16 atoms): one frame's MM-PBSA and its per-residue decomposition with PB
+    polar terms, on explicit focusing grids shared by the three subsystems
+    (193^3 at 1.0 A centred on the complex, then 321^3 at 0.4 A centred on the
+    interface), beside the GB decomposition of the same frame."""
+    top, base, rec, lig, seed = synthetic.config_system(4)
+    x = synthetic.noisy_frame(base, seed, 0)
+    _note("large: system built")
+    ra, la = top.build_selection(rec), top.build_selection(lig)
+    kw = dict(solute_dielectric=4.0, solvent_dielectric=80.0, salt_concentration=0.15,
+              **synthetic.focusing_grids(x, ra, la))
+    pb = rst.PbParams(**kw)
+    xn = x / 10.0
+    ts = time.perf_counter()
+    rst.decompose_binding_energy(top, xn, rec, lig)  # topology upload, split, first use
+    first = time.perf_counter() - ts
+    _note(f"large: first GB decomposition {first:.2f} s")
+    ts = time.perf_counter()
+    rst.decompose_binding_energy(top, xn, rec, lig)
+    t_gb = time.perf_counter() - ts
+    ts = time.perf_counter()
+    e = rst.compute_binding_energy_single_frame(top, xn, rec, lig, pb_params=pb)
+    t_frame = time.perf_counter() - ts
+    _note(f"large: PB frame {t_frame:.2f} s")
+    ts = time.perf_counter()
+    d, dpb = rst.decompose_binding_energy_pb(top, xn, rec, lig, pb_params=pb)
+    t_dec = time.perf_counter() - ts
+    _note(f"large: PB decomposition {t_dec:.2f} s")
+    pol = sum(c.polar for c in d.receptor_residues() + d.ligand_residues())
+    return {"workload": "150,000-atom complex, 9,376 residues; PB 193^3 at 1.0 A then 321^3 "
+                        "at 0.4 A on the interface centroid; 1 frame",
+            "frame_mmpbsa_s": t_frame, "pb_decomposition_s": t_dec, "gb_decomposition_s": t_gb,
+            "first_call_s": first, "delta_total": e.delta_total, "delta_pb": dpb,
+            "residue_sum_minus_delta_pb": pol - dpb,
+            "frame_delta_polar_minus_decomposition_delta_pb": e.delta_polar - dpb}
+
+
 def measure_traffic(frames_per_step, grid_threads):
     """HBM bytes per launch of the timed smoother passes from two rocprofv3
     counter runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of one batched
@@ -189,6 +231,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-many", action="store_true",
                     help="skip the many-complex legs (uneven batch, serving)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 150,000-atom decomposition leg")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the 150,000-atom decomposition leg and print its JSON")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs behind roofline.traffic")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -216,6 +262,10 @@ def main():
             raise SystemExit("copy calibration failed")
         lib.mmpbsa_device_free(src)
         lib.mmpbsa_device_free(dst)
+
+    if args.large_only:
+        print(json.dumps({"large_decomposition": large_decomposition_leg(rst, synthetic)}))
+        return
 
     top, base, rec, lig, seed = synthetic.config_system(1)
     B = args.frames_per_step
@@ -290,6 +340,8 @@ def main():
     if not args.no_extras and not args.no_many:
         extras["uneven_batch"] = uneven_leg(rst, synthetic)
         extras["serving"] = serving_leg(rst, synthetic, not args.no_cpu)
+    if not args.no_extras and not args.no_large:
+        extras["large_decomposition"] = large_decomposition_leg(rst, synthetic)
 
     plain_lat = []
     for f in range(args.plain_frames):
