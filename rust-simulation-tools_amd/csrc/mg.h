@@ -92,6 +92,17 @@ int smooth_zc(int nz);             // z-chunk length for a level of nz planes
 // prolongation to the source while loading it (first post-smoothing pass).
 void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
                int max_blocks, const MgStatus *st, hipStream_t s);
+// Temporally blocked smoother: one pass = three red-black sweeps (a whole
+// smoothing phase of the V-cycle), same buffers and flags as mg_smooth, with
+// tiles of smooth3_tiles * ceil(nz / smooth3_zc) workgroups per solve.  The
+// result equals three mg_smooth passes bit for bit.  Needs 4 * nodes < 2^31.
+__host__ __device__ int smooth3_zc(int nz);
+int smooth3_tiles(int nx, int ny);
+void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
+                int max_blocks, const MgStatus *st, hipStream_t s);
+// e2 += prolongation of C's correction (post-smoothing input of smooth3)
+void mg_prolong_e2(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_nodes,
+                   const MgStatus *st, hipStream_t s);
 void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
                    hipStream_t s);
 void mg_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_coarse,

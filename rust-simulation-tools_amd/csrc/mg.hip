@@ -22,6 +22,8 @@
 #include "common.h"
 #include "mg.h"
 
+#include <type_traits>
+
 namespace mmpbsa {
 
 __device__ __forceinline__ bool stopped(const MgStatus *st) { return st->state != 0; }
@@ -506,6 +508,480 @@ void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first
         hipLaunchKernelGGL((smooth_kernel<false, true>), grid, dim3(256), 0, s, L, C, dir, st);
     else
         hipLaunchKernelGGL((smooth_kernel<false, false>), grid, dim3(256), 0, s, L, C, dir, st);
+    LAUNCH_OK();
+}
+
+// ---- temporally blocked smoother: three red-black sweeps per pass -------------
+// The schedule of smooth_kernel with K = 6 half-sweeps: half-sweep k of step t
+// updates plane t - k + 1 (k odd red, k even black), so a pass reads the
+// correction, the right-hand side and the code bytes once and writes the
+// correction once for three sweeps.  With a halo of 6 the tile core is 20 x 20
+// inside the 32 x 32 region; half-sweep k only has to be exact on the region
+// shrunk by k nodes, so it skips the outer k - 1 rings.  In every step the
+// active column of a pair (x, x + 1) is the one with x + y + t even, the same
+// for all six half-sweeps.
+//
+// Coefficients: the right-hand side of both columns of a pair and their code
+// bytes stay in registers for the six planes a step touches.  Faces whose
+// seven code bits (six faces, kappa) say everything come from a 128-entry LDS
+// table {a0..a5, 1/diag} built with the update's own expressions.  Nodes with a
+// fractional face are compacted per wave at the start of each step; one lane
+// per such node loads its six faces from the FP32 face arrays before the
+// step's prefetches are issued, so waiting for them never waits for the
+// prefetches, and writes {a0..a5, 1/diag} to a per-wave LDS slot.  A wave with
+// more such nodes than slots loads the rest inside the half-sweep.
+//
+// Memory access is through buffer resources: the plane offset is a scalar
+// offset, per-lane offsets are fixed for the whole pass, and lanes without a
+// node use an offset past every array, for which buffer loads return zero and
+// buffer stores do nothing.
+namespace s3 {
+constexpr int K = 6;                   // half-sweeps per pass
+constexpr int H = K;                   // halo
+constexpr int RW = 32, RH = 32, RN = RW * RH;
+constexpr int TILE = RW - 2 * H;       // core edge: 20
+constexpr int RING = 8;                // correction planes t-6 .. t+1
+constexpr int PF = RN / 256;           // plane values per thread: 4
+constexpr int PP = RN / 2 / 256;       // column pairs per thread: 2
+constexpr int WB = (TILE * TILE + 255) / 256;  // core values written back per thread: 2
+constexpr int SLOTS = 32;              // fractional-face slots per wave and step
+constexpr int TAB = 128;               // code-table entries
+constexpr uint32_t kOob = 0x80000000u; // offset outside every array
+constexpr int kRsrcWord3 = 0x00020000; // raw buffer, 32-bit data format
+static_assert(RING >= K + 2 && (RING & (RING - 1)) == 0, "ring holds planes t-K .. t+1");
+static_assert(PP * 256 * 2 == RN && PF * 256 == RN && RW == 32, "region layout");
+}  // namespace s3
+
+__host__ __device__ int smooth3_zc(int nz)
+{
+    const int nch = (nz + 32) / 64 > 1 ? (nz + 32) / 64 : 1;
+    return (nz + nch - 1) / nch;
+}
+int smooth3_tiles(int nx, int ny) { return ceil_div(nx, s3::TILE) * ceil_div(ny, s3::TILE); }
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             s3::kRsrcWord3);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ uint32_t bload8(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so)
+{
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, (int)vo, (int)so, 0);
+}
+
+template <bool FIRST, bool PROLONG>
+__global__ __launch_bounds__(256, 4) void smooth3_kernel(const MgLevelDev *__restrict__ Ls,
+                                                      const MgLevelDev *__restrict__ Cs, int dir,
+                                                      const MgStatus *__restrict__ st)
+{
+    using namespace s3;
+    __shared__ float ring[RING * RN];
+    __shared__ float4 tab[2 * TAB];
+    __shared__ float4 spc[4 * SLOTS * 2];
+    const uint32_t flat = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int s = (int)(flat / gridDim.x);
+    const int b = (int)(flat - (uint32_t)s * gridDim.x);
+    if (stopped(st + s)) return;
+    const MgLevelDev L = Ls[s];
+    if (L.nx == 0) return;
+    const int nx = L.nx, ny = L.ny, nz = L.nz, zc = smooth3_zc(L.nz);
+    const int ntx = (nx + TILE - 1) / TILE, nty = (ny + TILE - 1) / TILE;
+    const int nch = (nz + zc - 1) / zc;
+    if (b >= ntx * nty * nch) return;
+    const int tile = b % (ntx * nty), ch = b / (ntx * nty);
+    const int cx = (nx + ntx - 1) / ntx, cy = (ny + nty - 1) / nty;
+    const int x0 = (tile % ntx) * cx, x1 = min(nx, x0 + cx);
+    const int y0 = (tile / ntx) * cy, y1 = min(ny, y0 + cy);
+    const int z0 = ch * zc, z1 = min(nz, z0 + zc);
+    const int xl = x0 - H, yl = y0 - H;
+    const uint32_t sxy = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t N = sxy * (uint32_t)nz;  // mg_smooth3 checks 4 N < 2^31
+    const bool uni = L.uni != 0;
+    const auto src = buf_rsrc(dir ? L.e : L.e2, 4u * N);
+    const auto dst = buf_rsrc((FIRST || dir) ? L.e2 : L.e, 4u * N);
+    const auto frs = buf_rsrc(L.f, 4u * N);
+    const auto crs = buf_rsrc(uni ? nullptr : L.cw, uni ? 0u : N);
+    const auto exr = buf_rsrc(uni ? nullptr : L.ex, uni ? 0u : 4u * N);
+    const auto eyr = buf_rsrc(uni ? nullptr : L.ey, uni ? 0u : 4u * N);
+    const auto ezr = buf_rsrc(uni ? nullptr : L.ez, uni ? 0u : 4u * N);
+    MgLevelDev C{};
+    if (PROLONG) C = Cs[s];
+    const auto ceg = to_global<const float>(PROLONG ? C.e : L.f);
+    const float ih = L.ihx;  // the grids are cubic: ihx == ihy == ihz
+    const float omega = L.omega, ck2 = L.c_k2;
+    const uint32_t cin = __float_as_uint(L.c_in), cout = __float_as_uint(L.c_out);
+    const float uih = L.eps_u * ih, udg_inv = 1.0f / (6.0f * L.eps_u * ih);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int t0 = z0 - K + 1, t1 = z1 + K - 1;
+
+    // code table: entry w (face bits 0-5, kappa bit 6) -> {a0..a5, 1/diag}
+    if (!uni && tid < TAB) {
+        const uint32_t w = (uint32_t)tid;
+        const float a0 = face_sel(w, 0, cin, cout), a1 = face_sel(w, 1, cin, cout);
+        const float a2 = face_sel(w, 2, cin, cout), a3 = face_sel(w, 3, cin, cout);
+        const float a4 = face_sel(w, 4, cin, cout), a5 = face_sel(w, 5, cin, cout);
+        const float k2 = (w & kCodeKappa) ? ck2 : 0.0f;
+        const float dg = fmaf(((a0 + a1) + (a2 + a3)) + (a4 + a5), ih, k2);
+        tab[2 * tid] = make_float4(a0, a1, a2, a3);
+        tab[2 * tid + 1] = make_float4(a4, a5, __builtin_amdgcn_rcpf(dg), 0.0f);
+    }
+
+    // plane loads: value n = tid + 256 j at region row (tid >> 5) + 8 j, column tid & 31
+    const int lx = xl + (tid & 31), ly = yl + (tid >> 5);
+    uint32_t pvo[PF];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const int y = ly + 8 * j;
+        pvo[j] = (lx >= 0 && lx < nx && y >= 0 && y < ny) ? 4u * (uint32_t)(lx + nx * y) : kOob;
+    }
+    // column pairs m: region row ry = (tid >> 4) + 16 m, columns rx0, rx0 + 1
+    const int ry0 = tid >> 4, rx0 = 2 * (tid & 15);
+    const int li0 = ry0 * RW + rx0;  // LDS index of column 0 of pair 0 (pair 1: + 16 RW)
+    const int par = (xl + rx0 + yl + ry0) & 1;  // active column at step t: par ^ (t & 1)
+    uint32_t cvo[PP];   // node index of column 0 (kOob when the pair leaves the grid)
+    uint32_t rdpk = 0;  // byte 2m + c: ring distance of column c if it is updated, else 0
+    // spread[c]: bit 2 (k - 1) + m set when half-sweep k updates column c of pair m
+#pragma unroll
+    for (int m = 0; m < PP; ++m) {
+        const int ry = ry0 + 16 * m, y = yl + ry, x = xl + rx0;
+        const bool pin = y >= 0 && y < ny && x >= 0 && x + 1 < nx;
+        cvo[m] = pin ? (uint32_t)(x + nx * y) : kOob;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int rx = rx0 + c, xa = x + c;
+            const int rd = min(min(rx, RW - 1 - rx), min(ry, RH - 1 - ry));
+            const bool in = xa >= 1 && xa <= nx - 2 && y >= 1 && y <= ny - 2;
+            rdpk |= (uint32_t)(in ? rd : 0) << (8 * (2 * m + c));
+        }
+    }
+    uint32_t spread[2] = {0u, 0u};
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 1; k <= K; ++k)
+#pragma unroll
+            for (int m = 0; m < PP; ++m)
+                if (((rdpk >> (8 * (2 * m + c))) & 0xffu) >= (uint32_t)k)
+                    spread[c] |= 1u << (2 * (k - 1) + m);
+    // write-back: core value n = tid + 256 j at core row n / TILE
+    uint32_t wvo[WB];
+    int wli[WB];
+#pragma unroll
+    for (int j = 0; j < WB; ++j) {
+        const int n = tid + 256 * j, r = n / TILE, c = n - r * TILE;
+        const bool ok = r < y1 - y0 && c < x1 - x0;
+        wvo[j] = ok ? 4u * (uint32_t)((x0 + c) + nx * (y0 + r)) : kOob;
+        wli[j] = ok ? (r + H) * RW + (c + H) : 0;
+    }
+
+    auto slot = [&](int z) { return ring + ((z + 2 * RING) & (RING - 1)) * RN; };
+    float pf[PF];
+    auto load_plane = [&](int z) {
+        if (FIRST || z < 0 || z >= nz) {
+#pragma unroll
+            for (int j = 0; j < PF; ++j) pf[j] = 0.0f;
+            return;
+        }
+        const uint32_t zo = 4u * sxy * (uint32_t)z;
+#pragma unroll
+        for (int j = 0; j < PF; ++j) {
+            float x = bload(src, pvo[j], zo);
+            if (PROLONG) {
+                const bool v = pvo[j] != kOob;
+                x += prolong_value(C, ceg, v ? lx : 0, v ? ly + 8 * j : 0, z);
+                x = v ? x : 0.0f;
+            }
+            pf[j] = x;
+        }
+    };
+    auto store_plane = [&](int z) {
+        float *p = slot(z);
+#pragma unroll
+        for (int j = 0; j < PF; ++j) p[tid + 256 * j] = pf[j];
+    };
+    auto write_back = [&](int z) {
+        if (z < z0 || z >= z1) return;
+        const float *p = slot(z);
+        const uint32_t zo = 4u * sxy * (uint32_t)z;
+#pragma unroll
+        for (int j = 0; j < WB; ++j)
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(p[wli[j]]), dst, (int)wvo[j],
+                                                  (int)zo, 0);
+    };
+    // coefficients of a plane: f of both columns of both pairs, code bytes packed
+    struct Coef {
+        float f[PP][2];
+        uint32_t w;
+    };
+    struct Next {
+        float f[PP][2];
+        uint32_t b[PP][2];
+    };
+    auto load_coef = [&](int z, Next &Q, bool UNI) {
+        // planes outside 1 .. nz - 2 are never updated: every offset out of range
+        const bool zin = z >= 1 && z <= nz - 2;
+        const uint32_t zo = zin ? sxy * (uint32_t)z : 0u;
+#pragma unroll
+        for (int m = 0; m < PP; ++m) {
+            const uint32_t o = zin ? cvo[m] : kOob, fo = o == kOob ? kOob : 4u * o;
+            Q.f[m][0] = bload(frs, fo, 4u * zo);
+            Q.f[m][1] = bload(frs, fo + 4u, 4u * zo);
+            Q.b[m][0] = UNI ? 0u : bload8(crs, o, zo);
+            Q.b[m][1] = UNI ? 0u : bload8(crs, o + 1u, zo);
+        }
+    };
+    auto pack = [](const Next &Q) {
+        Coef c;
+#pragma unroll
+        for (int m = 0; m < PP; ++m) {
+            c.f[m][0] = Q.f[m][0];
+            c.f[m][1] = Q.f[m][1];
+        }
+        c.w = Q.b[0][0] | (Q.b[0][1] << 8) | (Q.b[1][0] << 16) | (Q.b[1][1] << 24);
+        return c;
+    };
+    auto zok = [&](int k, int z) {
+        return z >= 1 && z <= nz - 2 && z >= z0 - (K - k) && z <= z1 - 1 + (K - k);
+    };
+
+    using Fast = std::integral_constant<bool, false>;
+    using Slow = std::integral_constant<bool, true>;
+    // overflow flags of the two step parities live in the unused fourth
+    // word of table entries 0 and 1
+    int *const wg_ovf0 = reinterpret_cast<int *>(&tab[1]) + 3;
+    int *const wg_ovf1 = reinterpret_cast<int *>(&tab[3]) + 3;
+    auto wg_ovf = [&](int par_t) { return par_t ? wg_ovf1 : wg_ovf0; };
+    uint32_t *req = reinterpret_cast<uint32_t *>(spc) + wv * SLOTS * 8;
+
+    // the step loop, compiled separately for uniform and non-uniform levels
+    auto run = [&](auto uni_tag) {
+        constexpr bool UNI = decltype(uni_tag)::value;
+        Coef Q[K];  // Q[i]: plane t - i
+        Next nxt;
+
+        // Fractional-face nodes of step t.  Update j = 2 (k - 1) + m (half-
+        // sweep k, pair m) of a lane is one when bit j of spm is set; the wave
+        // numbers them lane by lane (exclusive prefix sum of the lane counts,
+        // spx) and in bit order within a lane, so update j uses slot spx +
+        // popcount of the lower bits.  Slots past SLOTS are read in the
+        // half-sweep itself.
+        uint32_t spm = 0;
+        int spx = 0, total = 0;
+        float sa[6];
+        uint32_t sreq = 0xffffffffu;
+        auto specials_issue = [&](int t) {
+            const int c = par ^ (t & 1);
+            uint32_t zmask = 0;  // half-sweeps with a plane to update (uniform)
+#pragma unroll
+            for (int k = 1; k <= K; ++k)
+                zmask |= zok(k, t - k + 1) ? 3u << (2 * (k - 1)) : 0u;
+            uint32_t hb = 0, kb = 0;  // bit j: code bit 7 (fractional) / bit 6 (kappa)
+#pragma unroll
+            for (int k = 1; k <= K; ++k) {
+                const uint32_t w = Q[k - 1].w >> (8 * c);
+                hb |= ((w >> 7) & 1u) << (2 * (k - 1));
+                hb |= ((w >> 23) & 1u) << (2 * (k - 1) + 1);
+                kb |= ((w >> 6) & 1u) << (2 * (k - 1));
+                kb |= ((w >> 22) & 1u) << (2 * (k - 1) + 1);
+            }
+            spm = hb & zmask & (c ? spread[1] : spread[0]);
+            const int cnt = __popc(spm);
+            int inc = cnt;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int y = __shfl_up(inc, d, 64);
+                inc += lane >= d ? y : 0;
+            }
+            spx = inc - cnt;
+            total = __builtin_amdgcn_readfirstlane(__shfl(inc, 63, 64));
+            if (total == 0) return;
+            // requests: node index and kappa bit of each numbered update
+            const uint32_t a0 = cvo[0] + (uint32_t)c + sxy * (uint32_t)(t + 1);
+            const uint32_t a1 = cvo[1] + (uint32_t)c + sxy * (uint32_t)(t + 1);
+            uint32_t rem = spm;
+            int sl = spx;
+            while (__ballot(rem != 0u)) {
+                if (rem) {
+                    const int j = __ffs(rem) - 1;
+                    rem &= rem - 1u;
+                    const uint32_t g = ((j & 1) ? a1 : a0) - sxy * (uint32_t)((j >> 1) + 1);
+                    if (sl < SLOTS) req[sl] = (g << 1) | ((kb >> j) & 1u);
+                    ++sl;
+                }
+            }
+            sreq = lane < min(total, SLOTS) ? req[lane] : 0xffffffffu;
+            const uint32_t g = sreq >> 1;
+            const bool v = sreq != 0xffffffffu;
+            sa[0] = bload(exr, v ? 4u * g : kOob, 0u);
+            sa[1] = bload(exr, v ? 4u * (g - 1u) : kOob, 0u);
+            sa[2] = bload(eyr, v ? 4u * g : kOob, 0u);
+            sa[3] = bload(eyr, v ? 4u * (g - (uint32_t)nx) : kOob, 0u);
+            sa[4] = bload(ezr, v ? 4u * g : kOob, 0u);
+            sa[5] = bload(ezr, v ? 4u * (g - sxy) : kOob, 0u);
+        };
+        auto specials_store = [&]() {
+            if (total == 0 || sreq == 0xffffffffu) return;
+            // face order of the update: x+, x-, y+, y-, z+, z-
+            const float a0 = sa[0], a1 = sa[1], a2 = sa[2], a3 = sa[3], a4 = sa[4], a5 = sa[5];
+            const float k2 = (sreq & 1u) ? ck2 : 0.0f;
+            const float dg = fmaf(((a0 + a1) + (a2 + a3)) + (a4 + a5), ih, k2);
+            spc[2 * (wv * SLOTS + lane)] = make_float4(a0, a1, a2, a3);
+            spc[2 * (wv * SLOTS + lane) + 1] =
+                make_float4(a4, a5, __builtin_amdgcn_rcpf(dg), 0.0f);
+        };
+
+        // half-sweep k of step t on plane t - k + 1; SLOW: some wave of the
+        // workgroup has more fractional-face nodes this step than slots
+        auto stage = [&](int k, int t, auto slow_tag) {
+            constexpr bool SLOW = decltype(slow_tag)::value;
+            const int z = t - k + 1;
+            if (!zok(k, z)) return;
+            const int c = par ^ (t & 1);
+            const float *__restrict__ pm = slot(z - 1);
+            float *__restrict__ pc = slot(z);
+            const float *__restrict__ pp = slot(z + 1);
+            const Coef &q = Q[k - 1];
+#pragma unroll
+            for (int m = 0; m < PP; ++m) {
+                const int jb = 2 * m + c;
+                if (((rdpk >> (8 * jb)) & 0xffu) < (uint32_t)k) continue;
+                const int i = li0 + 16 * RW * m + c;
+                const float fv = c ? q.f[m][1] : q.f[m][0];
+                const float exx = pc[i + 1], emx = pc[i - 1], eyy = pc[i + RW], emy = pc[i - RW];
+                const float ezz = pp[i], emz = pm[i], old = pc[i];
+                float gs;
+                if (UNI) {
+                    const float sum = ((exx + emx) + (eyy + emy)) + (ezz + emz);
+                    gs = fmaf(sum, uih, fv) * udg_inv;
+                } else {
+                    const uint32_t w = (q.w >> (8 * jb)) & 0xffu;
+                    const int j = 2 * (k - 1) + m;
+                    const int sl = spx + __popc(spm & ((1u << j) - 1u));
+                    const bool sp = (w & kCodeSpecial) != 0;
+                    const float4 *e = sp ? &spc[2 * (wv * SLOTS + min(sl, SLOTS - 1))]
+                                         : &tab[2 * (w & 0x7fu)];
+                    float4 A = e[0], B = e[1];
+                    if (SLOW && sp && sl >= SLOTS) {
+                        const uint32_t g = cvo[m] + (uint32_t)c + sxy * (uint32_t)z;
+                        A.x = bload(exr, 4u * g, 0u);
+                        A.y = bload(exr, 4u * (g - 1u), 0u);
+                        A.z = bload(eyr, 4u * g, 0u);
+                        A.w = bload(eyr, 4u * (g - (uint32_t)nx), 0u);
+                        B.x = bload(ezr, 4u * g, 0u);
+                        B.y = bload(ezr, 4u * (g - sxy), 0u);
+                        const float k2 = (w & kCodeKappa) ? ck2 : 0.0f;
+                        const float dg =
+                            fmaf(((A.x + A.y) + (A.z + A.w)) + (B.x + B.y), ih, k2);
+                        B.z = __builtin_amdgcn_rcpf(dg);
+                    }
+                    const float sum = fmaf(A.x, exx, fmaf(A.y, emx, fmaf(A.z, eyy,
+                                      fmaf(A.w, emy, fmaf(B.x, ezz, B.y * emz)))));
+                    gs = fmaf(sum, ih, fv) * B.z;
+                }
+                pc[i] = fmaf(omega, gs - old, old);
+            }
+        };
+
+        // prologue: planes t0 - 1 and t0 in the ring, coefficients of t0 and
+        // plane t0 + 1 in flight
+        load_plane(t0 - 1);
+        store_plane(t0 - 1);
+        load_plane(t0);
+        store_plane(t0);
+        load_coef(t0, nxt, UNI);
+        load_plane(t0 + 1);
+        if (tid < 2) *wg_ovf(tid) = 0;
+        for (int t = t0; t <= t1; ++t) {
+            store_plane(t + 1);
+#pragma unroll
+            for (int i = K - 1; i > 0; --i) Q[i] = Q[i - 1];
+            Q[0] = pack(nxt);
+            // order of issue: fractional faces, write-back, prefetches, so
+            // that waiting for the faces waits for nothing younger
+            if (!UNI) specials_issue(t);
+            write_back(t - K);
+            if (t < t1) {
+                load_coef(t + 1, nxt, UNI);
+                load_plane(t + 2);
+            }
+            if (UNI) {
+                __syncthreads();
+#pragma unroll
+                for (int k = 1; k <= K; ++k) {
+                    stage(k, t, Fast{});
+                    __syncthreads();
+                }
+                continue;
+            }
+            specials_store();
+            // overflow flags alternate between two words: the one for step
+            // t + 1 is cleared here, after every wave has read the previous
+            // step's
+            if (total > SLOTS && lane == 0) *wg_ovf(t & 1) = 1;
+            if (tid == 0) *wg_ovf((t + 1) & 1) = 0;
+            __syncthreads();
+            if (__builtin_amdgcn_readfirstlane(*wg_ovf(t & 1))) {
+#pragma unroll
+                for (int k = 1; k <= K; ++k) {
+                    stage(k, t, Slow{});
+                    __syncthreads();
+                }
+            } else {
+#pragma unroll
+                for (int k = 1; k <= K; ++k) {
+                    stage(k, t, Fast{});
+                    __syncthreads();
+                }
+            }
+        }
+    };
+    if (uni)
+        run(std::integral_constant<bool, true>{});
+    else
+        run(std::integral_constant<bool, false>{});
+}
+
+// e2 += prolongation of the coarse correction (the start of post-smoothing)
+__global__ __launch_bounds__(256) void prolong_e2_kernel(const MgLevelDev *__restrict__ Fs,
+                                                         const MgLevelDev *__restrict__ Cs,
+                                                         const MgStatus *__restrict__ st)
+{
+    const BatchBlock bb = batch_block();
+    const int s = bb.s;
+    if (stopped(st + s)) return;
+    MgLevelDev F = Fs[s];
+    if (F.nx == 0) return;
+    const int64_t idx = (int64_t)bb.b * blockDim.x + threadIdx.x;
+    if (idx >= nodes(F)) return;
+    const MgLevelDev C = Cs[s];
+    F.e = F.e2;
+    prolong_node(C, F, idx);
+}
+
+void mg_prolong_e2(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_nodes,
+                   const MgStatus *st, hipStream_t s)
+{
+    if (n_solves <= 0 || max_nodes <= 0) return;
+    const dim3 grid(ceil_div(max_nodes, 256), n_solves);
+    hipLaunchKernelGGL(prolong_e2_kernel, grid, dim3(256), 0, s, F, C, st);
+    LAUNCH_OK();
+}
+
+void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
+                int max_blocks, const MgStatus *st, hipStream_t s)
+{
+    if (n_solves <= 0 || max_blocks <= 0) return;
+    const dim3 grid(max_blocks, n_solves);
+    // the prolongation runs as its own kernel (mg_prolong_e2) before this pass
+    if (C) throw DeviceError(1, "mg_smooth3: no fused prolongation");  // invalid argument
+    if (first)
+        hipLaunchKernelGGL((smooth3_kernel<true, false>), grid, dim3(256), 0, s, L, C, dir, st);
+    else
+        hipLaunchKernelGGL((smooth3_kernel<false, false>), grid, dim3(256), 0, s, L, C, dir, st);
     LAUNCH_OK();
 }
 

@@ -426,6 +426,10 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     } H[2];
     std::vector<int64_t> maxnodes(maxT, 0), maxcoarse(maxT, 0);
     std::vector<int> sblocks(maxT, 0);  // smoother launch width per level
+    // one pass per smoothing phase (three sweeps, smooth3_kernel) unless
+    // MMPBSA_SMOOTHER=sweep asks for one pass per sweep (smooth_kernel)
+    const char *sm_env = std::getenv("MMPBSA_SMOOTHER");
+    const bool phase_pass = !(sm_env && std::strcmp(sm_env, "sweep") == 0);
     int64_t maxN0 = 0;
     for (int p = 0; p < 2; ++p) {
         H[p].F.assign(nF, null_level());
@@ -445,8 +449,13 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 H[p].F[(size_t)l * S + s] = lev(l);
                 H[p].C[(size_t)l * S + s] = lev(l + 1);
                 const GridDesc &g = tw.L[l].g;
-                sblocks[l] = std::max(sblocks[l], smooth_tiles(g.nx, g.ny) *
-                                                      ceil_div(g.nz, smooth_zc(g.nz)));
+                sblocks[l] = std::max(sblocks[l],
+                                      phase_pass ? smooth3_tiles(g.nx, g.ny) *
+                                                       ceil_div(g.nz, smooth3_zc(g.nz))
+                                                 : smooth_tiles(g.nx, g.ny) *
+                                                       ceil_div(g.nz, smooth_zc(g.nz)));
+                if (phase_pass && 4 * lv_nodes(g) >= ((int64_t)1 << 31))
+                    throw DeviceError(MMPBSA_ERR_TOO_LARGE, "multigrid level above 2^29 nodes");
                 maxnodes[l] = std::max(maxnodes[l], lv_nodes(tw.L[l].g));
                 maxcoarse[l] = std::max(maxcoarse[l], lv_nodes(tw.L[l + 1].g));
             }
@@ -522,6 +531,23 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     constexpr int npass = 3;
     // The first post-smoothing pass also applies prolongate_add from C.
     auto smooth_phase = [&](const MgLevelDev *F, const MgLevelDev *C, int l, bool pre) {
+        if (phase_pass) {
+            // pre: from zero into e2; post: e2 plus the prolongated coarse
+            // correction (added by its own kernel) into e
+            Ev ev{nullptr, nullptr, cycle_counter, !pre, false};
+            if (profiling && l == 0) {
+                HIP_OK(hipEventCreate(&ev.a));
+                HIP_OK(hipEventCreate(&ev.b));
+                HIP_OK(hipEventRecord(ev.a, w.stream));
+            }
+            if (!pre) mg_prolong_e2(F, C, S, maxnodes[l], dst, w.stream);
+            mg_smooth3(F, nullptr, S, pre ? 1 : 0, pre ? 1 : 0, sblocks[l], dst, w.stream);
+            if (profiling && l == 0) {
+                HIP_OK(hipEventRecord(ev.b, w.stream));
+                evs.push_back(ev);
+            }
+            return;
+        }
         for (int k = 0; k < npass; ++k) {
             const bool first = pre && k == 0;
             const bool prolong = !pre && k == 0;
