@@ -193,7 +193,7 @@ def measure_traffic(frames_per_step, grid_threads):
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             out = os.path.join(tmp, counter)
-            cmd = [prof, "--pmc", counter, "--kernel-include-regex", "smooth_kernel|copy_kernel",
+            cmd = [prof, "--pmc", counter, "--kernel-include-regex", "smooth3?_kernel|copy_kernel",
                    "--output-format", "csv", "-d", out, "-o", "run", "--"] + child
             r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
                                timeout=300, cwd=tmp, env=dict(os.environ, TMPDIR="/tmp"))
@@ -318,8 +318,8 @@ def main():
                         "frac": achieved / HBM_PEAK_GBS,
                         "traffic": traffic["bytes_per_launch"] if traffic else None,
                         "traffic_detail": traffic,
-                        "kernel": "smooth_kernel (finest level: one red-black sweep per "
-                                  "launch, batched over solves)",
+                        "kernel": "smooth3_kernel (finest level: one smoothing phase, three "
+                                  "red-black sweeps, per launch, batched over solves)",
                         "launches": n.value, "avg_launch_us": ms.value * 1e3 / n.value,
                         "alg_bytes_per_launch": by.value / n.value}
         # small system: batched vs plain where launches dominate

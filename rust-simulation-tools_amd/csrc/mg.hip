@@ -552,12 +552,19 @@ static_assert(RING >= K + 2 && (RING & (RING - 1)) == 0, "ring holds planes t-K 
 static_assert(PP * 256 * 2 == RN && PF * 256 == RN && RW == 32, "region layout");
 }  // namespace s3
 
-__host__ __device__ int smooth3_zc(int nz)
-{
-    const int nch = (nz + 32) / 64 > 1 ? (nz + 32) / 64 : 1;
-    return (nz + nch - 1) / nch;
-}
 int smooth3_tiles(int nx, int ny) { return ceil_div(nx, s3::TILE) * ceil_div(ny, s3::TILE); }
+
+// Planes per workgroup: about 64 (a chunk also streams 2 K = 12 planes it does
+// not write), shorter when a launch of n_solves such levels would otherwise
+// have fewer than 2048 workgroups (8 per CU), but at least 8.
+int smooth3_zc(int nx, int ny, int nz, int n_solves)
+{
+    const int64_t tiles = (int64_t)smooth3_tiles(nx, ny) * std::max(n_solves, 1);
+    int nch = std::max(1, (nz + 32) / 64);
+    nch = std::max<int>(nch, (int)((2048 + tiles - 1) / tiles));
+    nch = std::min(nch, std::max(1, nz / 8));
+    return ceil_div(nz, nch);
+}
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, uint32_t bytes)
 {
@@ -588,7 +595,7 @@ __global__ __launch_bounds__(256, 4) void smooth3_kernel(const MgLevelDev *__res
     if (stopped(st + s)) return;
     const MgLevelDev L = Ls[s];
     if (L.nx == 0) return;
-    const int nx = L.nx, ny = L.ny, nz = L.nz, zc = smooth3_zc(L.nz);
+    const int nx = L.nx, ny = L.ny, nz = L.nz, zc = L.zc;
     const int ntx = (nx + TILE - 1) / TILE, nty = (ny + TILE - 1) / TILE;
     const int nch = (nz + zc - 1) / zc;
     if (b >= ntx * nty * nch) return;

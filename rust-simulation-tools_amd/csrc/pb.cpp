@@ -310,6 +310,11 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     const bool profiling = prof.enabled && profile_level0;
     if (profiling) use_graph = false;
 
+    // one pass per smoothing phase (three sweeps, smooth3_kernel) unless
+    // MMPBSA_SMOOTHER=sweep asks for one pass per sweep (smooth_kernel)
+    const char *sm_env = std::getenv("MMPBSA_SMOOTHER");
+    const bool phase_pass = !(sm_env && std::strcmp(sm_env, "sweep") == 0);
+
     std::vector<TaskWork> W(S);
     int maxT = 0;
     for (int s = 0; s < S; ++s) {
@@ -357,7 +362,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const float ih = (float)(1.0 / (lv.g.h * lv.g.h));
             dv.ihx = dv.ihy = dv.ihz = ih;
             dv.omega = (float)estimate_omega(lv.g.nx, lv.g.ny, lv.g.nz);
-            dv.zc = smooth_zc(lv.g.nz);
+            dv.zc = phase_pass ? smooth3_zc(lv.g.nx, lv.g.ny, lv.g.nz, S) : smooth_zc(lv.g.nz);
             dv.e = lv.e.get();
             dv.e2 = lv.e2.get();
             dv.f = lv.f.get();
@@ -426,10 +431,6 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     } H[2];
     std::vector<int64_t> maxnodes(maxT, 0), maxcoarse(maxT, 0);
     std::vector<int> sblocks(maxT, 0);  // smoother launch width per level
-    // one pass per smoothing phase (three sweeps, smooth3_kernel) unless
-    // MMPBSA_SMOOTHER=sweep asks for one pass per sweep (smooth_kernel)
-    const char *sm_env = std::getenv("MMPBSA_SMOOTHER");
-    const bool phase_pass = !(sm_env && std::strcmp(sm_env, "sweep") == 0);
     int64_t maxN0 = 0;
     for (int p = 0; p < 2; ++p) {
         H[p].F.assign(nF, null_level());
@@ -451,7 +452,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 const GridDesc &g = tw.L[l].g;
                 sblocks[l] = std::max(sblocks[l],
                                       phase_pass ? smooth3_tiles(g.nx, g.ny) *
-                                                       ceil_div(g.nz, smooth3_zc(g.nz))
+                                                       ceil_div(g.nz, tw.L[l].d.zc)
                                                  : smooth_tiles(g.nx, g.ny) *
                                                        ceil_div(g.nz, smooth_zc(g.nz)));
                 if (phase_pass && 4 * lv_nodes(g) >= ((int64_t)1 << 31))
