@@ -73,7 +73,7 @@ struct MgOuterDev {
     const float *e;                     // correction to add (nullptr: none)
     float *f0;                          // FP32 residual for the V-cycle
     double *partials;                   // per-block sums of r^2
-    int nbk;                            // number of partials
+    int nbk;                            // number of partials (blocks per solve)
     FastDiv fd_nx, fd_nxy;
 };
 
@@ -101,6 +101,14 @@ int smooth3_tiles(int nx, int ny);
 int smooth3_zc(int nx, int ny, int nz, int n_solves);
 void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
                 int max_blocks, const MgStatus *st, hipStream_t s);
+// z-streaming versions of residual32 / prolong_e2 (same results bit for bit):
+// one thread per in-plane node over L.zc planes, mg_stream_blocks workgroups
+// per solve.
+int mg_stream_blocks(int nx, int ny, int nz, int zc);
+void mg_residual32s(const MgLevelDev *L, int n_solves, int max_blocks, const MgStatus *st,
+                    hipStream_t s);
+void mg_prolong_e2s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
+                    const MgStatus *st, hipStream_t s);
 // e2 += prolongation of C's correction (post-smoothing input of smooth3)
 void mg_prolong_e2(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_nodes,
                    const MgStatus *st, hipStream_t s);

@@ -192,6 +192,7 @@ __device__ __forceinline__ void restrict_node(const MgLevelDev &F, const MgLevel
                 if (x < 0 || x >= F.nx) continue;
                 const float w = (dx == 0 ? 2.0f : 1.0f) * (dy == 0 ? 2.0f : 1.0f) *
                                 (dz == 0 ? 2.0f : 1.0f);
+                // generic access: the tail kernel keeps small levels in LDS
                 sum = fmaf(w, F.res[x + (int64_t)F.nx * (y + (int64_t)F.ny * z)], sum);
                 ws += w;
             }
@@ -992,6 +993,203 @@ void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int firs
     LAUNCH_OK();
 }
 
+// ---- z-streaming per-node kernels ---------------------------------------------
+// One thread per in-plane index p = x + nx y (lanes contiguous in memory) over a
+// range of L.zc planes.  Its z neighbours come from a register window; the
+// in-plane neighbours, the right-hand side and the code byte of plane z + 1 are
+// loaded one plane ahead, and loads at fractional-face nodes are issued before
+// each plane's prefetches, so waiting for them never waits for the prefetches.
+// Each kernel evaluates the same expressions as its per-node counterpart
+// (residual_node, coeffs, prolong_value), so results are equal bit for bit.
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float v, uint32_t vo, uint32_t so)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)vo, (int)so, 0);
+}
+
+// workgroups of one solve: ceil(nx ny / 256) in-plane blocks times ceil(nz / zc) chunks
+__host__ __device__ inline int stream_blocks(int nx, int ny, int nz, int zc)
+{
+    return ((nx * ny + 255) / 256) * ((nz + zc - 1) / zc);
+}
+
+struct StreamPos {
+    int p, x, y, z0, z1;
+};
+// false: this thread has no column (past the plane or past the chunks)
+__device__ __forceinline__ bool stream_pos(int nx, int ny, int nz, int zc, int b, StreamPos &q)
+{
+    const int sxy = nx * ny, pb = (sxy + 255) / 256;
+    const int ch = b / pb;
+    q.p = (b - ch * pb) * 256 + (int)threadIdx.x;
+    q.z0 = ch * zc;
+    q.z1 = min(nz, q.z0 + zc);
+    if (q.p >= sxy || q.z0 >= nz) return false;
+    q.y = q.p / nx;
+    q.x = q.p - q.y * nx;
+    return true;
+}
+
+// residual32_kernel over e2 (pre-smoothed correction) -> res
+__global__ __launch_bounds__(256) void residual32s_kernel(const MgLevelDev *__restrict__ Ls,
+                                                          const MgStatus *__restrict__ st)
+{
+    using s3::kOob;
+    const BatchBlock bb = batch_block();
+    if (stopped(st + bb.s)) return;
+    const MgLevelDev L = Ls[bb.s];
+    if (L.nx == 0) return;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    StreamPos q;
+    if (bb.b >= stream_blocks(nx, ny, nz, L.zc) || !stream_pos(nx, ny, nz, L.zc, bb.b, q)) return;
+    const uint32_t sxy = (uint32_t)nx * (uint32_t)ny, N = sxy * (uint32_t)nz;
+    const bool uni = L.uni != 0, hasw = !uni && L.cw;
+    const auto er = buf_rsrc(L.e2, 4u * N), fr = buf_rsrc(L.f, 4u * N);
+    const auto rr = buf_rsrc(L.res, 4u * N);
+    const auto cr = buf_rsrc(hasw ? L.cw : nullptr, hasw ? N : 0u);
+    const auto exr = buf_rsrc(uni ? nullptr : L.ex, uni ? 0u : 4u * N);
+    const auto eyr = buf_rsrc(uni ? nullptr : L.ey, uni ? 0u : 4u * N);
+    const auto ezr = buf_rsrc(uni ? nullptr : L.ez, uni ? 0u : 4u * N);
+    const auto k2r = buf_rsrc(uni || hasw ? nullptr : L.k2, uni || hasw ? 0u : 4u * N);
+    const bool inpl = q.x >= 1 && q.x <= nx - 2 && q.y >= 1 && q.y <= ny - 2;
+    const uint32_t po = 4u * (uint32_t)q.p;
+    // in-plane neighbour offsets (out of range for boundary columns: loads give 0)
+    const uint32_t oxp = inpl ? po + 4u : kOob, oxm = inpl ? po - 4u : kOob;
+    const uint32_t oyp = inpl ? po + 4u * (uint32_t)nx : kOob;
+    const uint32_t oym = inpl ? po - 4u * (uint32_t)nx : kOob;
+    const float u = L.eps_u, ihx = L.ihx, ihy = L.ihy, ihz = L.ihz;
+    auto e_at = [&](int z) { return (z >= 0 && z < nz) ? bload(er, po, 4u * sxy * (uint32_t)z) : 0.0f; };
+    struct Pl {
+        float xp, xm, yp, ym, f;
+        uint32_t w;
+    };
+    auto load_pl = [&](int z, Pl &P) {
+        const bool zi = z >= 1 && z <= nz - 2;
+        const uint32_t zo = zi ? 4u * sxy * (uint32_t)z : 0u;
+        P.xp = bload(er, zi ? oxp : kOob, zo);
+        P.xm = bload(er, zi ? oxm : kOob, zo);
+        P.yp = bload(er, zi ? oyp : kOob, zo);
+        P.ym = bload(er, zi ? oym : kOob, zo);
+        P.f = bload(fr, zi ? po : kOob, zo);
+        P.w = hasw ? bload8(cr, zi ? (uint32_t)q.p : kOob, zo >> 2) : 0u;
+    };
+    float em = e_at(q.z0 - 1), ec = e_at(q.z0), ep = e_at(q.z0 + 1);
+    Pl cur, nxt;
+    load_pl(q.z0, cur);
+    for (int z = q.z0; z < q.z1; ++z) {
+        const uint32_t zo = 4u * sxy * (uint32_t)z;
+        const bool act = inpl && z >= 1 && z <= nz - 2;
+        // faces of a fractional-face node (or of every node without code bytes)
+        float exp_ = 0.0f, exm = 0.0f, eyp = 0.0f, eym = 0.0f, ezp = 0.0f, ezm = 0.0f, k2 = 0.0f;
+        const bool arr = !uni && act && (!hasw || (cur.w & kCodeSpecial));
+        {
+            const uint32_t g = 4u * (uint32_t)q.p;
+            exp_ = bload(exr, arr ? g : kOob, zo);
+            exm = bload(exr, arr ? g - 4u : kOob, zo);
+            eyp = bload(eyr, arr ? g : kOob, zo);
+            eym = bload(eyr, arr ? g - 4u * (uint32_t)nx : kOob, zo);
+            ezp = bload(ezr, arr ? g : kOob, zo);
+            ezm = bload(ezr, arr ? g : kOob, arr ? zo - 4u * sxy : 0u);  // plane z - 1
+            if (!hasw) k2 = bload(k2r, arr ? g : kOob, zo);
+        }
+        const float en = e_at(z + 2);
+        if (z + 1 < q.z1) load_pl(z + 1, nxt);
+        float r = 0.0f;
+        if (act) {
+            float nb, dg;
+            if (uni) {
+                nb = u * ((cur.xp + cur.xm) * ihx + (cur.yp + cur.ym) * ihy + (ep + em) * ihz);
+                dg = u * (2.0f * ihx + 2.0f * ihy + 2.0f * ihz);
+            } else {
+                if (hasw) {
+                    const uint32_t w = cur.w;
+                    k2 = (w & kCodeKappa) ? L.c_k2 : 0.0f;
+                    if (!(w & kCodeSpecial)) {
+                        exp_ = face_bit<float>(w, 0, L.c_in, L.c_out);
+                        exm = face_bit<float>(w, 1, L.c_in, L.c_out);
+                        eyp = face_bit<float>(w, 2, L.c_in, L.c_out);
+                        eym = face_bit<float>(w, 3, L.c_in, L.c_out);
+                        ezp = face_bit<float>(w, 4, L.c_in, L.c_out);
+                        ezm = face_bit<float>(w, 5, L.c_in, L.c_out);
+                    }
+                }
+                nb = fmaf(exp_, cur.xp, exm * cur.xm) * ihx + fmaf(eyp, cur.yp, eym * cur.ym) * ihy +
+                     fmaf(ezp, ep, ezm * em) * ihz;
+                dg = (exp_ + exm) * ihx + (eyp + eym) * ihy + (ezp + ezm) * ihz + k2;
+            }
+            r = cur.f + nb - dg * ec;
+        }
+        bstore(rr, r, po, zo);
+        em = ec;
+        ec = ep;
+        ep = en;
+        cur = nxt;
+    }
+}
+
+// prolong_e2_kernel: e2 += prolongation of the coarse correction
+__global__ __launch_bounds__(256) void prolong_e2s_kernel(const MgLevelDev *__restrict__ Fs,
+                                                          const MgLevelDev *__restrict__ Cs,
+                                                          const MgStatus *__restrict__ st)
+{
+    const BatchBlock bb = batch_block();
+    if (stopped(st + bb.s)) return;
+    const MgLevelDev F = Fs[bb.s];
+    if (F.nx == 0) return;
+    const MgLevelDev C = Cs[bb.s];
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    StreamPos q;
+    if (bb.b >= stream_blocks(nx, ny, nz, F.zc) || !stream_pos(nx, ny, nz, F.zc, bb.b, q)) return;
+    const uint32_t sxy = (uint32_t)nx * (uint32_t)ny, N = sxy * (uint32_t)nz;
+    const uint32_t cs = (uint32_t)C.nx, csz = (uint32_t)C.nx * (uint32_t)C.ny;
+    const auto er = buf_rsrc(F.e2, 4u * N);
+    const auto cr = buf_rsrc(C.e, 4u * csz * (uint32_t)C.nz);
+    // prolong_value's cell and weights in x and y are fixed for the column
+    const int x0 = min(q.x >> 1, max(C.nx - 2, 0)), y0 = min(q.y >> 1, max(C.ny - 2, 0));
+    const float dx = 0.5f * q.x - x0, dy = 0.5f * q.y - y0;
+    const uint32_t co = 4u * ((uint32_t)x0 + cs * (uint32_t)y0);
+    // coarse plane cz interpolated in x and y (c0 / c1 of prolong_value)
+    auto cplane = [&](int cz) {
+        const uint32_t zo = 4u * csz * (uint32_t)cz;
+        const float c00 = bload(cr, co, zo) * (1.0f - dx) + bload(cr, co + 4u, zo) * dx;
+        const float c10 = bload(cr, co + 4u * cs, zo) * (1.0f - dx) + bload(cr, co + 4u * cs + 4u, zo) * dx;
+        return c00 * (1.0f - dy) + c10 * dy;
+    };
+    const uint32_t po = 4u * (uint32_t)q.p;
+    int zc_lo = -1;
+    float c_lo = 0.0f, c_hi = 0.0f;
+    for (int z = q.z0; z < q.z1; ++z) {
+        const int z0c = min(z >> 1, max(C.nz - 2, 0));
+        if (z0c != zc_lo) {
+            if (z0c == zc_lo + 1) {
+                c_lo = c_hi;
+            } else {
+                c_lo = cplane(z0c);
+            }
+            c_hi = cplane(z0c + 1);
+            zc_lo = z0c;
+        }
+        const float dz = 0.5f * z - z0c;
+        const uint32_t zo = 4u * sxy * (uint32_t)z;
+        bstore(er, bload(er, po, zo) + (c_lo * (1.0f - dz) + c_hi * dz), po, zo);
+    }
+}
+
+void mg_residual32s(const MgLevelDev *L, int n_solves, int max_blocks, const MgStatus *st,
+                    hipStream_t s)
+{
+    if (n_solves <= 0 || max_blocks <= 0) return;
+    hipLaunchKernelGGL(residual32s_kernel, dim3(max_blocks, n_solves), dim3(256), 0, s, L, st);
+    LAUNCH_OK();
+}
+void mg_prolong_e2s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
+                    const MgStatus *st, hipStream_t s)
+{
+    if (n_solves <= 0 || max_blocks <= 0) return;
+    hipLaunchKernelGGL(prolong_e2s_kernel, dim3(max_blocks, n_solves), dim3(256), 0, s, F, C, st);
+    LAUNCH_OK();
+}
+int mg_stream_blocks(int nx, int ny, int nz, int zc) { return stream_blocks(nx, ny, nz, zc); }
+
 __global__ __launch_bounds__(256) void residual32_kernel(const MgLevelDev *__restrict__ Ls,
                                                          const MgStatus *__restrict__ st)
 {
@@ -1116,14 +1314,17 @@ __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restri
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     double r2acc = 0.0;
     if (idx < N) {
-        const float *__restrict__ e = O.e;
-        const double *__restrict__ p = O.phi_in;
+        // descriptor pointers are device memory: global (not flat) accesses
+        const auto e = to_global<const float>(O.e);
+        const auto p = to_global<const double>(O.phi_in);
+        const auto po = to_global<double>(O.phi_out);
+        const auto f0 = to_global<float>(O.f0);
         const double pc = e ? p[idx] + (double)e[idx] : p[idx];
-        if (e) O.phi_out[idx] = pc;
+        if (e) po[idx] = pc;
         int ix, iy, iz;
         split3(O, idx, ix, iy, iz);
         if (ix == 0 || iy == 0 || iz == 0 || ix == nx - 1 || iy == ny - 1 || iz == nz - 1) {
-            O.f0[idx] = 0.0f;
+            f0[idx] = 0.0f;
         } else {
             double pxp, pxm, pyp, pym, pzp, pzm;
             if (e) {
@@ -1146,13 +1347,41 @@ __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restri
                 exp_ = exm = eyp = eym = ezp = ezm = O.eps_u;
                 k2 = 0.0;
             } else {
-                coeffs<double>(O, idx, exp_, exm, eyp, eym, ezp, ezm, k2);
+                // coeffs<double> with global accesses
+                const auto cw = to_global<const uint8_t>(O.cw);
+                bool arrays = true;
+                if (O.cw) {
+                    const uint32_t w = cw[idx];
+                    k2 = (w & kCodeKappa) ? O.c_k2 : 0.0;
+                    if (!(w & kCodeSpecial)) {
+                        exp_ = face_bit<double>(w, 0, O.c_in, O.c_out);
+                        exm = face_bit<double>(w, 1, O.c_in, O.c_out);
+                        eyp = face_bit<double>(w, 2, O.c_in, O.c_out);
+                        eym = face_bit<double>(w, 3, O.c_in, O.c_out);
+                        ezp = face_bit<double>(w, 4, O.c_in, O.c_out);
+                        ezm = face_bit<double>(w, 5, O.c_in, O.c_out);
+                        arrays = false;
+                    }
+                } else {
+                    k2 = to_global<const double>(O.k2)[idx];
+                }
+                if (arrays) {
+                    const auto ex = to_global<const double>(O.ex);
+                    const auto ey = to_global<const double>(O.ey);
+                    const auto ez = to_global<const double>(O.ez);
+                    exp_ = ex[idx];
+                    exm = ex[idx - 1];
+                    eyp = ey[idx];
+                    eym = ey[idx - sy];
+                    ezp = ez[idx];
+                    ezm = ez[idx - sz];
+                }
             }
             const double nb = (exp_ * pxp + exm * pxm) * O.ihx + (eyp * pyp + eym * pym) * O.ihy +
                               (ezp * pzp + ezm * pzm) * O.ihz;
             const double dg = (exp_ + exm) * O.ihx + (eyp + eym) * O.ihy + (ezp + ezm) * O.ihz + k2;
-            const double r = O.scale * O.rho[idx] + nb - dg * pc;
-            O.f0[idx] = (float)r;
+            const double r = O.scale * to_global<const double>(O.rho)[idx] + nb - dg * pc;
+            f0[idx] = (float)r;
             r2acc = r * r;
         }
     }

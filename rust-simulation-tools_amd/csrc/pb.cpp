@@ -314,6 +314,9 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     // MMPBSA_SMOOTHER=sweep asks for one pass per sweep (smooth_kernel)
     const char *sm_env = std::getenv("MMPBSA_SMOOTHER");
     const bool phase_pass = !(sm_env && std::strcmp(sm_env, "sweep") == 0);
+    // z-streaming residual and prolongation unless MMPBSA_STREAM=0
+    const char *st_env = std::getenv("MMPBSA_STREAM");
+    const bool stream = !(st_env && std::strcmp(st_env, "0") == 0);
 
     std::vector<TaskWork> W(S);
     int maxT = 0;
@@ -431,6 +434,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     } H[2];
     std::vector<int64_t> maxnodes(maxT, 0), maxcoarse(maxT, 0);
     std::vector<int> sblocks(maxT, 0);  // smoother launch width per level
+    std::vector<int> strblk(maxT, 0);   // streaming-kernel launch width per level
     int64_t maxN0 = 0;
     for (int p = 0; p < 2; ++p) {
         H[p].F.assign(nF, null_level());
@@ -458,6 +462,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 if (phase_pass && 4 * lv_nodes(g) >= ((int64_t)1 << 31))
                     throw DeviceError(MMPBSA_ERR_TOO_LARGE, "multigrid level above 2^29 nodes");
                 maxnodes[l] = std::max(maxnodes[l], lv_nodes(tw.L[l].g));
+                strblk[l] = std::max(strblk[l], mg_stream_blocks(g.nx, g.ny, g.nz, tw.L[l].d.zc));
                 maxcoarse[l] = std::max(maxcoarse[l], lv_nodes(tw.L[l + 1].g));
             }
             MgTailDev &T = H[p].T[s];
@@ -541,7 +546,12 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 HIP_OK(hipEventCreate(&ev.b));
                 HIP_OK(hipEventRecord(ev.a, w.stream));
             }
-            if (!pre) mg_prolong_e2(F, C, S, maxnodes[l], dst, w.stream);
+            if (!pre) {
+                if (stream)
+                    mg_prolong_e2s(F, C, S, strblk[l], dst, w.stream);
+                else
+                    mg_prolong_e2(F, C, S, maxnodes[l], dst, w.stream);
+            }
             mg_smooth3(F, nullptr, S, pre ? 1 : 0, pre ? 1 : 0, sblocks[l], dst, w.stream);
             if (profiling && l == 0) {
                 HIP_OK(hipEventRecord(ev.b, w.stream));
@@ -572,7 +582,10 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const MgLevelDev *F = D[p].F + (size_t)l * S;
             const MgLevelDev *C = D[p].C + (size_t)l * S;
             smooth_phase(F, C, l, true);
-            mg_residual32(F, S, maxnodes[l], dst, w.stream);
+            if (stream)
+                mg_residual32s(F, S, strblk[l], dst, w.stream);
+            else
+                mg_residual32(F, S, maxnodes[l], dst, w.stream);
             mg_restrict(F, C, S, maxcoarse[l], dst, w.stream);
         }
         mg_tail(D[p].T, S, dst, w.stream);
