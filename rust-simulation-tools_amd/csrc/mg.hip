@@ -588,8 +588,11 @@ __global__ __launch_bounds__(256, 4) void smooth3_kernel(const MgLevelDev *__res
 {
     using namespace s3;
     __shared__ float ring[RING * RN];
-    __shared__ float4 tab[2 * TAB];
-    __shared__ float4 spc[4 * SLOTS * 2];
+    // face entries {a0..a3}, {a4, a5, 1/diag, -}: rows 0..127 the code table,
+    // then SLOTS per wave for the step's fractional-face nodes
+    __shared__ float4 fac[2 * (TAB + 4 * SLOTS)];
+    float4 *const tab = fac;
+    float4 *const spc = fac + 2 * TAB;
     const uint32_t flat = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
     const int s = (int)(flat / gridDim.x);
     const int b = (int)(flat - (uint32_t)s * gridDim.x);
@@ -844,23 +847,39 @@ __global__ __launch_bounds__(256, 4) void smooth3_kernel(const MgLevelDev *__res
 
         // half-sweep k of step t on plane t - k + 1; SLOW: some wave of the
         // workgroup has more fractional-face nodes this step than slots
+        // per step: the active column's ring distance and the byte offset,
+        // within a plane, of the node one row above it (so every in-plane
+        // neighbour is a non-negative offset from one address)
+        uint32_t rdm[PP], ibm[PP];
+        auto step_cols = [&](int t) {
+            const int c = par ^ (t & 1);
+#pragma unroll
+            for (int m = 0; m < PP; ++m) {
+                rdm[m] = (rdpk >> (8 * (2 * m + c))) & 0xffu;
+                ibm[m] = 4u * (uint32_t)(li0 + 16 * RW * m + c - RW);
+            }
+        };
         auto stage = [&](int k, int t, auto slow_tag) {
             constexpr bool SLOW = decltype(slow_tag)::value;
             const int z = t - k + 1;
             if (!zok(k, z)) return;
             const int c = par ^ (t & 1);
-            const float *__restrict__ pm = slot(z - 1);
-            float *__restrict__ pc = slot(z);
-            const float *__restrict__ pp = slot(z + 1);
+            const char *ringb = reinterpret_cast<const char *>(ring);
+            const uint32_t sc = 4u * (uint32_t)(slot(z) - ring);
+            const uint32_t sm = 4u * (uint32_t)(slot(z - 1) - ring) + 4u * RW;
+            const uint32_t sp = 4u * (uint32_t)(slot(z + 1) - ring) + 4u * RW;
             const Coef &q = Q[k - 1];
 #pragma unroll
             for (int m = 0; m < PP; ++m) {
                 const int jb = 2 * m + c;
-                if (((rdpk >> (8 * jb)) & 0xffu) < (uint32_t)k) continue;
-                const int i = li0 + 16 * RW * m + c;
+                if (rdm[m] < (uint32_t)k) continue;
+                const float *r = reinterpret_cast<const float *>(ringb + sc + ibm[m]);
                 const float fv = c ? q.f[m][1] : q.f[m][0];
-                const float exx = pc[i + 1], emx = pc[i - 1], eyy = pc[i + RW], emy = pc[i - RW];
-                const float ezz = pp[i], emz = pm[i], old = pc[i];
+                const float emy = r[0], emx = r[RW - 1], old = r[RW], exx = r[RW + 1];
+                const float eyy = r[2 * RW];
+                const float ezz = *reinterpret_cast<const float *>(ringb + sp + ibm[m]);
+                const float emz = *reinterpret_cast<const float *>(ringb + sm + ibm[m]);
+                float *pc_i = const_cast<float *>(r) + RW;
                 float gs;
                 if (UNI) {
                     const float sum = ((exx + emx) + (eyy + emy)) + (ezz + emz);
@@ -869,9 +888,13 @@ __global__ __launch_bounds__(256, 4) void smooth3_kernel(const MgLevelDev *__res
                     const uint32_t w = (q.w >> (8 * jb)) & 0xffu;
                     const int j = 2 * (k - 1) + m;
                     const int sl = spx + __popc(spm & ((1u << j) - 1u));
-                    const bool sp = (w & kCodeSpecial) != 0;
-                    const float4 *e = sp ? &spc[2 * (wv * SLOTS + min(sl, SLOTS - 1))]
-                                         : &tab[2 * (w & 0x7fu)];
+                    const bool sp = w >= kCodeSpecial;
+                    // face entry: the wave's slot or the table row (both
+                    // computed, then selected: no branch); a workgroup on the
+                    // fast path has every slot below SLOTS
+                    const uint32_t se = (uint32_t)(TAB + wv * SLOTS) +
+                                        (uint32_t)(SLOW ? min(sl, SLOTS - 1) : sl);
+                    const float4 *e = fac + 2u * (sp ? se : (w & 0x7fu));
                     float4 A = e[0], B = e[1];
                     if (SLOW && sp && sl >= SLOTS) {
                         const uint32_t g = cvo[m] + (uint32_t)c + sxy * (uint32_t)z;
@@ -890,7 +913,7 @@ __global__ __launch_bounds__(256, 4) void smooth3_kernel(const MgLevelDev *__res
                                       fmaf(A.w, emy, fmaf(B.x, ezz, B.y * emz)))));
                     gs = fmaf(sum, ih, fv) * B.z;
                 }
-                pc[i] = fmaf(omega, gs - old, old);
+                *pc_i = fmaf(omega, gs - old, old);
             }
         };
 
@@ -908,6 +931,7 @@ __global__ __launch_bounds__(256, 4) void smooth3_kernel(const MgLevelDev *__res
 #pragma unroll
             for (int i = K - 1; i > 0; --i) Q[i] = Q[i - 1];
             Q[0] = pack(nxt);
+            step_cols(t);
             // order of issue: fractional faces, write-back, prefetches, so
             // that waiting for the faces waits for nothing younger
             if (!UNI) specials_issue(t);
