@@ -109,6 +109,10 @@ void mg_residual32s(const MgLevelDev *L, int n_solves, int max_blocks, const MgS
                     hipStream_t s);
 void mg_prolong_e2s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
                     const MgStatus *st, hipStream_t s);
+// restrict into C.f, one thread per coarse column over C.zc coarse planes
+// (mg_stream_blocks of the coarse level)
+void mg_restrict_s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
+                   const MgStatus *st, hipStream_t s);
 // e2 += prolongation of C's correction (post-smoothing input of smooth3)
 void mg_prolong_e2(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_nodes,
                    const MgStatus *st, hipStream_t s);

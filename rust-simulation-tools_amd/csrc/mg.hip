@@ -1198,6 +1198,78 @@ __global__ __launch_bounds__(256) void prolong_e2s_kernel(const MgLevelDev *__re
     }
 }
 
+// restrict_kernel: one thread per coarse column over C.zc coarse planes.  The
+// nine fine values of plane 2 cz + 1 serve as the next coarse plane's plane
+// 2 (cz + 1) - 1; the 27 terms are accumulated in restrict_node's order with
+// its skips, so the result is equal bit for bit.
+__global__ __launch_bounds__(256) void restrict_s_kernel(const MgLevelDev *__restrict__ Fs,
+                                                         const MgLevelDev *__restrict__ Cs,
+                                                         const MgStatus *__restrict__ st)
+{
+    using s3::kOob;
+    const BatchBlock bb = batch_block();
+    if (stopped(st + bb.s)) return;
+    const MgLevelDev C = Cs[bb.s];
+    if (C.nx == 0) return;
+    const MgLevelDev F = Fs[bb.s];
+    StreamPos q;
+    if (bb.b >= stream_blocks(C.nx, C.ny, C.nz, C.zc) || !stream_pos(C.nx, C.ny, C.nz, C.zc, bb.b, q))
+        return;
+    const uint32_t fsxy = (uint32_t)F.nx * (uint32_t)F.ny, csxy = (uint32_t)C.nx * (uint32_t)C.ny;
+    const auto rr = buf_rsrc(F.res, 4u * fsxy * (uint32_t)F.nz);
+    const auto fr = buf_rsrc(C.f, 4u * csxy * (uint32_t)C.nz);
+    // fine in-plane offsets of the 3 x 3 neighbourhood (kOob outside the plane)
+    uint32_t off[9];
+    bool vin[9];
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int x = 2 * q.x + dx, y = 2 * q.y + dy, k = 3 * (dy + 1) + (dx + 1);
+            vin[k] = x >= 0 && x < F.nx && y >= 0 && y < F.ny;
+            off[k] = vin[k] ? 4u * (uint32_t)(x + F.nx * y) : kOob;
+        }
+    auto load9 = [&](int z, float *v) {
+        const bool zi = z >= 0 && z < F.nz;
+        const uint32_t so = zi ? 4u * fsxy * (uint32_t)z : 0u;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] = bload(rr, zi ? off[k] : kOob, so);
+    };
+    float lo[9], mid[9], hi[9];
+    load9(2 * q.z0 - 1, lo);
+    for (int cz = q.z0; cz < q.z1; ++cz) {
+        load9(2 * cz, mid);
+        load9(2 * cz + 1, hi);
+        float sum = 0.0f, ws = 0.0f;
+        const float *pl[3] = {lo, mid, hi};
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int z = 2 * cz + dz;
+            if (z < 0 || z >= F.nz) continue;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                if (!vin[k]) continue;
+                const int dy = k / 3 - 1, dx = k % 3 - 1;
+                const float w = (dx == 0 ? 2.0f : 1.0f) * (dy == 0 ? 2.0f : 1.0f) *
+                                (dz == 0 ? 2.0f : 1.0f);
+                sum = fmaf(w, pl[dz + 1][k], sum);
+                ws += w;
+            }
+        }
+        bstore(fr, 0.25f * sum / ws, 4u * (uint32_t)q.p, 4u * csxy * (uint32_t)cz);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) lo[k] = hi[k];
+    }
+}
+
+void mg_restrict_s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
+                   const MgStatus *st, hipStream_t s)
+{
+    if (n_solves <= 0 || max_blocks <= 0) return;
+    hipLaunchKernelGGL(restrict_s_kernel, dim3(max_blocks, n_solves), dim3(256), 0, s, F, C, st);
+    LAUNCH_OK();
+}
+
 void mg_residual32s(const MgLevelDev *L, int n_solves, int max_blocks, const MgStatus *st,
                     hipStream_t s)
 {

@@ -435,6 +435,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     std::vector<int64_t> maxnodes(maxT, 0), maxcoarse(maxT, 0);
     std::vector<int> sblocks(maxT, 0);  // smoother launch width per level
     std::vector<int> strblk(maxT, 0);   // streaming-kernel launch width per level
+    std::vector<int> rstblk(maxT, 0);   // streaming restriction width (coarse level)
     int64_t maxN0 = 0;
     for (int p = 0; p < 2; ++p) {
         H[p].F.assign(nF, null_level());
@@ -463,6 +464,9 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                     throw DeviceError(MMPBSA_ERR_TOO_LARGE, "multigrid level above 2^29 nodes");
                 maxnodes[l] = std::max(maxnodes[l], lv_nodes(tw.L[l].g));
                 strblk[l] = std::max(strblk[l], mg_stream_blocks(g.nx, g.ny, g.nz, tw.L[l].d.zc));
+                const GridDesc &gc = tw.L[l + 1].g;
+                rstblk[l] = std::max(rstblk[l],
+                                     mg_stream_blocks(gc.nx, gc.ny, gc.nz, tw.L[l + 1].d.zc));
                 maxcoarse[l] = std::max(maxcoarse[l], lv_nodes(tw.L[l + 1].g));
             }
             MgTailDev &T = H[p].T[s];
@@ -586,7 +590,10 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 mg_residual32s(F, S, strblk[l], dst, w.stream);
             else
                 mg_residual32(F, S, maxnodes[l], dst, w.stream);
-            mg_restrict(F, C, S, maxcoarse[l], dst, w.stream);
+            if (stream)
+                mg_restrict_s(F, C, S, rstblk[l], dst, w.stream);
+            else
+                mg_restrict(F, C, S, maxcoarse[l], dst, w.stream);
         }
         mg_tail(D[p].T, S, dst, w.stream);
         for (int l = maxT - 1; l >= 0; --l) {
