@@ -221,57 +221,6 @@ __global__ void ext_init_kernel(GridDesc g, const uint8_t *__restrict__ sas, uin
     ext[idx] = (bnd && !sas[idx]) ? 1 : 0;
 }
 
-// One thread per line along `axis`; forward then backward propagation.
-__global__ void flood_line_kernel(GridDesc g, int axis, const uint8_t *__restrict__ sas,
-                                  uint8_t *__restrict__ ext, int *__restrict__ changed)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t base, stride, len;
-    if (axis == 0) {
-        if (t >= (int64_t)g.ny * g.nz) return;
-        base = t * g.nx;
-        stride = 1;
-        len = g.nx;
-    } else if (axis == 1) {
-        if (t >= (int64_t)g.nx * g.nz) return;
-        const int64_t x = t % g.nx, z = t / g.nx;
-        base = x + (int64_t)g.nx * g.ny * z;
-        stride = g.nx;
-        len = g.ny;
-    } else {
-        if (t >= (int64_t)g.nx * g.ny) return;
-        base = t;
-        stride = (int64_t)g.nx * g.ny;
-        len = g.nz;
-    }
-    int ch = 0;
-    uint8_t prev = 0;
-    for (int64_t k = 0; k < len; ++k) {
-        const int64_t i = base + k * stride;
-        const uint8_t s = sas[i];
-        uint8_t e = ext[i];
-        if (!s && !e && prev) {
-            e = 1;
-            ext[i] = 1;
-            ch = 1;
-        }
-        prev = s ? 0 : e;
-    }
-    prev = 0;
-    for (int64_t k = len - 1; k >= 0; --k) {
-        const int64_t i = base + k * stride;
-        const uint8_t s = sas[i];
-        uint8_t e = ext[i];
-        if (!s && !e && prev) {
-            e = 1;
-            ext[i] = 1;
-            ch = 1;
-        }
-        prev = s ? 0 : e;
-    }
-    if (ch) *changed = 1;
-}
-
 // Bounded separable squared EDT (values capped at T+1) and the SES test
 // inside <=> min squared index distance to the exterior > T, T = floor((p/h)^2).
 __global__ void dil_x_kernel(GridDesc g, int R, int T, const uint8_t *__restrict__ ext,
@@ -310,35 +259,112 @@ __global__ void dil_yz_kernel(GridDesc g, int R, int T, int axis, const uint8_t 
     out[idx] = final_test ? (uint8_t)(best > T ? 1 : 0) : (uint8_t)best;
 }
 
-void launch_ses(const GridDesc &g, const uint8_t *sas, uint8_t *ext, uint8_t *work_a,
-                uint8_t *work_b, int thresh, uint8_t *inside_out, int *flag_dev, int *flag_host,
-                hipStream_t s)
+// One thread per line along `axis` of map blockIdx.y of the table: forward
+// then backward propagation through outside-SAS nodes.
+__global__ void flood_batch_kernel(const FloodJob *__restrict__ jobs, int axis)
 {
-    const int64_t N = g.size();
-    hipLaunchKernelGGL(ext_init_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, sas, ext);
-    LAUNCH_OK();
-    const int64_t lines[3] = {(int64_t)g.ny * g.nz, (int64_t)g.nx * g.nz, (int64_t)g.nx * g.ny};
-    for (int round = 0; round < 100000; ++round) {
-        launch_fill_i32(flag_dev, 0, 1, s);
-        for (int ax = 0; ax < 3; ++ax) {
-            hipLaunchKernelGGL(flood_line_kernel, dim3(ceil_div(lines[ax], 128)), dim3(128), 0, s,
-                               g, ax, sas, ext, flag_dev);
-            LAUNCH_OK();
-        }
-        HIP_OK(hipMemcpyAsync(flag_host, flag_dev, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        if (!*flag_host) break;
+    const FloodJob J = jobs[blockIdx.y];
+    if (*J.active == 0) return;  // this map reached its fixpoint in an earlier round
+    const GridDesc g = J.g;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t base, stride, len;
+    if (axis == 0) {
+        if (t >= (int64_t)g.ny * g.nz) return;
+        base = t * g.nx;
+        stride = 1;
+        len = g.nx;
+    } else if (axis == 1) {
+        if (t >= (int64_t)g.nx * g.nz) return;
+        const int64_t x = t % g.nx, z = t / g.nx;
+        base = x + (int64_t)g.nx * g.ny * z;
+        stride = g.nx;
+        len = g.ny;
+    } else {
+        if (t >= (int64_t)g.nx * g.ny) return;
+        base = t;
+        stride = (int64_t)g.nx * g.ny;
+        len = g.nz;
     }
-    const int R = (int)std::floor(std::sqrt((double)thresh));
-    hipLaunchKernelGGL(dil_x_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, R, thresh, ext,
-                       work_a);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(dil_yz_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, R, thresh, 1,
-                       work_a, work_b, 0);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(dil_yz_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, R, thresh, 2,
-                       work_b, inside_out, 1);
-    LAUNCH_OK();
+    const uint8_t *__restrict__ sas = J.sas;
+    uint8_t *__restrict__ ext = J.ext;
+    int ch = 0;
+    uint8_t prev = 0;
+    for (int64_t k = 0; k < len; ++k) {
+        const int64_t i = base + k * stride;
+        const uint8_t sv = sas[i];
+        uint8_t e = ext[i];
+        if (!sv && !e && prev) {
+            e = 1;
+            ext[i] = 1;
+            ch = 1;
+        }
+        prev = sv ? 0 : e;
+    }
+    prev = 0;
+    for (int64_t k = len - 1; k >= 0; --k) {
+        const int64_t i = base + k * stride;
+        const uint8_t sv = sas[i];
+        uint8_t e = ext[i];
+        if (!sv && !e && prev) {
+            e = 1;
+            ext[i] = 1;
+            ch = 1;
+        }
+        prev = sv ? 0 : e;
+    }
+    if (ch) *J.changed = 1;
+}
+
+void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d_flags,
+                      hipStream_t s)
+{
+    const int n = (int)jobs.size();
+    if (n == 0) return;
+    std::vector<FloodJob> ht(n);
+    for (int k = 0; k < n; ++k) {
+        const SesJob &J = jobs[k];
+        const int64_t N = J.g.size();
+        hipLaunchKernelGGL(ext_init_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, J.g, J.sas,
+                           J.ext);
+        LAUNCH_OK();
+        ht[k] = FloodJob{J.g, J.sas, J.ext, d_flags + k, d_flags + n + k};
+    }
+    HIP_OK(hipMemcpyAsync(d_table, ht.data(), sizeof(FloodJob) * n, hipMemcpyHostToDevice, s));
+    launch_fill_i32(d_flags + n, 1, n, s);
+    // rounds of each map until a round changes nothing (one host check per
+    // round).  Maps are flooded one after another: sharing launches between
+    // maps measured slower, as each map's x sweeps then compete for the
+    // caches and every map waited for the slowest one's rounds.
+    int hf = 0;
+    for (int k = 0; k < n; ++k) {
+        const GridDesc &g = jobs[k].g;
+        const int64_t ln[3] = {(int64_t)g.ny * g.nz, (int64_t)g.nx * g.nz, (int64_t)g.nx * g.ny};
+        for (int round = 0; round < 100000; ++round) {
+            launch_fill_i32(d_flags + k, 0, 1, s);
+            for (int ax = 0; ax < 3; ++ax) {
+                hipLaunchKernelGGL(flood_batch_kernel, dim3(ceil_div(ln[ax], 128), 1), dim3(128),
+                                   0, s, d_table + k, ax);
+                LAUNCH_OK();
+            }
+            HIP_OK(hipMemcpyAsync(&hf, d_flags + k, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            if (!hf) break;
+        }
+    }
+
+    for (const SesJob &J : jobs) {
+        const int64_t N = J.g.size();
+        const int R = (int)std::floor(std::sqrt((double)J.thresh));
+        hipLaunchKernelGGL(dil_x_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, J.g, R,
+                           J.thresh, J.ext, J.work_a);
+        LAUNCH_OK();
+        hipLaunchKernelGGL(dil_yz_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, J.g, R,
+                           J.thresh, 1, J.work_a, J.work_b, 0);
+        LAUNCH_OK();
+        hipLaunchKernelGGL(dil_yz_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, J.g, R,
+                           J.thresh, 2, J.work_b, J.inside_out, 1);
+        LAUNCH_OK();
+    }
 }
 
 // ---- face dielectrics, assign_dielectrics (pb_grid.rs:827-940) -------------

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "common.h"
 
@@ -94,10 +95,25 @@ void launch_spread(const GridDesc &g, int n, const double *xyz, const double *q,
 // inside[i] = exists atom a with |p_i - x_a|^2 < (r_a + probe)^2
 void launch_inside(const GridDesc &g, const double *xyz, const double *radii, double probe,
                    double max_r, const CellList &cl, uint8_t *inside, hipStream_t s);
-// SES classification from the SAS map (flood fill + bounded distance transform).
-void launch_ses(const GridDesc &g, const uint8_t *sas, uint8_t *ext, uint8_t *work_a,
-                uint8_t *work_b, int thresh, uint8_t *inside_out, int *flag_dev,
-                int *flag_host, hipStream_t s);
+// SES classification from SAS maps (flood fill of the exterior + bounded
+// distance transform) for many maps: each map's flood rounds (one host check
+// per round), then each map's distance test.
+struct SesJob {
+    GridDesc g;
+    const uint8_t *sas;
+    uint8_t *ext, *work_a, *work_b, *inside_out;
+    int thresh;
+};
+struct FloodJob {
+    GridDesc g;
+    const uint8_t *sas;
+    uint8_t *ext;
+    int *changed;
+    const int *active;  // 0 once a round changed nothing in this map
+};
+// d_table: device room for jobs.size() FloodJobs; d_flags: 2 jobs.size() ints
+void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d_flags,
+                      hipStream_t s);
 // Face dielectrics (FP64 + FP32 copies) from the SES map.
 void launch_faces(const GridDesc &g, const uint8_t *inside, const double *xyz,
                   const double *radii, double max_r, const CellList &cl, double eps_in,

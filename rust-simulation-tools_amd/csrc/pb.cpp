@@ -59,74 +59,125 @@ static double estimate_omega(int nx, int ny, int nz)
     return 2.0 / (1.0 + std::sqrt(1.0 - r * r));
 }
 
-// assign_dielectrics + assign_kappa for one grid (pb_grid.rs:827-985)
-static void build_maps(Worker &w, const DevTopo &t, const double *d_xyz, const DevCells &c,
-                       const GridDesc &g, const mmpbsa_pb_params &p, double ein, double eout,
-                       bool ionic, double kappa, MapSet &m)
+// Dielectric and kappa maps (pb_grid.rs:296-313, :827-985) for many maps at
+// once (uniform media need none): the SAS tests, the flood fills (shared
+// launches, one host check per round), the distance tests and the faces of
+// every map are issued together, with one host check of the face overflow
+// flags, so small maps do not leave the GPU idle between host round trips.
+struct MapReq {
+    const DevTopo *t;
+    const double *d_xyz;
+    const DevCells *c;
+    GridDesc g;
+    const mmpbsa_pb_params *p;
+    double ein, eout;
+    bool ionic;
+    double kappa;
+    MapSet *m;
+};
+
+static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
 {
-    const int64_t N = g.size();
-    const bool uni_eps = std::fabs(ein - eout) < DBL_EPSILON;
-    if (uni_eps && !ionic) {
-        m.uni = true;
-        m.eps_u = ein;
-        return;
-    }
-    m.uni = false;
-    m.ex = DBuf<double>(w.pool, N);
-    m.ey = DBuf<double>(w.pool, N);
-    m.ez = DBuf<double>(w.pool, N);
-    m.k2 = DBuf<double>(w.pool, N);
-    m.fx = DBuf<float>(w.pool, N);
-    m.fy = DBuf<float>(w.pool, N);
-    m.fz = DBuf<float>(w.pool, N);
-    m.k2f = DBuf<float>(w.pool, N);
-    if (uni_eps) {
-        launch_fill_f64(m.ex.get(), ein, N, w.stream);
-        launch_fill_f64(m.ey.get(), ein, N, w.stream);
-        launch_fill_f64(m.ez.get(), ein, N, w.stream);
-        launch_fill_f32(m.fx.get(), (float)ein, N, w.stream);
-        launch_fill_f32(m.fy.get(), (float)ein, N, w.stream);
-        launch_fill_f32(m.fz.get(), (float)ein, N, w.stream);
-    } else {
-        DBuf<uint8_t> sas(w.pool, N), inside(w.pool, N);
-        const double probe = p.probe_radius;
+    struct Tmp {
+        DBuf<uint8_t> sas, inside, ext, wa, wb;
+    };
+    std::vector<Tmp> tmp(rq.size());
+    std::vector<SesJob> ses;
+    std::vector<size_t> faces;  // maps that need faces from an SES map
+    for (size_t k = 0; k < rq.size(); ++k) {
+        MapReq &r = rq[k];
+        MapSet &m = *r.m;
+        const GridDesc &g = r.g;
+        const int64_t N = g.size();
+        const bool uni_eps = std::fabs(r.ein - r.eout) < DBL_EPSILON;
+        if (uni_eps && !r.ionic) {
+            m.uni = true;
+            m.eps_u = r.ein;
+            continue;
+        }
+        m.uni = false;
+        m.ex = DBuf<double>(w.pool, N);
+        m.ey = DBuf<double>(w.pool, N);
+        m.ez = DBuf<double>(w.pool, N);
+        m.k2 = DBuf<double>(w.pool, N);
+        m.fx = DBuf<float>(w.pool, N);
+        m.fy = DBuf<float>(w.pool, N);
+        m.fz = DBuf<float>(w.pool, N);
+        m.k2f = DBuf<float>(w.pool, N);
+        if (uni_eps) {
+            launch_fill_f64(m.ex.get(), r.ein, N, w.stream);
+            launch_fill_f64(m.ey.get(), r.ein, N, w.stream);
+            launch_fill_f64(m.ez.get(), r.ein, N, w.stream);
+            launch_fill_f32(m.fx.get(), (float)r.ein, N, w.stream);
+            launch_fill_f32(m.fy.get(), (float)r.ein, N, w.stream);
+            launch_fill_f32(m.fz.get(), (float)r.ein, N, w.stream);
+            continue;
+        }
+        Tmp &tm = tmp[k];
+        tm.inside = DBuf<uint8_t>(w.pool, N);
+        const double probe = r.p->probe_radius;
         if (probe <= 0.0) {
-            launch_inside(g, d_xyz, t.radii.p, 0.0, t.max_radius, c.cl, inside.get(), w.stream);
+            launch_inside(g, r.d_xyz, r.t->radii.p, 0.0, r.t->max_radius, r.c->cl,
+                          tm.inside.get(), w.stream);
         } else {
             const double th2 = (probe / g.h) * (probe / g.h);
             const double T = std::floor(th2);
             if (T > 250.0)
                 throw DeviceError(MMPBSA_ERR_TOO_LARGE, "probe_radius / grid_spacing above 15.8");
-            launch_inside(g, d_xyz, t.radii.p, probe, t.max_radius, c.cl, sas.get(), w.stream);
-            DBuf<uint8_t> ext(w.pool, N), wa(w.pool, N), wb(w.pool, N);
-            DBuf<int> flag(w.pool, 1);
-            launch_ses(g, sas.get(), ext.get(), wa.get(), wb.get(), (int)T, inside.get(),
-                       flag.get(), w.pin_i + 8, w.stream);
+            tm.sas = DBuf<uint8_t>(w.pool, N);
+            tm.ext = DBuf<uint8_t>(w.pool, N);
+            tm.wa = DBuf<uint8_t>(w.pool, N);
+            tm.wb = DBuf<uint8_t>(w.pool, N);
+            launch_inside(g, r.d_xyz, r.t->radii.p, probe, r.t->max_radius, r.c->cl,
+                          tm.sas.get(), w.stream);
+            ses.push_back(SesJob{g, tm.sas.get(), tm.ext.get(), tm.wa.get(), tm.wb.get(),
+                                 tm.inside.get(), (int)T});
         }
-        DBuf<int> ovf(w.pool, 1);
-        launch_fill_i32(ovf.get(), 0, 1, w.stream);
-        launch_faces(g, inside.get(), d_xyz, t.radii.p, t.max_radius, c.cl, ein, eout, m.ex.get(),
-                     m.ey.get(), m.ez.get(), m.fx.get(), m.fy.get(), m.fz.get(), ovf.get(),
-                     w.stream);
-        HIP_OK(hipMemcpyAsync(w.pin_i + 9, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        faces.push_back(k);
+    }
+    if (!ses.empty()) {
+        DBuf<FloodJob> table(w.pool, ses.size());
+        DBuf<int> flags(w.pool, 2 * ses.size());
+        launch_ses_batch(ses, table.get(), flags.get(), w.stream);
+    }
+    if (!faces.empty()) {
+        DBuf<int> ovf(w.pool, faces.size());
+        launch_fill_i32(ovf.get(), 0, (int64_t)faces.size(), w.stream);
+        for (size_t f = 0; f < faces.size(); ++f) {
+            MapReq &r = rq[faces[f]];
+            MapSet &m = *r.m;
+            launch_faces(r.g, tmp[faces[f]].inside.get(), r.d_xyz, r.t->radii.p, r.t->max_radius,
+                         r.c->cl, r.ein, r.eout, m.ex.get(), m.ey.get(), m.ez.get(), m.fx.get(),
+                         m.fy.get(), m.fz.get(), ovf.get() + f, w.stream);
+        }
+        std::vector<int> ho(faces.size());
+        HIP_OK(hipMemcpyAsync(ho.data(), ovf.get(), sizeof(int) * ho.size(),
+                              hipMemcpyDeviceToHost, w.stream));
         w.sync();
-        if (w.pin_i[9]) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "face interval list overflow");
+        for (int v : ho)
+            if (v) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "face interval list overflow");
     }
-    if (ionic) {
-        DBuf<uint8_t> ion(w.pool, N);
-        launch_inside(g, d_xyz, t.radii.p, p.ion_radius, t.max_radius, c.cl, ion.get(), w.stream);
-        const double kb2 = p.solvent_dielectric * kappa * kappa;
-        launch_kappa_map(g, ion.get(), kb2, m.k2.get(), m.k2f.get(), w.stream);
-    } else {
-        launch_fill_f64(m.k2.get(), 0.0, N, w.stream);
-        launch_fill_f32(m.k2f.get(), 0.0f, N, w.stream);
+    for (MapReq &r : rq) {
+        MapSet &m = *r.m;
+        if (m.uni) continue;
+        const int64_t N = r.g.size();
+        if (r.ionic) {
+            DBuf<uint8_t> ion(w.pool, N);
+            launch_inside(r.g, r.d_xyz, r.t->radii.p, r.p->ion_radius, r.t->max_radius, r.c->cl,
+                          ion.get(), w.stream);
+            const double kb2 = r.p->solvent_dielectric * r.kappa * r.kappa;
+            launch_kappa_map(r.g, ion.get(), kb2, m.k2.get(), m.k2f.get(), w.stream);
+        } else {
+            launch_fill_f64(m.k2.get(), 0.0, N, w.stream);
+            launch_fill_f32(m.k2f.get(), 0.0f, N, w.stream);
+        }
+        m.c_in = r.ein;
+        m.c_out = r.eout;
+        m.c_k2 = r.ionic ? r.p->solvent_dielectric * r.kappa * r.kappa : 0.0;
+        m.cw = DBuf<uint8_t>(w.pool, N);
+        mg_pack_codes(r.g.nx, r.g.ny, r.g.nz, m.ex.get(), m.ey.get(), m.ez.get(), m.k2.get(),
+                      m.c_in, m.c_out, m.cw.get(), w.stream);
     }
-    m.c_in = ein;
-    m.c_out = eout;
-    m.c_k2 = ionic ? p.solvent_dielectric * kappa * kappa : 0.0;
-    m.cw = DBuf<uint8_t>(w.pool, N);
-    mg_pack_codes(g.nx, g.ny, g.nz, m.ex.get(), m.ey.get(), m.ez.get(), m.k2.get(), m.c_in,
-                  m.c_out, m.cw.get(), w.stream);
 }
 
 static MgOuterDev outer_view(const GridDesc &g, const MapSet &m, const double *rho)
@@ -824,12 +875,21 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
     }
     // coarse focusing solves with Debye-Hueckel boundaries (pb_energy.rs:150-186)
     std::vector<MgTask *> tasks;
+    {
+        std::vector<MapReq> rq;
+        for (PbJob *j : act) {
+            PbJob::State &S = *j->st;
+            if (!S.focus) continue;
+            for (auto &sd : S.side)
+                rq.push_back(MapReq{j->t, j->d_xyz, &S.cells, S.cg, &j->p, sd.ein, sd.eout,
+                                    sd.ionic, S.kappa, &sd.cm});
+        }
+        build_maps_batch(w, rq);
+    }
     for (PbJob *j : act) {
         PbJob::State &S = *j->st;
         if (!S.focus) continue;
         for (auto &sd : S.side) {
-            build_maps(w, *j->t, j->d_xyz, S.cells, S.cg, j->p, sd.ein, sd.eout, sd.ionic, S.kappa,
-                       sd.cm);
             sd.cphi = DBuf<double>(w.pool, S.cg.size());
             launch_fill_f64(sd.cphi.get(), 0.0, S.cg.size(), w.stream);
             DBuf<double> dhw(w.pool, dh_work_size(S.cg, j->t->h.n));
@@ -893,11 +953,19 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
     }
     // fine solves
     tasks.clear();
+    {
+        std::vector<MapReq> rq;
+        for (PbJob *j : act) {
+            PbJob::State &S = *j->st;
+            for (auto &sd : S.side)
+                rq.push_back(MapReq{j->t, j->d_xyz, &S.cells, S.fine, &j->p, sd.ein, sd.eout,
+                                    sd.ionic, S.kappa, &sd.fm});
+        }
+        build_maps_batch(w, rq);
+    }
     for (PbJob *j : act) {
         PbJob::State &S = *j->st;
         for (auto &sd : S.side) {
-            build_maps(w, *j->t, j->d_xyz, S.cells, S.fine, j->p, sd.ein, sd.eout, sd.ionic,
-                       S.kappa, sd.fm);
             sd.ft.g = S.fine;
             sd.ft.m = &sd.fm;
             sd.ft.rho = S.rho_f.get();
