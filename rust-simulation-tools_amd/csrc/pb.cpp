@@ -595,17 +595,18 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         if (phase_pass) {
             // pre: from zero into e2; post: e2 plus the prolongated coarse
             // correction (added by its own kernel) into e
-            Ev ev{nullptr, nullptr, cycle_counter, !pre, false};
-            if (profiling && l == 0) {
-                HIP_OK(hipEventCreate(&ev.a));
-                HIP_OK(hipEventCreate(&ev.b));
-                HIP_OK(hipEventRecord(ev.a, w.stream));
-            }
             if (!pre) {
                 if (stream)
                     mg_prolong_e2s(F, C, S, strblk[l], dst, w.stream);
                 else
                     mg_prolong_e2(F, C, S, maxnodes[l], dst, w.stream);
+            }
+            // the events bracket the smoother launch alone
+            Ev ev{nullptr, nullptr, cycle_counter, !pre, false};
+            if (profiling && l == 0) {
+                HIP_OK(hipEventCreate(&ev.a));
+                HIP_OK(hipEventCreate(&ev.b));
+                HIP_OK(hipEventRecord(ev.a, w.stream));
             }
             mg_smooth3(F, nullptr, S, pre ? 1 : 0, pre ? 1 : 0, sblocks[l], dst, w.stream);
             if (profiling && l == 0) {
