@@ -101,18 +101,17 @@ int smooth3_tiles(int nx, int ny);
 int smooth3_zc(int nx, int ny, int nz, int n_solves);
 void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
                 int max_blocks, const MgStatus *st, hipStream_t s);
-// z-streaming versions of residual32 / prolong_e2 (same results bit for bit):
-// one thread per in-plane node over L.zc planes, mg_stream_blocks workgroups
-// per solve.
+// z-streaming version of prolong_e2 (same results bit for bit): one thread
+// per in-plane node over L.zc planes, mg_stream_blocks workgroups per solve.
 int mg_stream_blocks(int nx, int ny, int nz, int zc);
-void mg_residual32s(const MgLevelDev *L, int n_solves, int max_blocks, const MgStatus *st,
-                    hipStream_t s);
 void mg_prolong_e2s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
                     const MgStatus *st, hipStream_t s);
-// restrict into C.f, one thread per coarse column over C.zc coarse planes
-// (mg_stream_blocks of the coarse level)
-void mg_restrict_s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
-                   const MgStatus *st, hipStream_t s);
+// Residual of e2 and its restriction into C.f in one pass (no residual
+// array), equal bit for bit to mg_residual32 then mg_restrict;
+// mg_rr_blocks(coarse dims, C.zc) workgroups per solve.
+int mg_rr_blocks(int cnx, int cny, int cnz, int czc);
+void mg_residual_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
+                          const MgStatus *st, hipStream_t s);
 // e2 += prolongation of C's correction (post-smoothing input of smooth3)
 void mg_prolong_e2(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_nodes,
                    const MgStatus *st, hipStream_t s);

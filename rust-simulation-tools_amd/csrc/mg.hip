@@ -557,7 +557,9 @@ int smooth3_tiles(int nx, int ny) { return ceil_div(nx, s3::TILE) * ceil_div(ny,
 
 // Planes per workgroup: about 64 (a chunk also streams 2 K = 12 planes it does
 // not write), shorter when a launch of n_solves such levels would otherwise
-// have fewer than 2048 workgroups (8 per CU), but at least 8.
+// have fewer than 2048 workgroups (8 per CU), but at least 8.  (Whole columns
+// do fewer steps but measured no faster: the halo reads of neighbouring tiles
+// then meet less often in L2.)
 int smooth3_zc(int nx, int ny, int nz, int n_solves)
 {
     const int64_t tiles = (int64_t)smooth3_tiles(nx, ny) * std::max(n_solves, 1);
@@ -1053,103 +1055,6 @@ __device__ __forceinline__ bool stream_pos(int nx, int ny, int nz, int zc, int b
     return true;
 }
 
-// residual32_kernel over e2 (pre-smoothed correction) -> res
-__global__ __launch_bounds__(256) void residual32s_kernel(const MgLevelDev *__restrict__ Ls,
-                                                          const MgStatus *__restrict__ st)
-{
-    using s3::kOob;
-    const BatchBlock bb = batch_block();
-    if (stopped(st + bb.s)) return;
-    const MgLevelDev L = Ls[bb.s];
-    if (L.nx == 0) return;
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    StreamPos q;
-    if (bb.b >= stream_blocks(nx, ny, nz, L.zc) || !stream_pos(nx, ny, nz, L.zc, bb.b, q)) return;
-    const uint32_t sxy = (uint32_t)nx * (uint32_t)ny, N = sxy * (uint32_t)nz;
-    const bool uni = L.uni != 0, hasw = !uni && L.cw;
-    const auto er = buf_rsrc(L.e2, 4u * N), fr = buf_rsrc(L.f, 4u * N);
-    const auto rr = buf_rsrc(L.res, 4u * N);
-    const auto cr = buf_rsrc(hasw ? L.cw : nullptr, hasw ? N : 0u);
-    const auto exr = buf_rsrc(uni ? nullptr : L.ex, uni ? 0u : 4u * N);
-    const auto eyr = buf_rsrc(uni ? nullptr : L.ey, uni ? 0u : 4u * N);
-    const auto ezr = buf_rsrc(uni ? nullptr : L.ez, uni ? 0u : 4u * N);
-    const auto k2r = buf_rsrc(uni || hasw ? nullptr : L.k2, uni || hasw ? 0u : 4u * N);
-    const bool inpl = q.x >= 1 && q.x <= nx - 2 && q.y >= 1 && q.y <= ny - 2;
-    const uint32_t po = 4u * (uint32_t)q.p;
-    // in-plane neighbour offsets (out of range for boundary columns: loads give 0)
-    const uint32_t oxp = inpl ? po + 4u : kOob, oxm = inpl ? po - 4u : kOob;
-    const uint32_t oyp = inpl ? po + 4u * (uint32_t)nx : kOob;
-    const uint32_t oym = inpl ? po - 4u * (uint32_t)nx : kOob;
-    const float u = L.eps_u, ihx = L.ihx, ihy = L.ihy, ihz = L.ihz;
-    auto e_at = [&](int z) { return (z >= 0 && z < nz) ? bload(er, po, 4u * sxy * (uint32_t)z) : 0.0f; };
-    struct Pl {
-        float xp, xm, yp, ym, f;
-        uint32_t w;
-    };
-    auto load_pl = [&](int z, Pl &P) {
-        const bool zi = z >= 1 && z <= nz - 2;
-        const uint32_t zo = zi ? 4u * sxy * (uint32_t)z : 0u;
-        P.xp = bload(er, zi ? oxp : kOob, zo);
-        P.xm = bload(er, zi ? oxm : kOob, zo);
-        P.yp = bload(er, zi ? oyp : kOob, zo);
-        P.ym = bload(er, zi ? oym : kOob, zo);
-        P.f = bload(fr, zi ? po : kOob, zo);
-        P.w = hasw ? bload8(cr, zi ? (uint32_t)q.p : kOob, zo >> 2) : 0u;
-    };
-    float em = e_at(q.z0 - 1), ec = e_at(q.z0), ep = e_at(q.z0 + 1);
-    Pl cur, nxt;
-    load_pl(q.z0, cur);
-    for (int z = q.z0; z < q.z1; ++z) {
-        const uint32_t zo = 4u * sxy * (uint32_t)z;
-        const bool act = inpl && z >= 1 && z <= nz - 2;
-        // faces of a fractional-face node (or of every node without code bytes)
-        float exp_ = 0.0f, exm = 0.0f, eyp = 0.0f, eym = 0.0f, ezp = 0.0f, ezm = 0.0f, k2 = 0.0f;
-        const bool arr = !uni && act && (!hasw || (cur.w & kCodeSpecial));
-        {
-            const uint32_t g = 4u * (uint32_t)q.p;
-            exp_ = bload(exr, arr ? g : kOob, zo);
-            exm = bload(exr, arr ? g - 4u : kOob, zo);
-            eyp = bload(eyr, arr ? g : kOob, zo);
-            eym = bload(eyr, arr ? g - 4u * (uint32_t)nx : kOob, zo);
-            ezp = bload(ezr, arr ? g : kOob, zo);
-            ezm = bload(ezr, arr ? g : kOob, arr ? zo - 4u * sxy : 0u);  // plane z - 1
-            if (!hasw) k2 = bload(k2r, arr ? g : kOob, zo);
-        }
-        const float en = e_at(z + 2);
-        if (z + 1 < q.z1) load_pl(z + 1, nxt);
-        float r = 0.0f;
-        if (act) {
-            float nb, dg;
-            if (uni) {
-                nb = u * ((cur.xp + cur.xm) * ihx + (cur.yp + cur.ym) * ihy + (ep + em) * ihz);
-                dg = u * (2.0f * ihx + 2.0f * ihy + 2.0f * ihz);
-            } else {
-                if (hasw) {
-                    const uint32_t w = cur.w;
-                    k2 = (w & kCodeKappa) ? L.c_k2 : 0.0f;
-                    if (!(w & kCodeSpecial)) {
-                        exp_ = face_bit<float>(w, 0, L.c_in, L.c_out);
-                        exm = face_bit<float>(w, 1, L.c_in, L.c_out);
-                        eyp = face_bit<float>(w, 2, L.c_in, L.c_out);
-                        eym = face_bit<float>(w, 3, L.c_in, L.c_out);
-                        ezp = face_bit<float>(w, 4, L.c_in, L.c_out);
-                        ezm = face_bit<float>(w, 5, L.c_in, L.c_out);
-                    }
-                }
-                nb = fmaf(exp_, cur.xp, exm * cur.xm) * ihx + fmaf(eyp, cur.yp, eym * cur.ym) * ihy +
-                     fmaf(ezp, ep, ezm * em) * ihz;
-                dg = (exp_ + exm) * ihx + (eyp + eym) * ihy + (ezp + ezm) * ihz + k2;
-            }
-            r = cur.f + nb - dg * ec;
-        }
-        bstore(rr, r, po, zo);
-        em = ec;
-        ec = ep;
-        ep = en;
-        cur = nxt;
-    }
-}
-
 // prolong_e2_kernel: e2 += prolongation of the coarse correction
 __global__ __launch_bounds__(256) void prolong_e2s_kernel(const MgLevelDev *__restrict__ Fs,
                                                           const MgLevelDev *__restrict__ Cs,
@@ -1198,85 +1103,245 @@ __global__ __launch_bounds__(256) void prolong_e2s_kernel(const MgLevelDev *__re
     }
 }
 
-// restrict_kernel: one thread per coarse column over C.zc coarse planes.  The
-// nine fine values of plane 2 cz + 1 serve as the next coarse plane's plane
-// 2 (cz + 1) - 1; the 27 terms are accumulated in restrict_node's order with
-// its skips, so the result is equal bit for bit.
-__global__ __launch_bounds__(256) void restrict_s_kernel(const MgLevelDev *__restrict__ Fs,
-                                                         const MgLevelDev *__restrict__ Cs,
-                                                         const MgStatus *__restrict__ st)
+// Residual and restriction in one pass (residual32_kernel then
+// restrict_kernel without the residual array).  A workgroup owns a 16 x 16
+// tile of coarse columns and a chunk of C.zc coarse planes, and streams the
+// fine planes 2 cz0 - 1 .. 2 cz1 - 1: the pre-smoothed correction e2 goes
+// through a 4-plane LDS ring (35 x 35 nodes, loaded two planes ahead), each
+// fine plane's residual on the 33 x 33 nodes the tile restricts from is
+// written to LDS (double-buffered, one barrier per plane), and every coarse
+// column extends its fma chain in restrict_node's order and skips, so both
+// the residuals and the coarse right-hand sides equal the two kernels' bit
+// for bit.
+namespace rr {
+constexpr int CT = 16;               // coarse tile edge
+constexpr int FR = 2 * CT + 1;       // fine residual region edge: 33
+constexpr int ER = FR + 2;           // correction region edge: 35
+constexpr int NE = (ER * ER + 255) / 256;  // correction values per thread: 5
+constexpr int NR = (FR * FR + 255) / 256;  // residual nodes per thread: 5
+}  // namespace rr
+
+int mg_rr_blocks(int cnx, int cny, int cnz, int czc)
 {
+    return ceil_div(cnx, rr::CT) * ceil_div(cny, rr::CT) * ceil_div(cnz, czc);
+}
+
+__global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__restrict__ Fs,
+                                                          const MgLevelDev *__restrict__ Cs,
+                                                          const MgStatus *__restrict__ st)
+{
+    using namespace rr;
     using s3::kOob;
+    __shared__ float ering[4 * ER * ER];
+    __shared__ float rpl[2 * FR * FR];
     const BatchBlock bb = batch_block();
     if (stopped(st + bb.s)) return;
     const MgLevelDev C = Cs[bb.s];
     if (C.nx == 0) return;
     const MgLevelDev F = Fs[bb.s];
-    StreamPos q;
-    if (bb.b >= stream_blocks(C.nx, C.ny, C.nz, C.zc) || !stream_pos(C.nx, C.ny, C.nz, C.zc, bb.b, q))
-        return;
-    const uint32_t fsxy = (uint32_t)F.nx * (uint32_t)F.ny, csxy = (uint32_t)C.nx * (uint32_t)C.ny;
-    const auto rr = buf_rsrc(F.res, 4u * fsxy * (uint32_t)F.nz);
-    const auto fr = buf_rsrc(C.f, 4u * csxy * (uint32_t)C.nz);
-    // fine in-plane offsets of the 3 x 3 neighbourhood (kOob outside the plane)
-    uint32_t off[9];
-    bool vin[9];
+    const int ntx = (C.nx + CT - 1) / CT, nty = (C.ny + CT - 1) / CT;
+    const int nch = (C.nz + C.zc - 1) / C.zc;
+    if (bb.b >= ntx * nty * nch) return;
+    const int tile = bb.b % (ntx * nty), ch = bb.b / (ntx * nty);
+    const int cx0 = (tile % ntx) * CT, cy0 = (tile / ntx) * CT;
+    const int cz0 = ch * C.zc, cz1 = min(C.nz, cz0 + C.zc);
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    const int fx0 = 2 * cx0 - 1, fy0 = 2 * cy0 - 1;  // residual region origin
+    const uint32_t sxy = (uint32_t)nx * (uint32_t)ny, N = sxy * (uint32_t)nz;
+    const uint32_t csxy = (uint32_t)C.nx * (uint32_t)C.ny;
+    const bool uni = F.uni != 0, hasw = !uni && F.cw;
+    const auto er = buf_rsrc(F.e2, 4u * N), fr = buf_rsrc(F.f, 4u * N);
+    const auto cr = buf_rsrc(hasw ? F.cw : nullptr, hasw ? N : 0u);
+    const auto exr = buf_rsrc(uni ? nullptr : F.ex, uni ? 0u : 4u * N);
+    const auto eyr = buf_rsrc(uni ? nullptr : F.ey, uni ? 0u : 4u * N);
+    const auto ezr = buf_rsrc(uni ? nullptr : F.ez, uni ? 0u : 4u * N);
+    const auto k2r = buf_rsrc(uni || hasw ? nullptr : F.k2, uni || hasw ? 0u : 4u * N);
+    const auto cfr = buf_rsrc(C.f, 4u * csxy * (uint32_t)C.nz);
+    const int tid = threadIdx.x;
+    const float u = F.eps_u, ihx = F.ihx, ihy = F.ihy, ihz = F.ihz;
+
+    // correction region value n = tid + 256 j: node (fx0 - 1 + n % ER, fy0 - 1 + n / ER)
+    uint32_t eo[NE];
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy)
+    for (int j = 0; j < NE; ++j) {
+        const int n = tid + 256 * j, ry = n / ER, rx = n - ry * ER;
+        const int x = fx0 - 1 + rx, y = fy0 - 1 + ry;
+        eo[j] = (n < ER * ER && x >= 0 && x < nx && y >= 0 && y < ny) ? 4u * (uint32_t)(x + nx * y)
+                                                                        : kOob;
+    }
+    // residual node n = tid + 256 j: node (fx0 + n % FR, fy0 + n / FR); bit j of
+    // rin: interior column (in-plane neighbours exist), the rest give residual 0
+    uint32_t ro[NR];
+    int rli[NR];
+    uint32_t rin = 0u;
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int x = 2 * q.x + dx, y = 2 * q.y + dy, k = 3 * (dy + 1) + (dx + 1);
-            vin[k] = x >= 0 && x < F.nx && y >= 0 && y < F.ny;
-            off[k] = vin[k] ? 4u * (uint32_t)(x + F.nx * y) : kOob;
-        }
-    auto load9 = [&](int z, float *v) {
-        const bool zi = z >= 0 && z < F.nz;
-        const uint32_t so = zi ? 4u * fsxy * (uint32_t)z : 0u;
+    for (int j = 0; j < NR; ++j) {
+        const int n = tid + 256 * j, ry = n / FR, rx = n - ry * FR;
+        const int x = fx0 + rx, y = fy0 + ry;
+        const bool in = n < FR * FR && x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2;
+        ro[j] = in ? (uint32_t)(x + nx * y) : kOob;
+        rli[j] = (ry + 1) * ER + (rx + 1);
+        if (in) rin |= 1u << j;
+    }
+    auto eslot = [&](int z) { return ering + ((z + 8) & 3) * (ER * ER); };
+    float ev[NE];
+    auto load_e = [&](int z) {
+        const bool zi = z >= 0 && z < nz;
+        const uint32_t zo = zi ? 4u * sxy * (uint32_t)z : 0u;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) v[k] = bload(rr, zi ? off[k] : kOob, so);
+        for (int j = 0; j < NE; ++j) ev[j] = bload(er, zi ? eo[j] : kOob, zo);
     };
-    float lo[9], mid[9], hi[9];
-    load9(2 * q.z0 - 1, lo);
-    for (int cz = q.z0; cz < q.z1; ++cz) {
-        load9(2 * cz, mid);
-        load9(2 * cz + 1, hi);
-        float sum = 0.0f, ws = 0.0f;
-        const float *pl[3] = {lo, mid, hi};
+    auto store_e = [&](int z) {
+        float *sl = eslot(z);
 #pragma unroll
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int z = 2 * cz + dz;
-            if (z < 0 || z >= F.nz) continue;
+        for (int j = 0; j < NE; ++j)
+            if (tid + 256 * j < ER * ER) sl[tid + 256 * j] = ev[j];
+    };
+    float fv[NR];
+    uint32_t wv[NR];
+    auto load_fw = [&](int z) {
+        const bool zi = z >= 1 && z <= nz - 2;
+        const uint32_t zo = zi ? 4u * sxy * (uint32_t)z : 0u;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                if (!vin[k]) continue;
-                const int dy = k / 3 - 1, dx = k % 3 - 1;
-                const float w = (dx == 0 ? 2.0f : 1.0f) * (dy == 0 ? 2.0f : 1.0f) *
-                                (dz == 0 ? 2.0f : 1.0f);
-                sum = fmaf(w, pl[dz + 1][k], sum);
-                ws += w;
+        for (int j = 0; j < NR; ++j) {
+            const uint32_t o = zi ? ro[j] : kOob;
+            fv[j] = bload(fr, o == kOob ? kOob : 4u * o, zo);
+            wv[j] = hasw ? bload8(cr, o, zo >> 2) : 0u;
+        }
+    };
+
+    const int zs = 2 * cz0 - 1, ze = 2 * cz1 - 1;  // fine planes streamed: zs .. ze
+    for (int z = zs - 1; z <= zs + 1; ++z) {
+        load_e(z);
+        store_e(z);
+    }
+    load_e(zs + 2);
+    load_fw(zs);
+    __syncthreads();
+
+    // coarse column of this thread and its fine 3 x 3 neighbourhood in rpl
+    const int ccx = cx0 + (tid & (CT - 1)), ccy = cy0 + (tid / CT);
+    const bool cown = ccx < C.nx && ccy < C.ny;
+    float sum = 0.0f, ws = 0.0f;
+    for (int z = zs; z <= ze; ++z) {
+        store_e(z + 2);
+        const bool zin = z >= 1 && z <= nz - 2;
+        // faces of fractional-face nodes (or of every node without code bytes)
+        // (loads only for the lanes that need them: a wave without such a
+        // node skips them)
+        float fa[NR][7];
+        const uint32_t zo = 4u * sxy * (uint32_t)(zin ? z : 0);
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) fa[j][k] = 0.0f;
+            const bool arr = !uni && zin && ((rin >> j) & 1u) && (!hasw || (wv[j] & kCodeSpecial));
+            if (arr) {
+                const uint32_t g = 4u * ro[j];
+                fa[j][0] = bload(exr, g, zo);
+                fa[j][1] = bload(exr, g - 4u, zo);
+                fa[j][2] = bload(eyr, g, zo);
+                fa[j][3] = bload(eyr, g - 4u * (uint32_t)nx, zo);
+                fa[j][4] = bload(ezr, g, zo);
+                fa[j][5] = bload(ezr, g, zo - 4u * sxy);  // plane z - 1
+                if (!hasw) fa[j][6] = bload(k2r, g, zo);
             }
         }
-        bstore(fr, 0.25f * sum / ws, 4u * (uint32_t)q.p, 4u * csxy * (uint32_t)cz);
+        const float *em = eslot(z - 1), *ec = eslot(z), *ep = eslot(z + 1);
+        float *rp = rpl + (z & 1) * (FR * FR);
+        float fcur[NR];
+        uint32_t wcur[NR];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) lo[k] = hi[k];
+        for (int j = 0; j < NR; ++j) {
+            fcur[j] = fv[j];
+            wcur[j] = wv[j];
+        }
+        if (z + 1 <= ze) {
+            load_e(z + 3);
+            load_fw(z + 1);
+        }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const int n = tid + 256 * j;
+            if (n >= FR * FR) continue;
+            float r = 0.0f;
+            if (zin && ((rin >> j) & 1u)) {
+                const int li = rli[j];
+                const float xp = ec[li + 1], xm = ec[li - 1], yp = ec[li + ER], ym = ec[li - ER];
+                const float zp = ep[li], zm = em[li], c = ec[li];
+                float nb, dg;
+                if (uni) {
+                    nb = u * ((xp + xm) * ihx + (yp + ym) * ihy + (zp + zm) * ihz);
+                    dg = u * (2.0f * ihx + 2.0f * ihy + 2.0f * ihz);
+                } else {
+                    float exp_ = fa[j][0], exm = fa[j][1], eyp = fa[j][2], eym = fa[j][3];
+                    float ezp = fa[j][4], ezm = fa[j][5], k2 = fa[j][6];
+                    if (hasw) {
+                        const uint32_t w = wcur[j];
+                        k2 = (w & kCodeKappa) ? F.c_k2 : 0.0f;
+                        if (!(w & kCodeSpecial)) {
+                            exp_ = face_bit<float>(w, 0, F.c_in, F.c_out);
+                            exm = face_bit<float>(w, 1, F.c_in, F.c_out);
+                            eyp = face_bit<float>(w, 2, F.c_in, F.c_out);
+                            eym = face_bit<float>(w, 3, F.c_in, F.c_out);
+                            ezp = face_bit<float>(w, 4, F.c_in, F.c_out);
+                            ezm = face_bit<float>(w, 5, F.c_in, F.c_out);
+                        }
+                    }
+                    nb = fmaf(exp_, xp, exm * xm) * ihx + fmaf(eyp, yp, eym * ym) * ihy +
+                         fmaf(ezp, zp, ezm * zm) * ihz;
+                    dg = (exp_ + exm) * ihx + (eyp + eym) * ihy + (ezp + ezm) * ihz + k2;
+                }
+                r = fcur[j] + nb - dg * c;
+            }
+            rp[n] = r;
+        }
+        __syncthreads();
+        // restriction: plane z is dz = 0 of coarse plane z / 2 (z even), or
+        // dz = +1 of (z - 1) / 2, which is then complete, and dz = -1 of
+        // (z + 1) / 2 (z odd)
+        auto terms = [&](int dz) {
+            if (z < 0 || z >= nz) return;
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int y = 2 * ccy + dy;
+                if (y < 0 || y >= ny) continue;
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int x = 2 * ccx + dx;
+                    if (x < 0 || x >= nx) continue;
+                    const float w = (dx == 0 ? 2.0f : 1.0f) * (dy == 0 ? 2.0f : 1.0f) *
+                                    (dz == 0 ? 2.0f : 1.0f);
+                    sum = fmaf(w, rp[(y - fy0) * FR + (x - fx0)], sum);
+                    ws += w;
+                }
+            }
+        };
+        if (cown) {
+            if ((z & 1) == 0) {
+                terms(0);
+            } else {
+                terms(1);
+                const int cz = (z - 1) >> 1;
+                if (cz >= cz0)
+                    bstore(cfr, 0.25f * sum / ws, 4u * (uint32_t)(ccx + C.nx * ccy),
+                           4u * csxy * (uint32_t)cz);
+                sum = 0.0f;
+                ws = 0.0f;
+                terms(-1);
+            }
+        }
     }
 }
 
-void mg_restrict_s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
-                   const MgStatus *st, hipStream_t s)
+void mg_residual_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
+                          const MgStatus *st, hipStream_t s)
 {
     if (n_solves <= 0 || max_blocks <= 0) return;
-    hipLaunchKernelGGL(restrict_s_kernel, dim3(max_blocks, n_solves), dim3(256), 0, s, F, C, st);
+    hipLaunchKernelGGL(resrestrict_kernel, dim3(max_blocks, n_solves), dim3(256), 0, s, F, C, st);
     LAUNCH_OK();
 }
 
-void mg_residual32s(const MgLevelDev *L, int n_solves, int max_blocks, const MgStatus *st,
-                    hipStream_t s)
-{
-    if (n_solves <= 0 || max_blocks <= 0) return;
-    hipLaunchKernelGGL(residual32s_kernel, dim3(max_blocks, n_solves), dim3(256), 0, s, L, st);
-    LAUNCH_OK();
-}
 void mg_prolong_e2s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
                     const MgStatus *st, hipStream_t s)
 {

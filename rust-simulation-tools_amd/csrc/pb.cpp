@@ -365,7 +365,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     // MMPBSA_SMOOTHER=sweep asks for one pass per sweep (smooth_kernel)
     const char *sm_env = std::getenv("MMPBSA_SMOOTHER");
     const bool phase_pass = !(sm_env && std::strcmp(sm_env, "sweep") == 0);
-    // z-streaming residual and prolongation unless MMPBSA_STREAM=0
+    // z-streaming residual + restriction and prolongation unless MMPBSA_STREAM=0
     const char *st_env = std::getenv("MMPBSA_STREAM");
     const bool stream = !(st_env && std::strcmp(st_env, "0") == 0);
 
@@ -486,7 +486,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     std::vector<int64_t> maxnodes(maxT, 0), maxcoarse(maxT, 0);
     std::vector<int> sblocks(maxT, 0);  // smoother launch width per level
     std::vector<int> strblk(maxT, 0);   // streaming-kernel launch width per level
-    std::vector<int> rstblk(maxT, 0);   // streaming restriction width (coarse level)
+    std::vector<int> rrblk(maxT, 0);    // residual + restriction width
     int64_t maxN0 = 0;
     for (int p = 0; p < 2; ++p) {
         H[p].F.assign(nF, null_level());
@@ -516,8 +516,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 maxnodes[l] = std::max(maxnodes[l], lv_nodes(tw.L[l].g));
                 strblk[l] = std::max(strblk[l], mg_stream_blocks(g.nx, g.ny, g.nz, tw.L[l].d.zc));
                 const GridDesc &gc = tw.L[l + 1].g;
-                rstblk[l] = std::max(rstblk[l],
-                                     mg_stream_blocks(gc.nx, gc.ny, gc.nz, tw.L[l + 1].d.zc));
+                rrblk[l] = std::max(rrblk[l], mg_rr_blocks(gc.nx, gc.ny, gc.nz, tw.L[l + 1].d.zc));
                 maxcoarse[l] = std::max(maxcoarse[l], lv_nodes(tw.L[l + 1].g));
             }
             MgTailDev &T = H[p].T[s];
@@ -638,14 +637,12 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const MgLevelDev *F = D[p].F + (size_t)l * S;
             const MgLevelDev *C = D[p].C + (size_t)l * S;
             smooth_phase(F, C, l, true);
-            if (stream)
-                mg_residual32s(F, S, strblk[l], dst, w.stream);
-            else
+            if (stream) {
+                mg_residual_restrict(F, C, S, rrblk[l], dst, w.stream);
+            } else {
                 mg_residual32(F, S, maxnodes[l], dst, w.stream);
-            if (stream)
-                mg_restrict_s(F, C, S, rstblk[l], dst, w.stream);
-            else
                 mg_restrict(F, C, S, maxcoarse[l], dst, w.stream);
+            }
         }
         mg_tail(D[p].T, S, dst, w.stream);
         for (int l = maxT - 1; l >= 0; --l) {

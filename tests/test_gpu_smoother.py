@@ -2,8 +2,8 @@
 non-uniform maps, focused and unfocused, single and batched:
 - the one-pass-per-phase smoother (three red-black sweeps per launch) against
   the one-pass-per-sweep smoother (the same updates in the same order);
-- the z-streaming residual and prolongation kernels against the per-node
-  ones (the same expressions)."""
+- the z-streaming prolongation and the fused residual + restriction pass
+  against the per-node kernels (the same expressions, the same fma chains)."""
 import os
 
 import numpy as np
