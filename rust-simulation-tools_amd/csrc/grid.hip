@@ -259,15 +259,45 @@ __global__ void dil_yz_kernel(GridDesc g, int R, int T, int axis, const uint8_t 
     out[idx] = final_test ? (uint8_t)(best > T ? 1 : 0) : (uint8_t)best;
 }
 
+// Forward fill on a 32-node chunk of a line: bits of F (exterior, or the
+// carry-in at bit 0) spread upwards through runs of open (outside-SAS)
+// nodes, as the node-by-node sweep would.  Log-step prefix: after the step
+// with shift s, F holds every open node at most 2 s - 1 open nodes above a
+// seed, and M the nodes whose s lower neighbours are all open.
+__device__ __forceinline__ uint32_t fill_up(uint32_t F, uint32_t open)
+{
+    uint32_t M = open;
+#pragma unroll
+    for (int s = 1; s < 32; s <<= 1) {
+        F |= M & (F << s);
+        M &= M << s;
+    }
+    return F;
+}
+__device__ __forceinline__ uint32_t fill_down(uint32_t F, uint32_t open)
+{
+    uint32_t M = open;
+#pragma unroll
+    for (int s = 1; s < 32; s <<= 1) {
+        F |= M & (F >> s);
+        M &= M >> s;
+    }
+    return F;
+}
+
 // One thread per line along `axis` of map blockIdx.y of the table: forward
-// then backward propagation through outside-SAS nodes.
+// then backward propagation through outside-SAS nodes, 32 nodes at a time
+// (the chunk's loads are independent; the sweep itself runs on bit masks).
+// After both passes every open run of the line that holds an exterior node
+// is exterior, as after the node-by-node sweeps.
 __global__ void flood_batch_kernel(const FloodJob *__restrict__ jobs, int axis)
 {
     const FloodJob J = jobs[blockIdx.y];
     if (*J.active == 0) return;  // this map reached its fixpoint in an earlier round
     const GridDesc g = J.g;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t base, stride, len;
+    int64_t base, stride;
+    int len;
     if (axis == 0) {
         if (t >= (int64_t)g.ny * g.nz) return;
         base = t * g.nx;
@@ -285,34 +315,70 @@ __global__ void flood_batch_kernel(const FloodJob *__restrict__ jobs, int axis)
         stride = (int64_t)g.nx * g.ny;
         len = g.nz;
     }
-    const uint8_t *__restrict__ sas = J.sas;
-    uint8_t *__restrict__ ext = J.ext;
-    int ch = 0;
-    uint8_t prev = 0;
-    for (int64_t k = 0; k < len; ++k) {
-        const int64_t i = base + k * stride;
-        const uint8_t sv = sas[i];
-        uint8_t e = ext[i];
-        if (!sv && !e && prev) {
-            e = 1;
-            ext[i] = 1;
-            ch = 1;
+    const auto sas = to_global<const uint8_t>(J.sas);
+    const auto ext = to_global<uint8_t>(J.ext);
+    const int nch = (len + 31) / 32;
+    // all 64 loads of a chunk are issued before the first use (nodes past the
+    // line's end re-read its last node and are masked off)
+    auto load = [&](int c, uint32_t &open, uint32_t &E) {
+        const int k0 = 32 * c, n = min(32, len - k0);
+        uint8_t sv[32], ev[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const int64_t i = base + (int64_t)(k0 + min(k, n - 1)) * stride;
+            sv[k] = sas[i];
+            ev[k] = ext[i];
         }
-        prev = sv ? 0 : e;
+        open = 0u;
+        E = 0u;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            open |= (uint32_t)(sv[k] == 0) << k;
+            E |= (uint32_t)(ev[k] != 0) << k;
+        }
+        const uint32_t live = n == 32 ? ~0u : (1u << n) - 1u;
+        open &= live;
+        E &= live;
+    };
+    auto store = [&](int c, uint32_t nw) {
+        const int k0 = 32 * c;
+        while (nw) {
+            const int k = __ffs(nw) - 1;
+            ext[base + (int64_t)(k0 + k) * stride] = 1;
+            nw &= nw - 1u;
+        }
+    };
+    uint32_t ch = 0u;
+    uint32_t carry = 0u;  // the previous chunk's last node is open and exterior
+    for (int c = 0; c < nch; ++c) {
+        uint32_t open, E;
+        load(c, open, E);
+        const int n = min(32, len - 32 * c);
+        const uint32_t F = fill_up((E | carry) & open, open);
+        store(c, F & ~E);
+        ch |= F & ~E;
+        carry = (F >> (n - 1)) & 1u;
     }
-    prev = 0;
-    for (int64_t k = len - 1; k >= 0; --k) {
-        const int64_t i = base + k * stride;
-        const uint8_t sv = sas[i];
-        uint8_t e = ext[i];
-        if (!sv && !e && prev) {
-            e = 1;
-            ext[i] = 1;
-            ch = 1;
-        }
-        prev = sv ? 0 : e;
+    carry = 0u;
+    for (int c = nch - 1; c >= 0; --c) {
+        uint32_t open, E;
+        load(c, open, E);
+        const int n = min(32, len - 32 * c);
+        const uint32_t F = fill_down((E | (carry << (n - 1))) & open, open);
+        store(c, F & ~E);
+        ch |= F & ~E;
+        carry = F & 1u;
     }
     if (ch) *J.changed = 1;
+}
+
+// End of a flood round of one map: active = changed, changed = 0.
+__global__ void flood_round_end_kernel(const FloodJob *__restrict__ job)
+{
+    if (threadIdx.x == 0) {
+        *job->active = *job->changed;
+        *job->changed = 0;
+    }
 }
 
 void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d_flags,
@@ -330,25 +396,31 @@ void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d
         ht[k] = FloodJob{J.g, J.sas, J.ext, d_flags + k, d_flags + n + k};
     }
     HIP_OK(hipMemcpyAsync(d_table, ht.data(), sizeof(FloodJob) * n, hipMemcpyHostToDevice, s));
+    launch_fill_i32(d_flags, 0, n, s);
     launch_fill_i32(d_flags + n, 1, n, s);
-    // rounds of each map until a round changes nothing (one host check per
-    // round).  Maps are flooded one after another: sharing launches between
-    // maps measured slower, as each map's x sweeps then compete for the
-    // caches and every map waited for the slowest one's rounds.
-    int hf = 0;
+    // rounds of each map until a round changes nothing.  kRounds rounds are
+    // queued per host check; a round after the fixpoint finds active = 0 and
+    // its launches exit at once.  Maps are flooded one after another: sharing
+    // launches between maps measured slower, as each map's x sweeps then
+    // compete for the caches and every map waited for the slowest one's rounds.
+    constexpr int kRounds = 4;
+    int act = 1;
     for (int k = 0; k < n; ++k) {
         const GridDesc &g = jobs[k].g;
         const int64_t ln[3] = {(int64_t)g.ny * g.nz, (int64_t)g.nx * g.nz, (int64_t)g.nx * g.ny};
-        for (int round = 0; round < 100000; ++round) {
-            launch_fill_i32(d_flags + k, 0, 1, s);
-            for (int ax = 0; ax < 3; ++ax) {
-                hipLaunchKernelGGL(flood_batch_kernel, dim3(ceil_div(ln[ax], 128), 1), dim3(128),
-                                   0, s, d_table + k, ax);
+        for (int round = 0; round < 100000; round += kRounds) {
+            for (int r = 0; r < kRounds; ++r) {
+                for (int ax = 0; ax < 3; ++ax) {
+                    hipLaunchKernelGGL(flood_batch_kernel, dim3(ceil_div(ln[ax], 128), 1),
+                                       dim3(128), 0, s, d_table + k, ax);
+                    LAUNCH_OK();
+                }
+                hipLaunchKernelGGL(flood_round_end_kernel, dim3(1), dim3(64), 0, s, d_table + k);
                 LAUNCH_OK();
             }
-            HIP_OK(hipMemcpyAsync(&hf, d_flags + k, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipMemcpyAsync(&act, d_flags + n + k, sizeof(int), hipMemcpyDeviceToHost, s));
             HIP_OK(hipStreamSynchronize(s));
-            if (!hf) break;
+            if (!act) break;
         }
     }
 

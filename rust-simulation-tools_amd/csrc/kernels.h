@@ -65,8 +65,11 @@ struct GridDesc {
 };
 int64_t boundary_count(const GridDesc &g);
 size_t dh_work_size(const GridDesc &g, int n);  // doubles of scratch
+// phi_a: the medium with kappa and eps_a; phi_b (or null): a salt-free medium
+// with eps_b, from the same pass over the atoms.
 void launch_dh_boundary(const GridDesc &g, int n, const double *xyz, const double *q,
-                        double kappa, double eps_out, double *phi, double *work, hipStream_t s);
+                        double kappa, double eps_a, double *phi_a, double eps_b, double *phi_b,
+                        double *work, hipStream_t s);
 
 // Decomposition atom sums: for each atom a in A, sum over b in B of the
 // half pair vdW / elec (dd) into work, then reduce into out_vdw / out_elec.
@@ -97,7 +100,7 @@ void launch_inside(const GridDesc &g, const double *xyz, const double *radii, do
                    double max_r, const CellList &cl, uint8_t *inside, hipStream_t s);
 // SES classification from SAS maps (flood fill of the exterior + bounded
 // distance transform) for many maps: each map's flood rounds (one host check
-// per round), then each map's distance test.
+// per four rounds), then each map's distance test.
 struct SesJob {
     GridDesc g;
     const uint8_t *sas;
@@ -109,7 +112,7 @@ struct FloodJob {
     const uint8_t *sas;
     uint8_t *ext;
     int *changed;
-    const int *active;  // 0 once a round changed nothing in this map
+    int *active;  // 0 once a round changed nothing in this map
 };
 // d_table: device room for jobs.size() FloodJobs; d_flags: 2 jobs.size() ints
 void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d_flags,

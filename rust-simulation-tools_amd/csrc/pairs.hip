@@ -499,16 +499,19 @@ __device__ inline void bnd_node(int nx, int ny, int nz, int64_t b, int &ix, int 
 
 int64_t boundary_count(const GridDesc &g) { return bnd_count(g.nx, g.ny, g.nz); }
 
-// set_dh_boundary, pb_solver.rs:68-93.  Positions relative to the grid origin
-// in FP32 (|dx| error ~1e-5 A at r >= the grid buffer), terms FP32, sums FP64.
-// Grid (boundary blocks, atom chunks): each block sums one atom chunk into
-// work[chunk][node]; dh_reduce_kernel adds the chunks in order.
-template <bool SALT>
-__global__ __launch_bounds__(256) void dh_boundary_kernel(GridDesc g, int n, int chunk,
+// set_dh_boundary, pb_solver.rs:68-93, for one or two media at once: sum A
+// = sum_i q_i exp(-kappa r) / r (kappa > 0) and sum B = sum_i q_i / r share
+// each pair's distance.  Positions relative to the grid origin in FP32 (|dx|
+// error ~1e-5 A at r >= the grid buffer), terms FP32 (one rsqrt per pair),
+// sums over 256 atoms FP32 then FP64.  Grid (boundary blocks, atom chunks):
+// each block sums one atom chunk into work[chunk][node] (A) and
+// work[chunks + chunk][node] (B); dh_reduce_kernel adds the chunks in order
+// and applies 332.0522 / eps_out in FP64.
+template <bool SALT, bool PAIR>
+__global__ __launch_bounds__(256) void dh_boundary_kernel(GridDesc g, int n, int chunk, int chunks,
                                                           const double *__restrict__ xyz,
                                                           const double *__restrict__ q,
-                                                          float kappa, double cq_scale,
-                                                          double *__restrict__ work)
+                                                          float kappa, double *__restrict__ work)
 {
     __shared__ float4 sa[256];
     const int64_t nbnd = bnd_count(g.nx, g.ny, g.nz);
@@ -519,42 +522,60 @@ __global__ __launch_bounds__(256) void dh_boundary_kernel(GridDesc g, int n, int
     const float px = (float)((double)ix * g.h), py = (float)((double)iy * g.h),
                 pz = (float)((double)iz * g.h);
     const int a_lo = blockIdx.y * chunk, a_hi = min(n, a_lo + chunk);
-    double acc = 0.0;
+    double acc_a = 0.0, acc_b = 0.0;
     for (int a0 = a_lo; a0 < a_hi; a0 += 256) {
         const int a = a0 + threadIdx.x;
         if (a < a_hi)
             sa[threadIdx.x] = make_float4((float)(xyz[3 * a] - g.ox), (float)(xyz[3 * a + 1] - g.oy),
-                                          (float)(xyz[3 * a + 2] - g.oz), (float)(cq_scale * q[a]));
+                                          (float)(xyz[3 * a + 2] - g.oz), (float)q[a]);
         __syncthreads();
         const int m = min(256, a_hi - a0);
-        float part = 0.0f;
+        float pa = 0.0f, pb = 0.0f;
         if (act) {
             for (int k = 0; k < m; ++k) {
                 const float4 A = sa[k];
                 const float dx = px - A.x, dy = py - A.y, dz = pz - A.z;
                 const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const float r = __builtin_sqrtf(r2);
-                const float w = SALT ? A.w * __expf(-kappa * r) : A.w;
-                part = r > 1e-10f ? fmaf(w, __builtin_amdgcn_rcpf(r), part) : part;
+                const bool ok = r2 > 1e-20f;
+                const float ir = __builtin_amdgcn_rsqf(r2);
+                const float t = A.w * ir;  // q / r
+                if (SALT) {
+                    const float e = __expf(-kappa * (r2 * ir));
+                    pa = ok ? fmaf(t, e, pa) : pa;
+                }
+                if (!SALT || PAIR) pb = ok ? pb + t : pb;
             }
-            acc += (double)part;
+            acc_a += (double)pa;
+            acc_b += (double)pb;
         }
         __syncthreads();
     }
-    if (act) work[(size_t)blockIdx.y * nbnd + b] = acc;
+    if (act) {
+        if (SALT) work[(size_t)blockIdx.y * nbnd + b] = acc_a;
+        if (!SALT || PAIR) work[(size_t)(blockIdx.y + (SALT ? chunks : 0)) * nbnd + b] = acc_b;
+    }
 }
 
+// phi_x[node] = scale_x * (sum of the chunks of sum x), x = A (work rows
+// 0 .. chunks - 1) and, with phi_b, B (rows chunks .. 2 chunks - 1)
 __global__ void dh_reduce_kernel(GridDesc g, int chunks, const double *__restrict__ work,
-                                 double *__restrict__ phi)
+                                 double scale_a, double *__restrict__ phi_a, double scale_b,
+                                 double *__restrict__ phi_b)
 {
     const int64_t nbnd = bnd_count(g.nx, g.ny, g.nz);
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbnd) return;
-    double s = 0.0;
-    for (int c = 0; c < chunks; ++c) s += work[(size_t)c * nbnd + b];
     int ix, iy, iz;
     bnd_node(g.nx, g.ny, g.nz, b, ix, iy, iz);
-    phi[(int64_t)ix + (int64_t)g.nx * (iy + (int64_t)g.ny * iz)] = s;
+    const int64_t idx = (int64_t)ix + (int64_t)g.nx * (iy + (int64_t)g.ny * iz);
+    double s = 0.0;
+    for (int c = 0; c < chunks; ++c) s += work[(size_t)c * nbnd + b];
+    phi_a[idx] = scale_a * s;
+    if (phi_b) {
+        double t = 0.0;
+        for (int c = 0; c < chunks; ++c) t += work[(size_t)(chunks + c) * nbnd + b];
+        phi_b[idx] = scale_b * t;
+    }
 }
 
 static int dh_chunks(const GridDesc &g, int n)
@@ -566,11 +587,12 @@ static int dh_chunks(const GridDesc &g, int n)
 
 size_t dh_work_size(const GridDesc &g, int n)
 {
-    return (size_t)dh_chunks(g, n) * (size_t)std::max<int64_t>(1, boundary_count(g));
+    return 2 * (size_t)dh_chunks(g, n) * (size_t)std::max<int64_t>(1, boundary_count(g));
 }
 
 void launch_dh_boundary(const GridDesc &g, int n, const double *xyz, const double *q,
-                        double kappa, double eps_out, double *phi, double *work, hipStream_t s)
+                        double kappa, double eps_a, double *phi_a, double eps_b, double *phi_b,
+                        double *work, hipStream_t s)
 {
     const int64_t nb = boundary_count(g);
     if (nb == 0) return;
@@ -578,14 +600,29 @@ void launch_dh_boundary(const GridDesc &g, int n, const double *xyz, const doubl
     const int chunk = ceil_div(ceil_div(std::max(n, 1), chunks), 256) * 256;
     const int used = std::max(1, ceil_div(std::max(n, 1), chunk));
     const dim3 grid(ceil_div(nb, 256), used);
-    if (kappa > 0.0)
-        hipLaunchKernelGGL(dh_boundary_kernel<true>, grid, dim3(256), 0, s, g, n, chunk, xyz, q,
-                           (float)kappa, kCoulomb / eps_out, work);
-    else
-        hipLaunchKernelGGL(dh_boundary_kernel<false>, grid, dim3(256), 0, s, g, n, chunk, xyz, q,
-                           0.0f, kCoulomb / eps_out, work);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(dh_reduce_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, g, used, work, phi);
+    const double sa = kCoulomb / eps_a, sb = kCoulomb / eps_b;
+    if (kappa > 0.0) {
+        if (phi_b)
+            hipLaunchKernelGGL((dh_boundary_kernel<true, true>), grid, dim3(256), 0, s, g, n, chunk,
+                               used, xyz, q, (float)kappa, work);
+        else
+            hipLaunchKernelGGL((dh_boundary_kernel<true, false>), grid, dim3(256), 0, s, g, n,
+                               chunk, used, xyz, q, (float)kappa, work);
+        LAUNCH_OK();
+        hipLaunchKernelGGL(dh_reduce_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, g, used,
+                           work, sa, phi_a, sb, phi_b);
+    } else {
+        // without salt both media take the same sum
+        hipLaunchKernelGGL((dh_boundary_kernel<false, false>), grid, dim3(256), 0, s, g, n, chunk,
+                           used, xyz, q, 0.0f, work);
+        LAUNCH_OK();
+        hipLaunchKernelGGL(dh_reduce_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, g, used,
+                           work, sa, phi_a, 0.0, nullptr);
+        LAUNCH_OK();
+        if (phi_b)
+            hipLaunchKernelGGL(dh_reduce_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, g, used,
+                               work, sb, phi_b, 0.0, nullptr);
+    }
     LAUNCH_OK();
 }
 

@@ -760,6 +760,30 @@ struct PbJob::State {
     } side[2];
 };
 
+// Debye-Hueckel boundary values of grid g for the job's solvated medium (side
+// 0, with salt) and / or its reference medium (side 1, salt-free, eps_in):
+// one pass over the atoms when both are wanted.
+static void dh_sides(Worker &w, PbJob *j, const GridDesc &g, bool want0, bool want1,
+                     double *phi0, double *phi1)
+{
+    const PbJob::State &S = *j->st;
+    const auto &s0 = S.side[0], &s1 = S.side[1];
+    const int n = j->t->h.n;
+    DBuf<double> dhw(w.pool, dh_work_size(g, n));
+    const double k0 = s0.ionic ? S.kappa : 0.0, k1 = s1.ionic ? S.kappa : 0.0;
+    if (want0 && want1 && k1 == 0.0) {
+        launch_dh_boundary(g, n, j->d_xyz, j->t->q.p, k0, s0.eout, phi0, s1.eout, phi1, dhw.get(),
+                           w.stream);
+        return;
+    }
+    if (want0)
+        launch_dh_boundary(g, n, j->d_xyz, j->t->q.p, k0, s0.eout, phi0, 0.0, nullptr, dhw.get(),
+                           w.stream);
+    if (want1)
+        launch_dh_boundary(g, n, j->d_xyz, j->t->q.p, k1, s1.eout, phi1, 0.0, nullptr, dhw.get(),
+                           w.stream);
+}
+
 void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
 {
     std::vector<PbJob *> act;
@@ -890,9 +914,9 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
         for (auto &sd : S.side) {
             sd.cphi = DBuf<double>(w.pool, S.cg.size());
             launch_fill_f64(sd.cphi.get(), 0.0, S.cg.size(), w.stream);
-            DBuf<double> dhw(w.pool, dh_work_size(S.cg, j->t->h.n));
-            launch_dh_boundary(S.cg, j->t->h.n, j->d_xyz, j->t->q.p, sd.ionic ? S.kappa : 0.0,
-                               sd.eout, sd.cphi.get(), dhw.get(), w.stream);
+        }
+        dh_sides(w, j, S.cg, true, true, S.side[0].cphi.get(), S.side[1].cphi.get());
+        for (auto &sd : S.side) {
             sd.ct.g = S.cg;
             sd.ct.m = &sd.cm;
             sd.ct.rho = S.rho_c.get();
@@ -927,7 +951,9 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
         k = 0;
         for (PbJob *j : act) {
             PbJob::State &S = *j->st;
-            for (auto &sd : S.side) {
+            bool need[2] = {false, false};
+            for (int si = 0; si < 2; ++si) {
+                auto &sd = S.side[si];
                 sd.info->focused = 0;
                 sd.info->coarse_iterations = S.focus ? sd.ct.info.iterations : 0;
                 const bool ok = S.focus && sd.ct.info.converged && std::isfinite(mx[k]) &&
@@ -938,15 +964,13 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
                     launch_interp_boundary(S.fine, S.cg, sd.cphi.get(), sd.phi.get(), w.stream);
                     sd.info->focused = 1;
                 } else {
-                    DBuf<double> dhw(w.pool, dh_work_size(S.fine, j->t->h.n));
-                    launch_dh_boundary(S.fine, j->t->h.n, j->d_xyz, j->t->q.p,
-                                       sd.ionic ? S.kappa : 0.0, sd.eout, sd.phi.get(), dhw.get(),
-                                       w.stream);
+                    need[si] = true;
                 }
                 sd.cm = MapSet();
                 sd.cphi.release();
                 ++k;
             }
+            dh_sides(w, j, S.fine, need[0], need[1], S.side[0].phi.get(), S.side[1].phi.get());
         }
     }
     // fine solves
