@@ -207,6 +207,56 @@ void launch_inside(const GridDesc &g, const double *xyz, const double *radii, do
     LAUNCH_OK();
 }
 
+// Two inside maps with probes pa and pb from one cell-list traversal (the
+// reach of the larger probe covers both); each flag is the single-probe
+// test's own comparison, so both maps equal inside_kernel's.
+__global__ void inside2_kernel(GridDesc g, const double *__restrict__ xyz,
+                               const double *__restrict__ radii, double pa, double pb,
+                               double reach, CellList cl, uint8_t *__restrict__ out_a,
+                               uint8_t *__restrict__ out_b)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.size()) return;
+    int x, y, z;
+    node_of(g, idx, x, y, z);
+    const double px = g.ox + (double)x * g.h, py = g.oy + (double)y * g.h,
+                 pz = g.oz + (double)z * g.h;
+    const int cx0 = max(0, (int)floor((px - reach - cl.ox) * cl.inv));
+    const int cx1 = min(cl.dx - 1, (int)floor((px + reach - cl.ox) * cl.inv));
+    const int cy0 = max(0, (int)floor((py - reach - cl.oy) * cl.inv));
+    const int cy1 = min(cl.dy - 1, (int)floor((py + reach - cl.oy) * cl.inv));
+    const int cz0 = max(0, (int)floor((pz - reach - cl.oz) * cl.inv));
+    const int cz1 = min(cl.dz - 1, (int)floor((pz + reach - cl.oz) * cl.inv));
+    uint8_t ia = 0, ib = 0;
+    for (int cz = cz0; cz <= cz1 && !(ia && ib); ++cz)
+        for (int cy = cy0; cy <= cy1 && !(ia && ib); ++cy)
+            for (int cx = cx0; cx <= cx1 && !(ia && ib); ++cx) {
+                const int c = cx + cl.dx * (cy + cl.dy * cz);
+                for (int k = cl.start[c]; k < cl.start[c + 1]; ++k) {
+                    const int a = cl.atoms[k];
+                    const double dx = px - xyz[3 * a], dy = py - xyz[3 * a + 1],
+                                 dz = pz - xyz[3 * a + 2];
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const double ta = radii[a] + pa, tb = radii[a] + pb;
+                    ia |= d2 < ta * ta ? 1 : 0;
+                    ib |= d2 < tb * tb ? 1 : 0;
+                    if (ia && ib) break;
+                }
+            }
+    out_a[idx] = ia;
+    out_b[idx] = ib;
+}
+
+void launch_inside2(const GridDesc &g, const double *xyz, const double *radii, double pa,
+                    double pb, double max_r, const CellList &cl, uint8_t *out_a, uint8_t *out_b,
+                    hipStream_t s)
+{
+    const int64_t N = g.size();
+    hipLaunchKernelGGL(inside2_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, xyz, radii, pa,
+                       pb, max_r + std::max(pa, pb), cl, out_a, out_b);
+    LAUNCH_OK();
+}
+
 // ---- SES: flood fill of the exterior + bounded squared distance -----------
 // compute_inside_ses, pb_grid.rs:576-708.  Connected exterior = outside-SAS
 // nodes reachable from the box faces through outside-SAS nodes.  Line sweeps

@@ -298,22 +298,27 @@ __global__ __launch_bounds__(kTile) void mm_pair_kernel(const double *__restrict
         const double *rowB = T.ljB + (size_t)T.type[i] * T.n_types;
         const int m = min(kTile, n - j0);
         const int k0 = (ib == jb) ? t + 1 : 0;
+        const double iscnb = 1.0 / T.scnb, iscee = 1.0 / T.scee;
         for (int k = k0; k < m; ++k) {
             const double dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
             const double r2 = dx * dx + dy * dy + dz * dz;
             const bool is14 = (p4m[k >> 6] >> (k & 63)) & 1ull;
             const bool isex = (exm[k >> 6] >> (k & 63)) & 1ull;
             if (!is14 && isex) continue;
-            const double r = sqrt(r2);
-            const double r6 = r2 * r2 * r2;
-            const double r12 = r6 * r6;
+            // one division per pair: A / r^12 - B / r^6 and q q / r from 1 / r
+            // (terms within a few ulp of the divided forms)
+            const double ir = 1.0 / sqrt(r2);
+            const double ir2 = ir * ir;
+            const double ir6 = ir2 * ir2 * ir2;
             const double A = rowA[st[k]], B = rowB[st[k]];
+            const double lj = A * ir6 * ir6 - B * ir6;
+            const double cq = qi * sq[k] * ir;
             if (is14) {
-                dd_add(v14, (A / r12 - B / r6) / T.scnb);
-                dd_add(e14, qi * sq[k] / r / T.scee);
+                dd_add(v14, lj * iscnb);
+                dd_add(e14, cq * iscee);
             } else {
-                dd_add(vdw, A / r12 - B / r6);
-                dd_add(elec, qi * sq[k] / r);
+                dd_add(vdw, lj);
+                dd_add(elec, cq);
             }
         }
     }

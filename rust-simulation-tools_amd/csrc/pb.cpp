@@ -79,7 +79,7 @@ struct MapReq {
 static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
 {
     struct Tmp {
-        DBuf<uint8_t> sas, inside, ext, wa, wb;
+        DBuf<uint8_t> sas, inside, ext, wa, wb, ion;
     };
     std::vector<Tmp> tmp(rq.size());
     std::vector<SesJob> ses;
@@ -128,8 +128,15 @@ static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
             tm.ext = DBuf<uint8_t>(w.pool, N);
             tm.wa = DBuf<uint8_t>(w.pool, N);
             tm.wb = DBuf<uint8_t>(w.pool, N);
-            launch_inside(g, r.d_xyz, r.t->radii.p, probe, r.t->max_radius, r.c->cl,
-                          tm.sas.get(), w.stream);
+            if (r.ionic) {
+                // the ion-accessibility map of the kappa map from the same pass
+                tm.ion = DBuf<uint8_t>(w.pool, N);
+                launch_inside2(g, r.d_xyz, r.t->radii.p, probe, r.p->ion_radius,
+                               r.t->max_radius, r.c->cl, tm.sas.get(), tm.ion.get(), w.stream);
+            } else {
+                launch_inside(g, r.d_xyz, r.t->radii.p, probe, r.t->max_radius, r.c->cl,
+                              tm.sas.get(), w.stream);
+            }
             ses.push_back(SesJob{g, tm.sas.get(), tm.ext.get(), tm.wa.get(), tm.wb.get(),
                                  tm.inside.get(), (int)T});
         }
@@ -157,14 +164,20 @@ static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
         for (int v : ho)
             if (v) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "face interval list overflow");
     }
-    for (MapReq &r : rq) {
+    for (size_t k = 0; k < rq.size(); ++k) {
+        MapReq &r = rq[k];
         MapSet &m = *r.m;
         if (m.uni) continue;
         const int64_t N = r.g.size();
         if (r.ionic) {
-            DBuf<uint8_t> ion(w.pool, N);
-            launch_inside(r.g, r.d_xyz, r.t->radii.p, r.p->ion_radius, r.t->max_radius, r.c->cl,
-                          ion.get(), w.stream);
+            DBuf<uint8_t> ion;
+            if (tmp[k].ion.get()) {
+                ion = std::move(tmp[k].ion);
+            } else {
+                ion = DBuf<uint8_t>(w.pool, N);
+                launch_inside(r.g, r.d_xyz, r.t->radii.p, r.p->ion_radius, r.t->max_radius,
+                              r.c->cl, ion.get(), w.stream);
+            }
             const double kb2 = r.p->solvent_dielectric * r.kappa * r.kappa;
             launch_kappa_map(r.g, ion.get(), kb2, m.k2.get(), m.k2f.get(), w.stream);
         } else {
