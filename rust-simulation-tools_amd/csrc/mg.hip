@@ -1144,6 +1144,7 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
     using s3::kOob;
     __shared__ float ering[4 * ER * ER];
     __shared__ float rpl[2 * FR * FR];
+    __shared__ float4 tabr[2 * 128];  // code byte bits 0-6 -> {a0..a3}, {a4, a5, diag, -}
     const BatchBlock bb = batch_block();
     if (stopped(st + bb.s)) return;
     const MgLevelDev C = Cs[bb.s];
@@ -1169,6 +1170,22 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
     const auto cfr = buf_rsrc(C.f, 4u * csxy * (uint32_t)C.nz);
     const int tid = threadIdx.x;
     const float u = F.eps_u, ihx = F.ihx, ihy = F.ihy, ihz = F.ihz;
+    // faces and diagonal of every code without the fractional bit, from the
+    // residual's own expressions (the same bits); visible after the
+    // prologue's barrier
+    if (hasw && tid < 128) {
+        const uint32_t w = (uint32_t)tid;
+        const float a0 = face_bit<float>(w, 0, F.c_in, F.c_out);
+        const float a1 = face_bit<float>(w, 1, F.c_in, F.c_out);
+        const float a2 = face_bit<float>(w, 2, F.c_in, F.c_out);
+        const float a3 = face_bit<float>(w, 3, F.c_in, F.c_out);
+        const float a4 = face_bit<float>(w, 4, F.c_in, F.c_out);
+        const float a5 = face_bit<float>(w, 5, F.c_in, F.c_out);
+        const float k2 = (w & kCodeKappa) ? F.c_k2 : 0.0f;
+        const float dg = (a0 + a1) * ihx + (a2 + a3) * ihy + (a4 + a5) * ihz + k2;
+        tabr[2 * tid] = make_float4(a0, a1, a2, a3);
+        tabr[2 * tid + 1] = make_float4(a4, a5, dg, 0.0f);
+    }
 
     // correction region value n = tid + 256 j: node (fx0 - 1 + n % ER, fy0 - 1 + n / ER)
     uint32_t eo[NE];
@@ -1286,21 +1303,22 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
                 } else {
                     float exp_ = fa[j][0], exm = fa[j][1], eyp = fa[j][2], eym = fa[j][3];
                     float ezp = fa[j][4], ezm = fa[j][5], k2 = fa[j][6];
-                    if (hasw) {
-                        const uint32_t w = wcur[j];
-                        k2 = (w & kCodeKappa) ? F.c_k2 : 0.0f;
-                        if (!(w & kCodeSpecial)) {
-                            exp_ = face_bit<float>(w, 0, F.c_in, F.c_out);
-                            exm = face_bit<float>(w, 1, F.c_in, F.c_out);
-                            eyp = face_bit<float>(w, 2, F.c_in, F.c_out);
-                            eym = face_bit<float>(w, 3, F.c_in, F.c_out);
-                            ezp = face_bit<float>(w, 4, F.c_in, F.c_out);
-                            ezm = face_bit<float>(w, 5, F.c_in, F.c_out);
-                        }
+                    const uint32_t w = wcur[j];
+                    if (hasw && !(w & kCodeSpecial)) {
+                        const float4 A = tabr[2 * (w & 0x7fu)], B = tabr[2 * (w & 0x7fu) + 1];
+                        exp_ = A.x;
+                        exm = A.y;
+                        eyp = A.z;
+                        eym = A.w;
+                        ezp = B.x;
+                        ezm = B.y;
+                        dg = B.z;
+                    } else {
+                        if (hasw) k2 = (w & kCodeKappa) ? F.c_k2 : 0.0f;
+                        dg = (exp_ + exm) * ihx + (eyp + eym) * ihy + (ezp + ezm) * ihz + k2;
                     }
                     nb = fmaf(exp_, xp, exm * xm) * ihx + fmaf(eyp, yp, eym * ym) * ihy +
                          fmaf(ezp, zp, ezm * zm) * ihz;
-                    dg = (exp_ + exm) * ihx + (eyp + eym) * ihy + (ezp + ezm) * ihz + k2;
                 }
                 r = fcur[j] + nb - dg * c;
             }
