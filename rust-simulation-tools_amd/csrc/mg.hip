@@ -1129,6 +1129,11 @@ constexpr int FR = 2 * CT + 1;       // fine residual region edge: 33
 constexpr int ER = FR + 2;           // correction region edge: 35
 constexpr int NE = (ER * ER + 255) / 256;  // correction values per thread: 5
 constexpr int NR = (FR * FR + 255) / 256;  // residual nodes per thread: 5
+// residual plane in LDS: row stride RS, even columns at 0 .. 16 and odd ones at
+// 17 .. 32, so a coarse column's reads are consecutive across lanes and rows
+// two apart land 16 banks apart
+constexpr int RS = 40, ODD = (FR + 1) / 2;
+__host__ __device__ constexpr int rpos(int ry, int rx) { return ry * RS + ((rx & 1) ? ODD : 0) + (rx >> 1); }
 }  // namespace rr
 
 int mg_rr_blocks(int cnx, int cny, int cnz, int czc)
@@ -1136,14 +1141,14 @@ int mg_rr_blocks(int cnx, int cny, int cnz, int czc)
     return ceil_div(cnx, rr::CT) * ceil_div(cny, rr::CT) * ceil_div(cnz, czc);
 }
 
-__global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__restrict__ Fs,
+__global__ __launch_bounds__(256, 4) void resrestrict_kernel(const MgLevelDev *__restrict__ Fs,
                                                           const MgLevelDev *__restrict__ Cs,
                                                           const MgStatus *__restrict__ st)
 {
     using namespace rr;
     using s3::kOob;
     __shared__ float ering[4 * ER * ER];
-    __shared__ float rpl[2 * FR * FR];
+    __shared__ float rpl[2 * FR * RS];
     __shared__ float4 tabr[2 * 128];  // code byte bits 0-6 -> {a0..a3}, {a4, a5, diag, -}
     const BatchBlock bb = batch_block();
     if (stopped(st + bb.s)) return;
@@ -1199,7 +1204,7 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
     // residual node n = tid + 256 j: node (fx0 + n % FR, fy0 + n / FR); bit j of
     // rin: interior column (in-plane neighbours exist), the rest give residual 0
     uint32_t ro[NR];
-    int rli[NR];
+    int rli[NR], rpo[NR];
     uint32_t rin = 0u;
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
@@ -1208,6 +1213,7 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
         const bool in = n < FR * FR && x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2;
         ro[j] = in ? (uint32_t)(x + nx * y) : kOob;
         rli[j] = (ry + 1) * ER + (rx + 1);
+        rpo[j] = rpos(ry, rx);
         if (in) rin |= 1u << j;
     }
     auto eslot = [&](int z) { return ering + ((z + 8) & 3) * (ER * ER); };
@@ -1246,9 +1252,28 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
     load_fw(zs);
     __syncthreads();
 
-    // coarse column of this thread and its fine 3 x 3 neighbourhood in rpl
+    // coarse column of this thread and its fine 3 x 3 neighbourhood in rpl:
+    // restrict_node's weights (2 on the axis, 1 off it) with 0 where it skips
+    // a node outside the grid.  Such nodes hold residual 0 and the chain
+    // starts at +0 and never reaches -0, so fma(0, 0, sum) leaves its bits
+    // unchanged; the weight sums are small integers, exact in any order.
     const int ccx = cx0 + (tid & (CT - 1)), ccy = cy0 + (tid / CT);
     const bool cown = ccx < C.nx && ccy < C.ny;
+    float wxy[9];
+    float wsxy = 0.0f;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int x = 2 * ccx + dx, y = 2 * ccy + dy;
+            const bool in = x >= 0 && x < nx && y >= 0 && y < ny;
+            const float w = in ? (dx == 0 ? 2.0f : 1.0f) * (dy == 0 ? 2.0f : 1.0f) : 0.0f;
+            wxy[3 * (dy + 1) + (dx + 1)] = w;
+            wsxy += w;
+        }
+    // fine (x, y) = (2 ccx + dx, 2 ccy + dy) sits at region column 2 (ccx - cx0)
+    // + dx + 1 and row 2 (ccy - cy0) + dy + 1
+    const int cb = rpos(2 * (ccy - cy0) + 1, 2 * (ccx - cx0) + 1) - ODD;  // even column of dx = -1
     float sum = 0.0f, ws = 0.0f;
     for (int z = zs; z <= ze; ++z) {
         store_e(z + 2);
@@ -1275,7 +1300,7 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
             }
         }
         const float *em = eslot(z - 1), *ec = eslot(z), *ep = eslot(z + 1);
-        float *rp = rpl + (z & 1) * (FR * FR);
+        float *rp = rpl + (z & 1) * (FR * RS);
         float fcur[NR];
         uint32_t wcur[NR];
 #pragma unroll
@@ -1322,7 +1347,7 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
                 }
                 r = fcur[j] + nb - dg * c;
             }
-            rp[n] = r;
+            rp[rpo[j]] = r;
         }
         __syncthreads();
         // restriction: plane z is dz = 0 of coarse plane z / 2 (z even), or
@@ -1330,20 +1355,16 @@ __global__ __launch_bounds__(256) void resrestrict_kernel(const MgLevelDev *__re
         // (z + 1) / 2 (z odd)
         auto terms = [&](int dz) {
             if (z < 0 || z >= nz) return;
+            const float wz = dz == 0 ? 2.0f : 1.0f;
+            const float *q = rp + cb;
 #pragma unroll
             for (int dy = -1; dy <= 1; ++dy) {
-                const int y = 2 * ccy + dy;
-                if (y < 0 || y >= ny) continue;
-#pragma unroll
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int x = 2 * ccx + dx;
-                    if (x < 0 || x >= nx) continue;
-                    const float w = (dx == 0 ? 2.0f : 1.0f) * (dy == 0 ? 2.0f : 1.0f) *
-                                    (dz == 0 ? 2.0f : 1.0f);
-                    sum = fmaf(w, rp[(y - fy0) * FR + (x - fx0)], sum);
-                    ws += w;
-                }
+                const int ro_ = dy * RS;
+                sum = fmaf(wxy[3 * (dy + 1) + 0] * wz, q[ro_], sum);            // dx = -1
+                sum = fmaf(wxy[3 * (dy + 1) + 1] * wz, q[ro_ + ODD], sum);      // dx = 0
+                sum = fmaf(wxy[3 * (dy + 1) + 2] * wz, q[ro_ + 1], sum);        // dx = +1
             }
+            ws += wsxy * wz;
         };
         if (cown) {
             if ((z & 1) == 0) {
