@@ -65,3 +65,24 @@ def random_cluster(n, seed, box=12.0, min_sep=1.3, n_types=3):
     top = make_topology(q, radii, screen=rng.uniform(0.7, 0.95, n), types=types,
                         n_types=n_types, lj_a=A, lj_b=B, nb_parm_index=idx.reshape(-1))
     return top, x
+
+
+def hollow_shell(open_cap):
+    """150 atoms on a 7 A sphere (Fibonacci points, spacing ~2.3 A, radius
+    1.7 A: a 1.4 A probe cannot pass between them) around a +1 e atom of
+    radius 1.2 A at the origin; open_cap removes the atoms within 45 degrees
+    of +z, a ~5 A hole the probe passes.  Returns (topology, xyz in A)."""
+    n, R = 150, 7.0
+    k = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * k / n
+    r = np.sqrt(1.0 - z * z)
+    phi = np.pi * (3.0 - np.sqrt(5.0)) * k
+    pts = R * np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+    if open_cap:
+        pts = pts[pts[:, 2] / R < np.cos(np.pi / 4)]
+    q = np.where(np.arange(len(pts)) % 2 == 0, 0.3, -0.3)
+    q -= q.mean()  # a neutral shell
+    x = np.vstack([[0.0, 0.0, 0.0], pts])
+    charges = np.concatenate([[1.0], q])
+    radii = np.concatenate([[1.2], np.full(len(pts), 1.7)])
+    return make_topology(charges, radii), x

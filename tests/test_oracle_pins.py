@@ -296,6 +296,32 @@ def test_ses_brute_force(oracle):
     assert np.array_equal(inside.astype(bool), dmin > th)
 
 
+def test_ses_shell_cavity(oracle):
+    """The connected exterior around an enclosed cavity: inside a closed
+    hollow shell the annulus between the centre atom's and the shell's SAS
+    is outside the SAS but not reachable (SES interior); with a cap removed
+    the same points are exterior."""
+    from _topo import hollow_shell
+    L = oracle.lib()
+    u8 = C.POINTER(C.c_uint8)
+    for open_cap in (False, True):
+        top, x = hollow_shell(open_cap)
+        n = len(x)
+        g = oracle.OrcGrid()
+        L.orc_auto_grid(n, P(x), 0.5, 10.0, C.byref(g))
+        nx, ny, nz = g.n[0], g.n[1], g.n[2]
+        N = nx * ny * nz
+        inside, sas, ext = (np.zeros(N, np.uint8) for _ in range(3))
+        L.orc_inside_ses(C.byref(g), n, P(x), P(top.radii), 1.4, inside.ctypes.data_as(u8),
+                         sas.ctypes.data_as(u8), ext.ctypes.data_as(u8))
+        for pt in ([0.0, 0.0, 3.3], [3.3, 0.0, 0.0], [0.0, 0.0, -3.3], [0.0, -3.3, 0.0]):
+            i = [int(round((pt[d] - g.o[d]) / g.h[d])) for d in range(3)]
+            k = i[0] + nx * (i[1] + ny * i[2])
+            assert sas[k] == 0
+            assert ext[k] == (1 if open_cap else 0)
+            assert inside[k] == (0 if open_cap else 1)
+
+
 def test_face_fraction_sampling(oracle):
     L = oracle.lib()
     top, x = random_cluster(20, 5, box=6.0)
