@@ -771,8 +771,9 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         return z >= 1 && z <= nz - 2 && z >= z0 - (K - k) && z <= z1 - 1 + (K - k);
     };
 
-    using Fast = std::integral_constant<bool, false>;
-    using Slow = std::integral_constant<bool, true>;
+    using Fast = std::integral_constant<int, 0>;
+    using Slow = std::integral_constant<int, 1>;
+    using NoSpec = std::integral_constant<int, 2>;
     // overflow flags of the two step parities live in the unused fourth
     // word of table entries 0 and 1
     int *const wg_ovf0 = reinterpret_cast<int *>(&tab[1]) + 3;
@@ -794,9 +795,21 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         // half-sweep itself.
         uint32_t spm = 0;
         int spx = 0, total = 0;
+        bool nospec = false;  // no code word of the wave's six planes has the fractional bit
         float sa[6];
         uint32_t sreq = 0xffffffffu;
         auto specials_issue = [&](int t) {
+            uint32_t anyw = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) anyw |= Q[k].w;
+            nospec = __ballot((anyw & 0x80808080u) != 0u) == 0;
+            if (nospec) {
+                spm = 0;
+                spx = 0;
+                total = 0;
+                sreq = 0xffffffffu;
+                return;
+            }
             const int c = par ^ (t & 1);
             uint32_t zmask = 0;  // half-sweeps with a plane to update (uniform)
 #pragma unroll
@@ -871,8 +884,11 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                 ibm[m] = 4u * (uint32_t)(li0 + PR * RW * m + c - RW);
             }
         };
-        auto stage = [&](int k, int t, auto slow_tag) {
-            constexpr bool SLOW = decltype(slow_tag)::value;
+        // MODE 0: fast path; 1: SLOW (a wave overflowed its slots); 2: the
+        // wave has no fractional-face node in its six planes (table only)
+        auto stage = [&](int k, int t, auto mode_tag) {
+            constexpr int MODE = decltype(mode_tag)::value;
+            constexpr bool SLOW = MODE == 1, NOSPEC = MODE == 2;
             const int z = t - k + 1;
             if (!zok(k, z)) return;
             const int c = par ^ (t & 1);
@@ -899,8 +915,8 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                 } else {
                     const uint32_t w = (q.w >> (8 * jb)) & 0xffu;
                     const int j = 2 * (k - 1) + m;
-                    const int sl = spx + __popc(spm & ((1u << j) - 1u));
-                    const bool sp = w >= kCodeSpecial;
+                    const int sl = NOSPEC ? 0 : spx + __popc(spm & ((1u << j) - 1u));
+                    const bool sp = !NOSPEC && w >= kCodeSpecial;
                     // face entry: the wave's slot or the table row (both
                     // computed, then selected: no branch); a workgroup on the
                     // fast path has every slot below SLOTS
@@ -968,10 +984,17 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
             if (total > SLOTS && lane == 0) *wg_ovf(t & 1) = 1;
             if (tid == 0) *wg_ovf((t + 1) & 1) = 0;
             __syncthreads();
+            // the branches are uniform per wave and each passes six barriers
             if (__builtin_amdgcn_readfirstlane(*wg_ovf(t & 1))) {
 #pragma unroll
                 for (int k = 1; k <= K; ++k) {
                     stage(k, t, Slow{});
+                    __syncthreads();
+                }
+            } else if (nospec) {
+#pragma unroll
+                for (int k = 1; k <= K; ++k) {
+                    stage(k, t, NoSpec{});
                     __syncthreads();
                 }
             } else {
