@@ -903,14 +903,25 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                 if (rdm[m] < (uint32_t)k) continue;
                 const float *r = reinterpret_cast<const float *>(ringb + sc + ibm[m]);
                 const float fv = c ? q.f[m][1] : q.f[m][0];
-                const float emy = r[0], emx = r[RW - 1], old = r[RW], exx = r[RW + 1];
-                const float eyy = r[2 * RW];
-                const float ezz = *reinterpret_cast<const float *>(ringb + sp + ibm[m]);
-                const float emz = *reinterpret_cast<const float *>(ringb + sm + ibm[m]);
+                // pre-smoothing starts from zero: at its first half-sweep a
+                // node and all six neighbours are still +0, so the neighbour
+                // sum is +0 (the same bits as summing them)
+                constexpr bool ZERO = FIRST;
+                const bool zs = ZERO && k == 1;
+                const float old = r[RW];
+                float emy = 0.0f, emx = 0.0f, exx = 0.0f, eyy = 0.0f, ezz = 0.0f, emz = 0.0f;
+                if (!zs) {
+                    emy = r[0];
+                    emx = r[RW - 1];
+                    exx = r[RW + 1];
+                    eyy = r[2 * RW];
+                    ezz = *reinterpret_cast<const float *>(ringb + sp + ibm[m]);
+                    emz = *reinterpret_cast<const float *>(ringb + sm + ibm[m]);
+                }
                 float *pc_i = const_cast<float *>(r) + RW;
                 float gs;
                 if (UNI) {
-                    const float sum = ((exx + emx) + (eyy + emy)) + (ezz + emz);
+                    const float sum = zs ? 0.0f : ((exx + emx) + (eyy + emy)) + (ezz + emz);
                     gs = fmaf(sum, uih, fv) * udg_inv;
                 } else {
                     const uint32_t w = (q.w >> (8 * jb)) & 0xffu;
@@ -937,7 +948,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                             fmaf(((A.x + A.y) + (A.z + A.w)) + (B.x + B.y), ih, k2);
                         B.z = __builtin_amdgcn_rcpf(dg);
                     }
-                    const float sum = fmaf(A.x, exx, fmaf(A.y, emx, fmaf(A.z, eyy,
+                    const float sum = zs ? 0.0f : fmaf(A.x, exx, fmaf(A.y, emx, fmaf(A.z, eyy,
                                       fmaf(A.w, emy, fmaf(B.x, ezz, B.y * emz)))));
                     gs = fmaf(sum, ih, fv) * B.z;
                 }
