@@ -564,15 +564,16 @@ static_assert(PR % 2 == 0, "pair rows of one parity");
 
 int smooth3_tiles(int nx, int ny) { return ceil_div(nx, s3::TX) * ceil_div(ny, s3::TY); }
 
-// Planes per workgroup: about 64 (a chunk also streams 2 K = 12 planes it does
-// not write), shorter when a launch of n_solves such levels would otherwise
-// have fewer than 2048 workgroups (8 per CU), but at least 8.  (Whole columns
-// do fewer steps but measured no faster: the halo reads of neighbouring tiles
-// then meet less often in L2.)
+// Planes per workgroup: about 96 (a chunk also streams 2 K - 2 = 10 planes it
+// does not write), shorter when a launch of n_solves such levels would
+// otherwise have fewer than 2048 workgroups (8 per CU), but at least 8.
+// Measured on the 161^3 batch: chunks of ~96 planes (two per column) beat
+// 64, 40 and 32; whole columns were no faster (the halo reads of
+// neighbouring tiles then meet less often in L2).
 int smooth3_zc(int nx, int ny, int nz, int n_solves)
 {
     const int64_t tiles = (int64_t)smooth3_tiles(nx, ny) * std::max(n_solves, 1);
-    int nch = std::max(1, (nz + 32) / 64);
+    int nch = std::max(1, (nz + 48) / 96);
     nch = std::max<int>(nch, (int)((2048 + tiles - 1) / tiles));
     nch = std::min(nch, std::max(1, nz / 8));
     return ceil_div(nz, nch);
