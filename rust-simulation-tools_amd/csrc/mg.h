@@ -45,6 +45,10 @@ struct MgLevelDev {
     float *f;          // right-hand side
     float *res;        // residual scratch
     int zc;            // smoother: planes per workgroup (z-chunk length)
+    // finest level only (else null): the FP64 potential the phase smoother's
+    // post-smoothing pass adds its final correction to, phi += e, in place of
+    // storing e (the outer step then reads phi alone)
+    double *phi;
     // index helpers: (nx-1)/2 slots per row, ny-2 interior rows, nx, nx*ny
     FastDiv fd_half, fd_row, fd_nx, fd_nxy;
 };

@@ -592,6 +592,16 @@ __device__ __forceinline__ uint32_t bload8(__amdgpu_buffer_rsrc_t r, uint32_t vo
 {
     return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, (int)vo, (int)so, 0);
 }
+__device__ __forceinline__ double bload64(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ void bstore64(__amdgpu_buffer_rsrc_t r, double v, uint32_t vo,
+                                         uint32_t so)
+{
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)vo, (int)so, 0);
+}
 
 template <bool FIRST, bool PROLONG>
 __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__restrict__ Ls,
@@ -727,14 +737,44 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
 #pragma unroll
         for (int j = 0; j < PF; ++j) p[tid + NT * j] = pf[j];
     };
+    // finest level, post-smoothing: phi += e in FP64 instead of storing e (the
+    // outer step's own addition).  At the write-back point the final values of
+    // plane z are read from the ring and the loads of phi are issued; the
+    // addition and store follow the step's six half-sweeps (fold_finish), so
+    // the loads are not waited for (the ring slot itself is refilled at the
+    // next step's start, behind no barrier, so it is not read again).  Lanes
+    // outside the core are skipped by predicate (8 N may exceed kOob).
+    const bool fold = !FIRST && L.phi != nullptr;
+    const auto phr = buf_rsrc(fold ? L.phi : nullptr, fold ? 8u * N : 0u);
+    double phv[WB];
+    float fev[WB];
+    int fold_z = -1;
     auto write_back = [&](int z) {
         if (z < z0 || z >= z1) return;
-        const float *p = slot(z);
         const uint32_t zo = 4u * sxy * (uint32_t)z;
+        if (fold) {
+            const float *p = slot(z);
+#pragma unroll
+            for (int j = 0; j < WB; ++j) {
+                fev[j] = p[wli[j]];
+                phv[j] = wvo[j] != kOob ? bload64(phr, 2u * wvo[j], 2u * zo) : 0.0;
+            }
+            fold_z = z;
+            return;
+        }
+        const float *p = slot(z);
 #pragma unroll
         for (int j = 0; j < WB; ++j)
             __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(p[wli[j]]), dst, (int)wvo[j],
                                                   (int)zo, 0);
+    };
+    auto fold_finish = [&]() {
+        if (!fold || fold_z < 0) return;
+        const uint32_t zo = 4u * sxy * (uint32_t)fold_z;
+#pragma unroll
+        for (int j = 0; j < WB; ++j)
+            if (wvo[j] != kOob) bstore64(phr, phv[j] + (double)fev[j], 2u * wvo[j], 2u * zo);
+        fold_z = -1;
     };
     // coefficients of a plane: f of both columns of both pairs, code bytes packed
     struct Coef {
@@ -987,6 +1027,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                     stage(k, t, Fast{});
                     __syncthreads();
                 }
+                fold_finish();
                 continue;
             }
             specials_store();
@@ -1016,6 +1057,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                     __syncthreads();
                 }
             }
+            fold_finish();
         }
     };
     if (uni)

@@ -314,6 +314,7 @@ struct TaskWork {
     int tail_start = 0;
     int maxv = 1;
     DBuf<double> buf1, bc, partials;
+    bool fold = false;  // the phase pass adds the finest correction to phi
     MgOuterDev O[2];        // per cycle parity
     MgOuterDev O0;          // cycle-0 residual
 };
@@ -326,11 +327,12 @@ int64_t lv_work(const GridDesc &g)
     return (int64_t)(g.ny - 2) * (g.nz - 2) * ((g.nx - 1) / 2);
 }
 
-MgLevelDev level_with_e(const MgLevelDev &d, float *e, float *e2)
+MgLevelDev level_with_e(const MgLevelDev &d, float *e, float *e2, double *phi)
 {
     MgLevelDev r = d;
     r.e = e;
     r.e2 = e2;
+    r.phi = phi;
     return r;
 }
 
@@ -381,6 +383,9 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     // z-streaming residual + restriction and prolongation unless MMPBSA_STREAM=0
     const char *st_env = std::getenv("MMPBSA_STREAM");
     const bool stream = !(st_env && std::strcmp(st_env, "0") == 0);
+    // the phase pass adds the finest level's final correction to phi itself
+    // (one FP64 potential per solve, no ping-pong copy; per solve: TaskWork::fold)
+    const bool fold = phase_pass;
 
     std::vector<TaskWork> W(S);
     int maxT = 0;
@@ -466,7 +471,10 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             }
         }
         const int64_t N0 = lv_nodes(t->g);
-        tw.buf1 = DBuf<double>(w.pool, N0);
+        // only where the phase pass smooths the finest level (not a grid the
+        // tail kernel solves whole)
+        tw.fold = fold && tw.tail_start > 0;
+        if (!tw.fold) tw.buf1 = DBuf<double>(w.pool, N0);
         tw.bc = DBuf<double>(w.pool, N0);
         HIP_OK(hipMemcpyAsync(tw.bc.get(), t->phi, sizeof(double) * N0, hipMemcpyDeviceToDevice,
                               w.stream));
@@ -482,9 +490,15 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         tw.O0.e = nullptr;
         for (int p = 0; p < 2; ++p) {
             tw.O[p] = O;
-            tw.O[p].phi_in = bufs[1 - p];
-            tw.O[p].phi_out = bufs[p];
-            tw.O[p].e = E[p];
+            if (tw.fold) {
+                // the post-smoothing pass has already added e to phi in place
+                tw.O[p].phi_in = tw.O[p].phi_out = t->phi;
+                tw.O[p].e = nullptr;
+            } else {
+                tw.O[p].phi_in = bufs[1 - p];
+                tw.O[p].phi_out = bufs[p];
+                tw.O[p].e = E[p];
+            }
         }
     }
 
@@ -513,7 +527,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             // cycle's outer step while the other buffer serves as e2
             float *E[2] = {tw.L[0].e.get(), tw.L[0].e2.get()};
             auto lev = [&](int l) {
-                return l == 0 ? level_with_e(tw.L[0].d, E[p], E[1 - p]) : tw.L[l].d;
+                return l == 0 ? level_with_e(tw.L[0].d, E[p], E[1 - p], tw.fold ? tw.t->phi : nullptr)
+                              : tw.L[l].d;
             };
             for (int l = 0; l < tw.tail_start; ++l) {
                 H[p].F[(size_t)l * S + s] = lev(l);
@@ -735,7 +750,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         MgTask *t = tw.t;
         const int fin = hs[s].cycles;
         const int64_t N0 = lv_nodes(t->g);
-        if (fin & 1)
+        if (!tw.fold && (fin & 1))
             HIP_OK(hipMemcpyAsync(t->phi, tw.buf1.get(), sizeof(double) * N0,
                                   hipMemcpyDeviceToDevice, w.stream));
         if (hs[s].state == 2) {
