@@ -722,9 +722,11 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     if (graph) (void)hipGraphDestroy(graph);
     if (profiling && !evs.empty()) {
         // algorithmic bytes of one pass over a level-0 solve, per node: the
-        // correction read (unless starting from zero) and written, f, the
-        // code byte (non-uniform maps) and, when prolongating, the coarse
-        // correction (one eighth of a float per fine node)
+        // correction read (unless starting from zero) and written (the
+        // post-smoothing pass that adds it to phi reads and writes the FP64
+        // potential instead), f, the code byte (non-uniform maps) and, when
+        // prolongating, the coarse correction (one eighth of a float per fine
+        // node)
         double ms_tot = 0.0, bytes = 0.0;
         for (const Ev &ev : evs) {
             float ms = 0.0f;
@@ -733,7 +735,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             for (int s = 0; s < S; ++s) {
                 if (W[s].tail_start == 0 || ev.cycle > hs[s].cycles) continue;
                 const double n = (double)lv_nodes(W[s].t->g);
-                bytes += n * ((ev.reads_e ? 4.0 : 0.0) + 4.0 + 4.0 + (W[s].t->m->uni ? 0.0 : 1.0) +
+                const double out = (ev.reads_e && W[s].fold) ? 16.0 : 4.0;
+                bytes += n * ((ev.reads_e ? 4.0 : 0.0) + out + 4.0 + (W[s].t->m->uni ? 0.0 : 1.0) +
                               (ev.prolong ? 0.5 : 0.0));
             }
             (void)hipEventDestroy(ev.a);
