@@ -422,6 +422,110 @@ __global__ void flood_batch_kernel(const FloodJob *__restrict__ jobs, int axis)
     if (ch) *J.changed = 1;
 }
 
+// The same sweeps with the chunks of a line side by side: G lanes per line
+// (G a power of two >= the line's 32-node chunks), all chunk loads at once.
+// A chunk maps a carry-in to a carry-out (filled from 0 and from 1), and a
+// log-step scan of those maps across the group gives each chunk the carry
+// the node-by-node sweep brings to it; the backward pass then does the same
+// on the forward result.  Lanes past the line's end carry nothing.
+__device__ __forceinline__ uint32_t carry_compose(uint32_t later, uint32_t earlier)
+{
+    // maps as 2 bits: bit 0 = image of 0, bit 1 = image of 1
+    const uint32_t a = (later >> (earlier & 1u)) & 1u;
+    const uint32_t b = (later >> ((earlier >> 1) & 1u)) & 1u;
+    return a | (b << 1);
+}
+
+__global__ void flood_par_kernel(const FloodJob *__restrict__ jobs, int axis, int G)
+{
+    const FloodJob J = jobs[blockIdx.y];
+    if (*J.active == 0) return;
+    const GridDesc g = J.g;
+    const int64_t tg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = tg / G;  // line
+    const int c = (int)(tg - t * G);  // chunk
+    int64_t base = 0, stride = 1, nlines;
+    int len;
+    if (axis == 0) {
+        nlines = (int64_t)g.ny * g.nz;
+        base = t * g.nx;
+        stride = 1;
+        len = g.nx;
+    } else if (axis == 1) {
+        nlines = (int64_t)g.nx * g.nz;
+        const int64_t x = t % g.nx, z = t / g.nx;
+        base = x + (int64_t)g.nx * g.ny * z;
+        stride = g.nx;
+        len = g.ny;
+    } else {
+        nlines = (int64_t)g.nx * g.ny;
+        base = t;
+        stride = (int64_t)g.nx * g.ny;
+        len = g.nz;
+    }
+    // whole groups stay together for the shuffles (a line past the end is idle)
+    const bool line_ok = t < nlines;
+    const int k0 = 32 * c, n = line_ok ? max(0, min(32, len - k0)) : 0;
+    const auto sas = to_global<const uint8_t>(J.sas);
+    const auto ext = to_global<uint8_t>(J.ext);
+    uint32_t open = 0u, E = 0u;
+    if (n > 0) {
+        uint8_t sv[32], ev[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const int64_t i = base + (int64_t)(k0 + min(k, n - 1)) * stride;
+            sv[k] = sas[i];
+            ev[k] = ext[i];
+        }
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            open |= (uint32_t)(sv[k] == 0) << k;
+            E |= (uint32_t)(ev[k] != 0) << k;
+        }
+        const uint32_t live = n == 32 ? ~0u : (1u << n) - 1u;
+        open &= live;
+        E &= live;
+    }
+    const uint32_t top = n > 0 ? 1u << (n - 1) : 0u;
+    // forward: carry maps, inclusive scan from chunk 0, carry-in = the scan up
+    // to the previous chunk applied to 0
+    uint32_t fm = 0u;
+    if (n > 0) {
+        const uint32_t F0 = fill_up(E & open, open), F1 = fill_up((E | 1u) & open, open);
+        fm = ((F0 & top) ? 1u : 0u) | ((F1 & top) ? 2u : 0u);
+    }
+    uint32_t h = fm;
+    for (int d = 1; d < G; d <<= 1) {
+        const uint32_t hp = __shfl_up(h, d, G);
+        if (c >= d) h = carry_compose(h, hp);
+    }
+    const uint32_t hprev = __shfl_up(h, 1, G);
+    const uint32_t cin = c > 0 ? (hprev & 1u) : 0u;
+    const uint32_t Ff = n > 0 ? fill_up((E | cin) & open, open) : 0u;
+    // backward on the forward result: carry maps from the line's end
+    uint32_t bm = 0u;
+    if (n > 0) {
+        const uint32_t B0 = fill_down(Ff & open, open), B1 = fill_down((Ff | top) & open, open);
+        bm = (B0 & 1u) | ((B1 & 1u) << 1);
+    }
+    uint32_t hb = bm;
+    for (int d = 1; d < G; d <<= 1) {
+        const uint32_t hn = __shfl_down(hb, d, G);
+        if (c + d < G) hb = carry_compose(hb, hn);
+    }
+    const uint32_t hnext = __shfl_down(hb, 1, G);
+    const uint32_t cinb = (c + 1 < G) ? (hnext & 1u) : 0u;
+    const uint32_t Fb = n > 0 ? fill_down((Ff | (cinb ? top : 0u)) & open, open) : 0u;
+    uint32_t nw = Fb & ~E;
+    const bool ch = nw != 0u;
+    while (nw) {
+        const int k = __ffs(nw) - 1;
+        ext[base + (int64_t)(k0 + k) * stride] = 1;
+        nw &= nw - 1u;
+    }
+    if (ch) *J.changed = 1;
+}
+
 // End of a flood round of one map: active = changed, changed = 0.
 __global__ void flood_round_end_kernel(const FloodJob *__restrict__ job)
 {
@@ -461,8 +565,17 @@ void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d
         for (int round = 0; round < 100000; round += kRounds) {
             for (int r = 0; r < kRounds; ++r) {
                 for (int ax = 0; ax < 3; ++ax) {
-                    hipLaunchKernelGGL(flood_batch_kernel, dim3(ceil_div(ln[ax], 128), 1),
-                                       dim3(128), 0, s, d_table + k, ax);
+                    const int len = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
+                    const int nch = (len + 31) / 32;
+                    if (nch <= 64) {
+                        int G = 1;
+                        while (G < nch) G <<= 1;
+                        hipLaunchKernelGGL(flood_par_kernel, dim3(ceil_div(ln[ax] * G, 256), 1),
+                                           dim3(256), 0, s, d_table + k, ax, G);
+                    } else {
+                        hipLaunchKernelGGL(flood_batch_kernel, dim3(ceil_div(ln[ax], 128), 1),
+                                           dim3(128), 0, s, d_table + k, ax);
+                    }
                     LAUNCH_OK();
                 }
                 hipLaunchKernelGGL(flood_round_end_kernel, dim3(1), dim3(64), 0, s, d_table + k);
