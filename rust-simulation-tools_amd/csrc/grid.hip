@@ -162,98 +162,64 @@ void launch_spread(const GridDesc &g, int n, const double *xyz, const double *q,
 }
 
 // ---- inside tests (AtomSpatialHash::point_inside, pb_grid.rs:400-442) -------
-__global__ void inside_kernel(GridDesc g, const double *__restrict__ xyz,
-                              const double *__restrict__ radii, double probe, double reach,
-                              CellList cl, uint8_t *__restrict__ inside)
+// inside[i] = exists atom a with |p_i - x_a|^2 < (r_a + probe)^2, built by
+// atom: a workgroup per atom marks the nodes of its inflated sphere.  Each
+// node-atom test is the per-node search's own expression (p_i from the node
+// index, d^2 summed x, y, z, threshold (r + probe)^2), and marking is an
+// idempotent store of 1, so the map does not depend on the order.  Most
+// nodes lie farther than r + probe from every atom; by atom, no work is spent
+// on them (a search by node visits every cell within reach).
+__global__ __launch_bounds__(256) void inside_scatter_kernel(GridDesc g, const double *__restrict__ xyz,
+                                                             const double *__restrict__ radii,
+                                                             double pa, double pb,
+                                                             uint8_t *__restrict__ out_a,
+                                                             uint8_t *__restrict__ out_b)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.size()) return;
-    int x, y, z;
-    node_of(g, idx, x, y, z);
-    const double px = g.ox + (double)x * g.h, py = g.oy + (double)y * g.h,
-                 pz = g.oz + (double)z * g.h;
-    const int cx0 = max(0, (int)floor((px - reach - cl.ox) * cl.inv));
-    const int cx1 = min(cl.dx - 1, (int)floor((px + reach - cl.ox) * cl.inv));
-    const int cy0 = max(0, (int)floor((py - reach - cl.oy) * cl.inv));
-    const int cy1 = min(cl.dy - 1, (int)floor((py + reach - cl.oy) * cl.inv));
-    const int cz0 = max(0, (int)floor((pz - reach - cl.oz) * cl.inv));
-    const int cz1 = min(cl.dz - 1, (int)floor((pz + reach - cl.oz) * cl.inv));
-    uint8_t in = 0;
-    for (int cz = cz0; cz <= cz1 && !in; ++cz)
-        for (int cy = cy0; cy <= cy1 && !in; ++cy)
-            for (int cx = cx0; cx <= cx1 && !in; ++cx) {
-                const int c = cx + cl.dx * (cy + cl.dy * cz);
-                for (int k = cl.start[c]; k < cl.start[c + 1]; ++k) {
-                    const int a = cl.atoms[k];
-                    const double dx = px - xyz[3 * a], dy = py - xyz[3 * a + 1],
-                                 dz = pz - xyz[3 * a + 2];
-                    const double d2 = dx * dx + dy * dy + dz * dz;
-                    const double th = radii[a] + probe;
-                    if (d2 < th * th) {
-                        in = 1;
-                        break;
-                    }
-                }
-            }
-    inside[idx] = in;
+    const int a = blockIdx.x;
+    const double ax = xyz[3 * a], ay = xyz[3 * a + 1], az = xyz[3 * a + 2];
+    const double ta = radii[a] + pa, tb = radii[a] + pb;
+    const double R = out_b ? fmax(ta, tb) : ta;
+    // node ranges covering the sphere with a node to spare on each side (the
+    // exact test decides), clamped before the integer conversion
+    auto lo = [&](double v, int n) { return (int)fmin(fmax(floor(v) - 1.0, 0.0), (double)n); };
+    auto hi = [&](double v, int n) { return (int)fmax(fmin(ceil(v) + 1.0, (double)(n - 1)), -1.0); };
+    const int x0 = lo((ax - R - g.ox) / g.h, g.nx), x1 = hi((ax + R - g.ox) / g.h, g.nx);
+    const int y0 = lo((ay - R - g.oy) / g.h, g.ny), y1 = hi((ay + R - g.oy) / g.h, g.ny);
+    const int z0 = lo((az - R - g.oz) / g.h, g.nz), z1 = hi((az + R - g.oz) / g.h, g.nz);
+    if (x0 > x1 || y0 > y1 || z0 > z1) return;
+    const int wx = x1 - x0 + 1, wy = y1 - y0 + 1;
+    const int cnt = wx * wy * (z1 - z0 + 1);
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const int dz_ = k / (wx * wy), r_ = k - dz_ * (wx * wy);
+        const int dy_ = r_ / wx, x = x0 + (r_ - dy_ * wx), y = y0 + dy_, z = z0 + dz_;
+        const double px = g.ox + (double)x * g.h, py = g.oy + (double)y * g.h,
+                     pz = g.oz + (double)z * g.h;
+        const double dx = px - ax, dy = py - ay, dz = pz - az;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const int64_t idx = (int64_t)x + (int64_t)g.nx * (y + (int64_t)g.ny * z);
+        if (d2 < ta * ta) out_a[idx] = 1;
+        if (out_b && d2 < tb * tb) out_b[idx] = 1;
+    }
 }
 
-void launch_inside(const GridDesc &g, const double *xyz, const double *radii, double probe,
-                   double max_r, const CellList &cl, uint8_t *inside, hipStream_t s)
+void launch_inside(const GridDesc &g, int n, const double *xyz, const double *radii, double probe,
+                   uint8_t *inside, hipStream_t s)
 {
-    const int64_t N = g.size();
-    hipLaunchKernelGGL(inside_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, xyz, radii,
-                       probe, max_r + probe, cl, inside);
+    launch_fill_u8(inside, 0, g.size(), s);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(inside_scatter_kernel, dim3(n), dim3(256), 0, s, g, xyz, radii, probe,
+                       probe, inside, (uint8_t *)nullptr);
     LAUNCH_OK();
 }
 
-// Two inside maps with probes pa and pb from one cell-list traversal (the
-// reach of the larger probe covers both); each flag is the single-probe
-// test's own comparison, so both maps equal inside_kernel's.
-__global__ void inside2_kernel(GridDesc g, const double *__restrict__ xyz,
-                               const double *__restrict__ radii, double pa, double pb,
-                               double reach, CellList cl, uint8_t *__restrict__ out_a,
-                               uint8_t *__restrict__ out_b)
+void launch_inside2(const GridDesc &g, int n, const double *xyz, const double *radii, double pa,
+                    double pb, uint8_t *out_a, uint8_t *out_b, hipStream_t s)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.size()) return;
-    int x, y, z;
-    node_of(g, idx, x, y, z);
-    const double px = g.ox + (double)x * g.h, py = g.oy + (double)y * g.h,
-                 pz = g.oz + (double)z * g.h;
-    const int cx0 = max(0, (int)floor((px - reach - cl.ox) * cl.inv));
-    const int cx1 = min(cl.dx - 1, (int)floor((px + reach - cl.ox) * cl.inv));
-    const int cy0 = max(0, (int)floor((py - reach - cl.oy) * cl.inv));
-    const int cy1 = min(cl.dy - 1, (int)floor((py + reach - cl.oy) * cl.inv));
-    const int cz0 = max(0, (int)floor((pz - reach - cl.oz) * cl.inv));
-    const int cz1 = min(cl.dz - 1, (int)floor((pz + reach - cl.oz) * cl.inv));
-    uint8_t ia = 0, ib = 0;
-    for (int cz = cz0; cz <= cz1 && !(ia && ib); ++cz)
-        for (int cy = cy0; cy <= cy1 && !(ia && ib); ++cy)
-            for (int cx = cx0; cx <= cx1 && !(ia && ib); ++cx) {
-                const int c = cx + cl.dx * (cy + cl.dy * cz);
-                for (int k = cl.start[c]; k < cl.start[c + 1]; ++k) {
-                    const int a = cl.atoms[k];
-                    const double dx = px - xyz[3 * a], dy = py - xyz[3 * a + 1],
-                                 dz = pz - xyz[3 * a + 2];
-                    const double d2 = dx * dx + dy * dy + dz * dz;
-                    const double ta = radii[a] + pa, tb = radii[a] + pb;
-                    ia |= d2 < ta * ta ? 1 : 0;
-                    ib |= d2 < tb * tb ? 1 : 0;
-                    if (ia && ib) break;
-                }
-            }
-    out_a[idx] = ia;
-    out_b[idx] = ib;
-}
-
-void launch_inside2(const GridDesc &g, const double *xyz, const double *radii, double pa,
-                    double pb, double max_r, const CellList &cl, uint8_t *out_a, uint8_t *out_b,
-                    hipStream_t s)
-{
-    const int64_t N = g.size();
-    hipLaunchKernelGGL(inside2_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, g, xyz, radii, pa,
-                       pb, max_r + std::max(pa, pb), cl, out_a, out_b);
+    launch_fill_u8(out_a, 0, g.size(), s);
+    launch_fill_u8(out_b, 0, g.size(), s);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(inside_scatter_kernel, dim3(n), dim3(256), 0, s, g, xyz, radii, pa, pb,
+                       out_a, out_b);
     LAUNCH_OK();
 }
 

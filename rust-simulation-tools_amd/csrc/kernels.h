@@ -95,13 +95,12 @@ void launch_cell_build(const double *xyz, int n, double ox, double oy, double oz
                        hipStream_t s);
 void launch_spread(const GridDesc &g, int n, const double *xyz, const double *q,
                    unsigned long long *acc, double *rho, hipStream_t s);
-// inside[i] = exists atom a with |p_i - x_a|^2 < (r_a + probe)^2
-void launch_inside(const GridDesc &g, const double *xyz, const double *radii, double probe,
-                   double max_r, const CellList &cl, uint8_t *inside, hipStream_t s);
+// inside[i] = exists atom a (of n) with |p_i - x_a|^2 < (r_a + probe)^2
+void launch_inside(const GridDesc &g, int n, const double *xyz, const double *radii, double probe,
+                   uint8_t *inside, hipStream_t s);
 // the maps of probes pa and pb from one pass (equal to two launch_inside)
-void launch_inside2(const GridDesc &g, const double *xyz, const double *radii, double pa,
-                    double pb, double max_r, const CellList &cl, uint8_t *out_a, uint8_t *out_b,
-                    hipStream_t s);
+void launch_inside2(const GridDesc &g, int n, const double *xyz, const double *radii, double pa,
+                    double pb, uint8_t *out_a, uint8_t *out_b, hipStream_t s);
 // SES classification from SAS maps (flood fill of the exterior + bounded
 // distance transform) for many maps: each map's flood rounds (one host check
 // per four rounds), then each map's distance test.

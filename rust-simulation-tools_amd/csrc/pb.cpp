@@ -117,8 +117,7 @@ static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
         tm.inside = DBuf<uint8_t>(w.pool, N);
         const double probe = r.p->probe_radius;
         if (probe <= 0.0) {
-            launch_inside(g, r.d_xyz, r.t->radii.p, 0.0, r.t->max_radius, r.c->cl,
-                          tm.inside.get(), w.stream);
+            launch_inside(g, r.t->h.n, r.d_xyz, r.t->radii.p, 0.0, tm.inside.get(), w.stream);
         } else {
             const double th2 = (probe / g.h) * (probe / g.h);
             const double T = std::floor(th2);
@@ -131,11 +130,10 @@ static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
             if (r.ionic) {
                 // the ion-accessibility map of the kappa map from the same pass
                 tm.ion = DBuf<uint8_t>(w.pool, N);
-                launch_inside2(g, r.d_xyz, r.t->radii.p, probe, r.p->ion_radius,
-                               r.t->max_radius, r.c->cl, tm.sas.get(), tm.ion.get(), w.stream);
+                launch_inside2(g, r.t->h.n, r.d_xyz, r.t->radii.p, probe, r.p->ion_radius,
+                               tm.sas.get(), tm.ion.get(), w.stream);
             } else {
-                launch_inside(g, r.d_xyz, r.t->radii.p, probe, r.t->max_radius, r.c->cl,
-                              tm.sas.get(), w.stream);
+                launch_inside(g, r.t->h.n, r.d_xyz, r.t->radii.p, probe, tm.sas.get(), w.stream);
             }
             ses.push_back(SesJob{g, tm.sas.get(), tm.ext.get(), tm.wa.get(), tm.wb.get(),
                                  tm.inside.get(), (int)T});
@@ -175,8 +173,8 @@ static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
                 ion = std::move(tmp[k].ion);
             } else {
                 ion = DBuf<uint8_t>(w.pool, N);
-                launch_inside(r.g, r.d_xyz, r.t->radii.p, r.p->ion_radius, r.t->max_radius,
-                              r.c->cl, ion.get(), w.stream);
+                launch_inside(r.g, r.t->h.n, r.d_xyz, r.t->radii.p, r.p->ion_radius, ion.get(),
+                              w.stream);
             }
             const double kb2 = r.p->solvent_dielectric * r.kappa * r.kappa;
             launch_kappa_map(r.g, ion.get(), kb2, m.k2.get(), m.k2f.get(), w.stream);
