@@ -98,10 +98,10 @@ void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first
                int max_blocks, const MgStatus *st, hipStream_t s);
 // Temporally blocked smoother: one pass = three red-black sweeps (a whole
 // smoothing phase of the V-cycle), same buffers and flags as mg_smooth, with
-// smooth3_tiles * ceil(nz / L.zc) workgroups per solve, L.zc from
-// smooth3_zc.  The result equals three mg_smooth passes bit for bit.  Needs
-// 4 * nodes < 2^31.
+// smooth3_blocks workgroups per solve, L.zc from smooth3_zc.  The result
+// equals three mg_smooth passes bit for bit.  Needs 4 * nodes < 2^31.
 int smooth3_tiles(int nx, int ny);
+int smooth3_blocks(int nx, int ny, int nz, int zc);
 int smooth3_zc(int nx, int ny, int nz, int n_solves);
 void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
                 int max_blocks, const MgStatus *st, hipStream_t s);

@@ -550,7 +550,8 @@ constexpr int RING = 8;                // correction planes t-6 .. t+1
 constexpr int PF = RN / NT;            // plane values per thread: 4
 constexpr int PP = RN / 2 / NT;        // column pairs per thread: 2
 constexpr int PR = NT / 16;            // region rows between a thread's pairs: 16
-constexpr int WB = (TX * TY + NT - 1) / NT;  // core values written back per thread: 2
+constexpr int WX = TX + 2, WY = TY + 2;      // written nodes: core plus grid-face nodes
+constexpr int WB = (WX * WY + NT - 1) / NT;  // values written back per thread: 2
 constexpr int SLOTS = 32;              // fractional-face slots per wave and step
 constexpr int TAB = 128;               // code-table entries
 constexpr uint32_t kOob = 0x80000000u; // offset outside every array
@@ -560,9 +561,21 @@ static_assert(PP * NT * 2 == RN && PF * NT == RN && RW == 32 && NT % 64 == 0 && 
               "region layout");
 // both column pairs of a thread share the red-black parity of their rows
 static_assert(PR % 2 == 0, "pair rows of one parity");
+// Tiles and z-chunks cover the interior nodes 1 .. n - 2 of each axis (on
+// 161^3, 8 x 8 tiles instead of 9 x 9); a tile on a face of the grid also
+// writes back the boundary nodes beside its core.
+__host__ __device__ constexpr int inner(int n) { return n > 2 ? n - 2 : 1; }
 }  // namespace s3
 
-int smooth3_tiles(int nx, int ny) { return ceil_div(nx, s3::TX) * ceil_div(ny, s3::TY); }
+int smooth3_tiles(int nx, int ny)
+{
+    return ceil_div(s3::inner(nx), s3::TX) * ceil_div(s3::inner(ny), s3::TY);
+}
+
+int smooth3_blocks(int nx, int ny, int nz, int zc)
+{
+    return smooth3_tiles(nx, ny) * ceil_div(s3::inner(nz), zc);
+}
 
 // Planes per workgroup: about 96 (a chunk also streams 2 K - 2 = 10 planes it
 // does not write), shorter when a launch of n_solves such levels would
@@ -622,14 +635,22 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     const MgLevelDev L = Ls[s];
     if (L.nx == 0) return;
     const int nx = L.nx, ny = L.ny, nz = L.nz, zc = L.zc;
-    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
-    const int nch = (nz + zc - 1) / zc;
+    const int ix = inner(nx), iy = inner(ny), iz = inner(nz);
+    const int ntx = (ix + TX - 1) / TX, nty = (iy + TY - 1) / TY;
+    const int nch = (iz + zc - 1) / zc;
     if (b >= ntx * nty * nch) return;
     const int tile = b % (ntx * nty), ch = b / (ntx * nty);
-    const int cx = (nx + ntx - 1) / ntx, cy = (ny + nty - 1) / nty;
-    const int x0 = (tile % ntx) * cx, x1 = min(nx, x0 + cx);
-    const int y0 = (tile / ntx) * cy, y1 = min(ny, y0 + cy);
-    const int z0 = ch * zc, z1 = min(nz, z0 + zc);
+    const int cx = (ix + ntx - 1) / ntx, cy = (iy + nty - 1) / nty;
+    const int x0 = 1 + (tile % ntx) * cx, x1 = min(nx - 1, x0 + cx);
+    const int y0 = 1 + (tile / ntx) * cy, y1 = min(ny - 1, y0 + cy);
+    const int z0 = 1 + ch * zc, z1 = min(nz - 1, z0 + zc);
+    // nodes written back: the core plus the boundary nodes beside it.  Those
+    // are never updated but carry the source's values, which need not be zero:
+    // below a level of even size the prolongation extrapolates onto the
+    // boundary (prolongate_add, pb_solver.rs:628-681).
+    const int wx0 = x0 == 1 ? 0 : x0, wx1 = x1 == nx - 1 ? nx : x1;
+    const int wy0 = y0 == 1 ? 0 : y0, wy1 = y1 == ny - 1 ? ny : y1;
+    const int wz0 = z0 == 1 ? 0 : z0, wz1 = z1 == nz - 1 ? nz : z1;
     const int xl = x0 - H, yl = y0 - H;
     const uint32_t sxy = (uint32_t)nx * (uint32_t)ny;
     const uint32_t N = sxy * (uint32_t)nz;  // mg_smooth3 checks 4 N < 2^31
@@ -701,15 +722,15 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
             for (int m = 0; m < PP; ++m)
                 if (((rdpk >> (8 * (2 * m + c))) & 0xffu) >= (uint32_t)k)
                     spread[c] |= 1u << (2 * (k - 1) + m);
-    // write-back: core value n = tid + NT j at core row n / TX
+    // write-back: value n = tid + NT j at row n / WX of the written rectangle
     uint32_t wvo[WB];
     int wli[WB];
 #pragma unroll
     for (int j = 0; j < WB; ++j) {
-        const int n = tid + NT * j, r = n / TX, c = n - r * TX;
-        const bool ok = r < y1 - y0 && c < x1 - x0;
-        wvo[j] = ok ? 4u * (uint32_t)((x0 + c) + nx * (y0 + r)) : kOob;
-        wli[j] = ok ? (r + H) * RW + (c + H) : 0;
+        const int n = tid + NT * j, r = n / WX, c = n - r * WX;
+        const bool ok = r < wy1 - wy0 && c < wx1 - wx0;
+        wvo[j] = ok ? 4u * (uint32_t)((wx0 + c) + nx * (wy0 + r)) : kOob;
+        wli[j] = ok ? (wy0 + r - yl) * RW + (wx0 + c - xl) : 0;
     }
 
     auto slot = [&](int z) { return ring + ((z + 2 * RING) & (RING - 1)) * RN; };
@@ -750,7 +771,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     float fev[WB];
     int fold_z = -1;
     auto write_back = [&](int z) {
-        if (z < z0 || z >= z1) return;
+        if (z < wz0 || z >= wz1) return;
         const uint32_t zo = 4u * sxy * (uint32_t)z;
         if (fold) {
             const float *p = slot(z);
@@ -1057,6 +1078,12 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                     __syncthreads();
                 }
             }
+            fold_finish();
+        }
+        // the grid's last plane (never updated, still in the ring) follows
+        // the last step
+        if (wz1 > z1) {
+            write_back(z1);
             fold_finish();
         }
     };

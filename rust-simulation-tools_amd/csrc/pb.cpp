@@ -533,8 +533,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 H[p].C[(size_t)l * S + s] = lev(l + 1);
                 const GridDesc &g = tw.L[l].g;
                 sblocks[l] = std::max(sblocks[l],
-                                      phase_pass ? smooth3_tiles(g.nx, g.ny) *
-                                                       ceil_div(g.nz, tw.L[l].d.zc)
+                                      phase_pass ? smooth3_blocks(g.nx, g.ny, g.nz, tw.L[l].d.zc)
                                                  : smooth_tiles(g.nx, g.ny) *
                                                        ceil_div(g.nz, smooth_zc(g.nz)));
                 if (phase_pass && 4 * lv_nodes(g) >= ((int64_t)1 << 31))
