@@ -111,8 +111,9 @@ int mg_stream_blocks(int nx, int ny, int nz, int zc);
 void mg_prolong_e2s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
                     const MgStatus *st, hipStream_t s);
 // Residual of e2 and its restriction into C.f in one pass (no residual
-// array), equal bit for bit to mg_residual32 then mg_restrict;
-// mg_rr_blocks(coarse dims, C.zc) workgroups per solve.
+// array), equal bit for bit to mg_residual32 then mg_restrict at the interior
+// coarse nodes (the boundary's right-hand sides are never read, nor written
+// here); mg_rr_blocks(coarse dims, C.zc) workgroups per solve.
 int mg_rr_blocks(int cnx, int cny, int cnz, int czc);
 void mg_residual_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
                           const MgStatus *st, hipStream_t s);

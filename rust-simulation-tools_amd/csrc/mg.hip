@@ -1219,14 +1219,15 @@ __global__ __launch_bounds__(256) void prolong_e2s_kernel(const MgLevelDev *__re
 
 // Residual and restriction in one pass (residual32_kernel then
 // restrict_kernel without the residual array).  A workgroup owns a 16 x 16
-// tile of coarse columns and a chunk of C.zc coarse planes, and streams the
+// tile of interior coarse columns and a chunk of C.zc interior coarse planes
+// (a boundary node's right-hand side is never read), and streams the
 // fine planes 2 cz0 - 1 .. 2 cz1 - 1: the pre-smoothed correction e2 goes
 // through a 4-plane LDS ring (35 x 35 nodes, loaded two planes ahead), each
 // fine plane's residual on the 33 x 33 nodes the tile restricts from is
 // written to LDS (double-buffered, one barrier per plane), and every coarse
 // column extends its fma chain in restrict_node's order and skips, so both
-// the residuals and the coarse right-hand sides equal the two kernels' bit
-// for bit.
+// the residuals and the interior coarse right-hand sides equal the two
+// kernels' bit for bit.
 namespace rr {
 constexpr int CT = 16;               // coarse tile edge
 constexpr int FR = 2 * CT + 1;       // fine residual region edge: 33
@@ -1240,9 +1241,12 @@ constexpr int RS = 40, ODD = (FR + 1) / 2;
 __host__ __device__ constexpr int rpos(int ry, int rx) { return ry * RS + ((rx & 1) ? ODD : 0) + (rx >> 1); }
 }  // namespace rr
 
+// Tiles and chunks cover the interior coarse nodes only: nothing reads a
+// right-hand side on a level's boundary.
 int mg_rr_blocks(int cnx, int cny, int cnz, int czc)
 {
-    return ceil_div(cnx, rr::CT) * ceil_div(cny, rr::CT) * ceil_div(cnz, czc);
+    return ceil_div(s3::inner(cnx), rr::CT) * ceil_div(s3::inner(cny), rr::CT) *
+           ceil_div(s3::inner(cnz), czc);
 }
 
 __global__ __launch_bounds__(256, 4) void resrestrict_kernel(const MgLevelDev *__restrict__ Fs,
@@ -1259,12 +1263,12 @@ __global__ __launch_bounds__(256, 4) void resrestrict_kernel(const MgLevelDev *_
     const MgLevelDev C = Cs[bb.s];
     if (C.nx == 0) return;
     const MgLevelDev F = Fs[bb.s];
-    const int ntx = (C.nx + CT - 1) / CT, nty = (C.ny + CT - 1) / CT;
-    const int nch = (C.nz + C.zc - 1) / C.zc;
+    const int ntx = (s3::inner(C.nx) + CT - 1) / CT, nty = (s3::inner(C.ny) + CT - 1) / CT;
+    const int nch = (s3::inner(C.nz) + C.zc - 1) / C.zc;
     if (bb.b >= ntx * nty * nch) return;
     const int tile = bb.b % (ntx * nty), ch = bb.b / (ntx * nty);
-    const int cx0 = (tile % ntx) * CT, cy0 = (tile / ntx) * CT;
-    const int cz0 = ch * C.zc, cz1 = min(C.nz, cz0 + C.zc);
+    const int cx0 = 1 + (tile % ntx) * CT, cy0 = 1 + (tile / ntx) * CT;
+    const int cz0 = 1 + ch * C.zc, cz1 = min(C.nz - 1, cz0 + C.zc);
     const int nx = F.nx, ny = F.ny, nz = F.nz;
     const int fx0 = 2 * cx0 - 1, fy0 = 2 * cy0 - 1;  // residual region origin
     const uint32_t sxy = (uint32_t)nx * (uint32_t)ny, N = sxy * (uint32_t)nz;
@@ -1362,7 +1366,7 @@ __global__ __launch_bounds__(256, 4) void resrestrict_kernel(const MgLevelDev *_
     // starts at +0 and never reaches -0, so fma(0, 0, sum) leaves its bits
     // unchanged; the weight sums are small integers, exact in any order.
     const int ccx = cx0 + (tid & (CT - 1)), ccy = cy0 + (tid / CT);
-    const bool cown = ccx < C.nx && ccy < C.ny;
+    const bool cown = ccx < C.nx - 1 && ccy < C.ny - 1;
     float wxy[9];
     float wsxy = 0.0f;
 #pragma unroll
