@@ -943,7 +943,10 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
 #pragma unroll
             for (int m = 0; m < PP; ++m) {
                 rdm[m] = (rdpk >> (8 * (2 * m + c))) & 0xffu;
-                ibm[m] = 4u * (uint32_t)(li0 + PR * RW * m + c - RW);
+                // (clamped for region rows 0 and RH - 1, whose lanes never
+                // update, so that their rows -1 .. +1 stay inside the slot)
+                ibm[m] = 4u * (uint32_t)min(max(li0 + PR * RW * m + c - RW, 0),
+                                            (RH - 3) * RW + RW - 1);
             }
         };
         // MODE 0: fast path; 1: SLOW (a wave overflowed its slots); 2: the
@@ -959,10 +962,16 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
             const uint32_t sm = 4u * (uint32_t)(slot(z - 1) - ring) + 4u * RW;
             const uint32_t sp = 4u * (uint32_t)(slot(z + 1) - ring) + 4u * RW;
             const Coef &q = Q[k - 1];
+            // both updates of the thread are computed unconditionally (reads
+            // of in-region addresses only) and only the stores are predicated,
+            // so their loads and fma chains overlap
+            float nv[PP];
+            float *dp[PP];
+            bool act[PP];
 #pragma unroll
             for (int m = 0; m < PP; ++m) {
                 const int jb = 2 * m + c;
-                if (rdm[m] < (uint32_t)k) continue;
+                act[m] = rdm[m] >= (uint32_t)k;
                 const float *r = reinterpret_cast<const float *>(ringb + sc + ibm[m]);
                 const float fv = c ? q.f[m][1] : q.f[m][0];
                 // pre-smoothing starts from zero: at its first half-sweep a
@@ -989,7 +998,9 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                     const uint32_t w = (q.w >> (8 * jb)) & 0xffu;
                     const int j = 2 * (k - 1) + m;
                     const int sl = NOSPEC ? 0 : spx + __popc(spm & ((1u << j) - 1u));
-                    const bool sp = !NOSPEC && w >= kCodeSpecial;
+                    // a lane that does not update reads its table row: its slot
+                    // number is not one of the wave's
+                    const bool sp = !NOSPEC && act[m] && w >= kCodeSpecial;
                     // face entry: the wave's slot or the table row (both
                     // computed, then selected: no branch); a workgroup on the
                     // fast path has every slot below SLOTS
@@ -1014,8 +1025,12 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                                       fmaf(A.w, emy, fmaf(B.x, ezz, B.y * emz)))));
                     gs = fmaf(sum, ih, fv) * B.z;
                 }
-                *pc_i = fmaf(omega, gs - old, old);
+                dp[m] = pc_i;
+                nv[m] = fmaf(omega, gs - old, old);
             }
+#pragma unroll
+            for (int m = 0; m < PP; ++m)
+                if (act[m]) *dp[m] = nv[m];
         };
 
         // prologue: planes t0 - 1 and t0 in the ring, coefficients of t0 and
