@@ -367,6 +367,65 @@ def test_gb_hct_integral_quadrature(oracle):
             assert abs(psi - ser) <= 1e-12 * max(ser, 1e-300) + 1e-16
 
 
+@pytest.mark.parametrize("model", [0, 1, 2])
+@pytest.mark.parametrize("cutoff", [9999.0, 4.5])
+def test_gb_radii_multi_atom_quadrature(oracle, model, cutoff):
+    """Born radii of a 12-atom cluster (close and overlapping pairs, one atom
+    engulfed by a neighbour's scaled sphere) against an independent evaluation:
+    each pair's descreening integral 1/2 int (1 - cos theta(t)) / t^2 dt from
+    rho_i to r + s_j by quadrature, summed over the pairs within the cutoff,
+    then mapped by HCT (1/R = 1/rho - psi) or by the published OBC map
+    (Onufriev, Bashford and Case, Proteins 55:383 (2004): 1/R = 1/rho -
+    tanh(a P - b P^2 + c P^3) / radius with P = psi rho; OBC I (0.8, 0,
+    2.909125), OBC II (1.0, 0.8, 4.85)), capped at rgbmax.  For an engulfed
+    atom (rho_i < s_j - r) the reference adds the inner shell's integral
+    1/rho_i - 1/(s_j - r) twice, once more than the integral itself
+    (gb_energy.rs:207-211), and so does this evaluation."""
+    rng = np.random.default_rng(7)
+    n = 12
+    x = rng.uniform(-3.5, 3.5, (n, 3))
+    radii = rng.choice([1.2, 1.5, 1.7, 1.8, 2.0], n)
+    screen = rng.uniform(0.7, 0.9, n)
+    radii[0], radii[1], screen[1] = 1.0, 3.0, 0.9
+    x[0] = x[1] + np.array([0.4, 0.1, 0.0])
+    top = make_topology(np.zeros(n), radii, screen=screen)
+    gp = oracle.gb_params(model=model, cutoff=cutoff)
+    _, R = oracle.gb_energy(top, x, gp)
+    rho = radii - gp.offset
+    psi = np.zeros(n)
+    engulfed = 0
+    for i in range(n):
+        for j in range(n):
+            r = float(np.linalg.norm(x[i] - x[j]))
+            if i == j or r > cutoff:
+                continue
+            s = screen[j] * rho[j]
+            if rho[i] >= r + s:
+                continue
+
+            def f(t):
+                c = np.clip((t * t + r * r - s * s) / (2 * t * r), -1, 1)
+                return 0.5 * (1 - c) / (t * t)
+            kink = abs(r - s)
+            pts = [kink] if rho[i] < kink < r + s else None
+            val, _ = integrate.quad(f, rho[i], r + s, points=pts, epsabs=1e-15, epsrel=1e-13,
+                                    limit=200)
+            if rho[i] < s - r:
+                val += 1.0 / rho[i] - 1.0 / (s - r)
+                engulfed += 1
+            psi[i] += val
+    assert engulfed == 1
+    if model == 0:
+        inv = 1.0 / rho - psi
+    else:
+        a, b, c = (0.8, 0.0, 2.909125) if model == 1 else (1.0, 0.8, 4.85)
+        P = psi * rho
+        inv = 1.0 / rho - np.tanh(a * P - b * P * P + c * P ** 3) / radii
+    ref = np.where(inv > 0, 1.0 / np.where(inv > 0, inv, 1.0), gp.rgbmax)
+    ref = np.minimum(ref, gp.rgbmax)
+    assert np.all(np.abs(R - ref) <= 1e-10 * ref), (model, cutoff, np.max(np.abs(R - ref) / ref))
+
+
 def test_gb_energy_limits(oracle):
     top, x = random_cluster(30, 2, box=10.0)
     # eps_in == eps_out and no salt: zero exactly
