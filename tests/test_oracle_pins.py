@@ -125,6 +125,32 @@ def test_grid_known_answers(oracle):
     assert abs(e - 1.0) < 1e-10
 
 
+def test_auto_grid_invariants(oracle):
+    """auto_grid (pb_grid.rs:188-215) on point clouds of real extent, checked
+    through what any correct placement must satisfy on each axis: the grid is
+    centred on the bounding box, covers it with the buffer on both sides, and
+    has the fewest odd node count that does ((n - 3) h < extent + 2 buffer
+    <= (n - 1) h)."""
+    L = oracle.lib()
+    rng = np.random.default_rng(11)
+    for trial in range(40):
+        n = int(rng.integers(1, 60))
+        x = rng.normal(0, rng.uniform(1, 15), (n, 3)) + rng.uniform(-50, 50, 3)
+        h = float(rng.choice([0.37, 0.5, 0.6, 0.7, 1.0]))
+        buf = float(rng.choice([0.0, 2.5, 5.0, 10.0]))
+        g = oracle.OrcGrid()
+        L.orc_auto_grid(n, P(np.ascontiguousarray(x)), h, buf, C.byref(g))
+        mn, mx = x.min(0), x.max(0)
+        for k in range(3):
+            nk, o = g.n[k], g.o[k]
+            assert g.h[k] == h and nk % 2 == 1
+            span = (nk - 1) * h
+            ext = mx[k] - mn[k] + 2 * buf
+            assert abs((o + 0.5 * span) - 0.5 * (mn[k] + mx[k])) <= 1e-12 * (1 + abs(mn[k]) + abs(mx[k]))
+            assert o <= mn[k] - buf + 1e-9 and o + span >= mx[k] + buf - 1e-9
+            assert (nk - 3) * h < ext <= span + 1e-9
+
+
 def test_restrict_prolong_constants(oracle):
     # pb_solver.rs:1260-1283 (and exactness for odd dimensions)
     L = oracle.lib()
