@@ -541,6 +541,37 @@ def test_decomposition_sums(oracle):
     assert abs(sa - fe["delta_sa"]) < 1e-9
 
 
+def test_decomposition_gb_zero_shares(oracle):
+    """The reference shares dGB out among residues in proportion to
+    |q^2 (1/R_complex - 1/R_alone)| summed over their atoms
+    (decomposition.rs:309-351).  Independently of the proportions, a residue
+    of neutral atoms and a residue farther than the GB cutoff from every
+    partner atom (its atoms' Born radii cannot change) must get exactly zero,
+    and the two interface residues then carry all of dGB.  Sharing pair terms,
+    or spreading dGB by charge or atom count, would give them a share."""
+    rng = np.random.default_rng(5)
+    centres = np.array([[0.0, 0, 0], [0, 4.5, 0], [22.0, 0, 0], [26.0, 0, 0], [0, 0, 4.5]])
+    x = np.concatenate([c + rng.normal(0, 0.9, (4, 3)) for c in centres])
+    q = rng.normal(0, 0.5, 20)
+    q[4:8] = 0.0  # residue 1: neutral
+    radii = rng.choice([1.2, 1.5, 1.7], 20)
+    top = make_topology(q, radii, residue_pointers=(0, 4, 8, 12, 16))
+    gb = oracle.gb_params(cutoff=8.0)
+    lig = x[16:20]
+    far = np.linalg.norm(x[8:16, None, :] - lig[None, :, :], axis=-1).min()
+    assert far > 8.0 + 1.0
+    rr, ll = oracle.decompose(top, x, [0, 1, 2, 3], [4], gb=gb)
+    pol = {d["residue_index"]: d["polar"] for d in rr + ll}
+    from rust_simulation_tools.topology import extract_subtopology
+    ra, la = top.build_selection([0, 1, 2, 3]), top.build_selection([4])
+    dgb = oracle.gb_energy(top, x, gb)[0] - \
+        oracle.gb_energy(extract_subtopology(top, ra), x[ra], gb)[0] - \
+        oracle.gb_energy(extract_subtopology(top, la), x[la], gb)[0]
+    assert pol[1] == 0.0 and pol[2] == 0.0 and pol[3] == 0.0
+    assert pol[0] != 0.0 and pol[4] != 0.0 and abs(dgb) > 1e-3
+    assert abs(pol[0] + pol[4] - dgb) <= 1e-12 * max(1.0, abs(dgb))
+
+
 def _pb_small(**kw):
     base = dict(grid_spacing=1.0, grid_buffer=4.0, solute_dielectric=4.0,
                 solvent_dielectric=80.0, salt_concentration=0.15, tolerance=1e-6)
