@@ -1205,30 +1205,69 @@ __global__ __launch_bounds__(256) void prolong_e2s_kernel(const MgLevelDev *__re
     const int x0 = min(q.x >> 1, max(C.nx - 2, 0)), y0 = min(q.y >> 1, max(C.ny - 2, 0));
     const float dx = 0.5f * q.x - x0, dy = 0.5f * q.y - y0;
     const uint32_t co = 4u * ((uint32_t)x0 + cs * (uint32_t)y0);
-    // coarse plane cz interpolated in x and y (c0 / c1 of prolong_value)
-    auto cplane = [&](int cz) {
-        const uint32_t zo = 4u * csz * (uint32_t)cz;
-        const float c00 = bload(cr, co, zo) * (1.0f - dx) + bload(cr, co + 4u, zo) * dx;
-        const float c10 = bload(cr, co + 4u * cs, zo) * (1.0f - dx) + bload(cr, co + 4u * cs + 4u, zo) * dx;
+    // coarse plane cz interpolated in x and y (c0 / c1 of prolong_value), from
+    // its four corner values
+    auto craw = [&](int cz, float (&v)[4]) {
+        const bool ok = cz < C.nz;
+        const uint32_t zo = ok ? 4u * csz * (uint32_t)cz : 0u;
+        v[0] = bload(cr, ok ? co : s3::kOob, zo);
+        v[1] = bload(cr, ok ? co + 4u : s3::kOob, zo);
+        v[2] = bload(cr, ok ? co + 4u * cs : s3::kOob, zo);
+        v[3] = bload(cr, ok ? co + 4u * cs + 4u : s3::kOob, zo);
+    };
+    auto cmix = [&](const float (&v)[4]) {
+        const float c00 = v[0] * (1.0f - dx) + v[1] * dx;
+        const float c10 = v[2] * (1.0f - dx) + v[3] * dx;
         return c00 * (1.0f - dy) + c10 * dy;
     };
-    const uint32_t po = 4u * (uint32_t)q.p;
-    int zc_lo = -1;
+    auto cplane = [&](int cz) {
+        float v[4];
+        craw(cz, v);
+        return cmix(v);
+    };
+    const uint32_t po = 4u * (uint32_t)q.p, zs = 4u * sxy;
+    // A thread owns its column, so its loads and stores never alias: each fine
+    // plane is loaded D planes ahead of its store (D loads in flight instead
+    // of one), and the corners of coarse plane z0c + 1 + CR when c_hi moves to
+    // plane z0c + 1, so every wait is for a load issued about D fine planes
+    // earlier (vector memory loads retire in order: a wait for a recent load
+    // would also wait for the prefetches before it).  The arithmetic is
+    // unchanged.
+    constexpr int D = 8, CR = D / 2;
+    float fb[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) fb[i] = q.z0 + i < q.z1 ? bload(er, po, zs * (uint32_t)(q.z0 + i)) : 0.0f;
+    int zc_lo = -2;  // the first plane computes both coarse planes directly
     float c_lo = 0.0f, c_hi = 0.0f;
-    for (int z = q.z0; z < q.z1; ++z) {
-        const int z0c = min(z >> 1, max(C.nz - 2, 0));
-        if (z0c != zc_lo) {
-            if (z0c == zc_lo + 1) {
-                c_lo = c_hi;
-            } else {
-                c_lo = cplane(z0c);
+    float nv[CR][4];  // corners of coarse planes zc_lo + 2 .. zc_lo + 1 + CR
+    for (int zb = q.z0; zb < q.z1; zb += D) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const int z = zb + i;
+            if (z >= q.z1) break;
+            const float cur = fb[i];
+            fb[i] = z + D < q.z1 ? bload(er, po, zs * (uint32_t)(z + D)) : 0.0f;
+            const int z0c = min(z >> 1, max(C.nz - 2, 0));
+            if (z0c != zc_lo) {
+                if (z0c == zc_lo + 1) {
+                    c_lo = c_hi;
+                    c_hi = cmix(nv[0]);  // plane z0c + 1
+#pragma unroll
+                    for (int r = 0; r + 1 < CR; ++r)
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) nv[r][k] = nv[r + 1][k];
+                    craw(z0c + 1 + CR, nv[CR - 1]);
+                } else {
+                    c_lo = cplane(z0c);
+                    c_hi = cplane(z0c + 1);
+#pragma unroll
+                    for (int r = 0; r < CR; ++r) craw(z0c + 2 + r, nv[r]);
+                }
+                zc_lo = z0c;
             }
-            c_hi = cplane(z0c + 1);
-            zc_lo = z0c;
+            const float dz = 0.5f * z - z0c;
+            bstore(er, cur + (c_lo * (1.0f - dz) + c_hi * dz), po, zs * (uint32_t)z);
         }
-        const float dz = 0.5f * z - z0c;
-        const uint32_t zo = 4u * sxy * (uint32_t)z;
-        bstore(er, bload(er, po, zo) + (c_lo * (1.0f - dz) + c_hi * dz), po, zo);
     }
 }
 
