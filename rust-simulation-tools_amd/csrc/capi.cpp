@@ -1193,6 +1193,7 @@ int mmpbsa_device_alloc(int32_t device, int64_t bytes, void **out)
 int mmpbsa_device_free(void *ptr)
 {
     return guarded(nullptr, [&] {
+        std::unique_lock<std::shared_mutex> g(capture_gate());
         if (ptr) HIP_OK(hipFree(ptr));
     });
 }
@@ -1201,6 +1202,7 @@ int mmpbsa_device_upload(void *dst, const void *src, int64_t bytes)
 {
     if ((!dst || !src) && bytes > 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null pointer");
     return guarded(nullptr, [&] {
+        std::unique_lock<std::shared_mutex> g(capture_gate());
         if (bytes > 0) HIP_OK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
     });
 }
@@ -1209,6 +1211,7 @@ int mmpbsa_device_download(void *dst, const void *src, int64_t bytes)
 {
     if ((!dst || !src) && bytes > 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null pointer");
     return guarded(nullptr, [&] {
+        std::unique_lock<std::shared_mutex> g(capture_gate());
         if (bytes > 0) HIP_OK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
     });
 }
@@ -1218,6 +1221,7 @@ int mmpbsa_device_copy(void *dst, const void *src, int64_t bytes)
     if ((!dst || !src) && bytes > 0) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null pointer");
     if (bytes % 16) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "bytes must be a multiple of 16");
     return guarded(nullptr, [&] {
+        std::unique_lock<std::shared_mutex> g(capture_gate());
         launch_copy16(src, dst, bytes / 16, nullptr);
         HIP_OK(hipDeviceSynchronize());
     });

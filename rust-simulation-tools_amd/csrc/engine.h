@@ -8,6 +8,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -82,6 +83,12 @@ struct DBuf {
     T *get() const { return p; }
 };
 
+// A graph capture in one thread and a call that synchronises with the legacy
+// stream or the device in another (a synchronous copy, a free, a launch on
+// the null stream) must not overlap: HIP rejects the call while a capture is
+// open.  Captures hold this gate shared, such calls hold it exclusively.
+std::shared_mutex &capture_gate();
+
 // Owned device array (not pooled): topology residency.
 template <typename T>
 struct DArr {
@@ -92,10 +99,13 @@ struct DArr {
     DArr &operator=(const DArr &) = delete;
     ~DArr()
     {
-        if (p) (void)hipFree(p);
+        if (!p) return;
+        std::unique_lock<std::shared_mutex> g(capture_gate());
+        (void)hipFree(p);
     }
     void upload(const std::vector<T> &v)
     {
+        std::unique_lock<std::shared_mutex> g(capture_gate());
         if (p) (void)hipFree(p);
         p = nullptr;
         n = v.size();

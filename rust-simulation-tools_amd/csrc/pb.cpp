@@ -685,6 +685,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (use_graph) {
+        std::shared_lock<std::shared_mutex> gate(capture_gate());
         HIP_OK(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
         try {
             enqueue_cycle(1);

@@ -306,9 +306,16 @@ PairTopo DevTopo::view() const
 }
 
 // ---- pool -------------------------------------------------------------------
+std::shared_mutex &capture_gate()
+{
+    static std::shared_mutex m;
+    return m;
+}
+
 Pool::~Pool()
 {
     trim();
+    std::unique_lock<std::shared_mutex> g(capture_gate());
     for (auto &kv : live_) (void)hipFree(kv.first);
 }
 
@@ -350,6 +357,7 @@ void Pool::put(void *p)
 
 void Pool::trim()
 {
+    std::unique_lock<std::shared_mutex> g(capture_gate());
     for (auto &kv : free_) (void)hipFree(kv.second);
     free_.clear();
     cached_ = 0;
@@ -368,6 +376,7 @@ Worker::~Worker()
         (void)hipStreamSynchronize(stream);
         (void)hipStreamDestroy(stream);
     }
+    std::unique_lock<std::shared_mutex> g(capture_gate());
     if (pin) (void)hipHostFree(pin);
     if (pin_i) (void)hipHostFree(pin_i);
 }

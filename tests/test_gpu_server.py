@@ -59,6 +59,27 @@ def test_alone_vs_mix_identity(systems):
         srv.close()
 
 
+def test_direct_calls_while_server_captures(systems):
+    """Direct calls (topology uploads: synchronous copies and frees) while the
+    server thread captures its cycle graphs.  HIP rejects such calls while a
+    capture is open; the library's capture gate keeps them apart, so neither
+    side fails and the results stay equal."""
+    (ta, fa, ra, la), (tb, fb, rb, lb), _ = systems
+    pb = rst.PbParams(**PB_SMALL)
+    srv = rst.Server(max_round_frames=16)
+    try:
+        tickets = [srv.submit(ta, fa, ra, la, pb_params=pb) for _ in range(4)]
+        direct = [rst.compute_binding_energy_frames(tb, fb, rb, lb, pb_params=pb, batched=True)
+                  for _ in range(4)]
+        served = [srv.wait(t)[0] for t in tickets]
+        for r in direct[1:]:
+            _same(direct[0], r)
+        for r in served[1:]:
+            _same(served[0], r)
+    finally:
+        srv.close()
+
+
 def test_budget_admission(systems):
     ta, fa, ra, la = systems[0]
     pb = rst.PbParams(**PB_SMALL)
