@@ -860,7 +860,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         bool nospec = false;  // no code word of the wave's six planes has the fractional bit
         float sa[6];
         uint32_t sreq = 0xffffffffu;
-        auto specials_issue = [&](int t) {
+        auto specials_issue = [&](int t, bool all) {
             uint32_t anyw = 0;
 #pragma unroll
             for (int k = 0; k < K; ++k) anyw |= Q[k].w;
@@ -873,10 +873,11 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                 return;
             }
             const int c = par ^ (t & 1);
-            uint32_t zmask = 0;  // half-sweeps with a plane to update (uniform)
+            uint32_t zmask = all ? (1u << (2 * K)) - 1u : 0u;  // half-sweeps with a plane
+            if (!all)
 #pragma unroll
-            for (int k = 1; k <= K; ++k)
-                zmask |= zok(k, t - k + 1) ? 3u << (2 * (k - 1)) : 0u;
+                for (int k = 1; k <= K; ++k)
+                    zmask |= zok(k, t - k + 1) ? 3u << (2 * (k - 1)) : 0u;
             uint32_t hb = 0, kb = 0;  // bit j: code bit 7 (fractional) / bit 6 (kappa)
 #pragma unroll
             for (int k = 1; k <= K; ++k) {
@@ -951,11 +952,13 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         };
         // MODE 0: fast path; 1: SLOW (a wave overflowed its slots); 2: the
         // wave has no fractional-face node in its six planes (table only)
-        auto stage = [&](int k, int t, auto mode_tag) {
+        // all: every half-sweep of step t has its plane in range (no test;
+        // uniform, so the scalar unit branches over the tests)
+        auto stage = [&](int k, int t, auto mode_tag, bool all) {
             constexpr int MODE = decltype(mode_tag)::value;
             constexpr bool SLOW = MODE == 1, NOSPEC = MODE == 2;
             const int z = t - k + 1;
-            if (!zok(k, z)) return;
+            if (!all && !zok(k, z)) return;
             const int c = par ^ (t & 1);
             const char *ringb = reinterpret_cast<const char *>(ring);
             const uint32_t sc = 4u * (uint32_t)(slot(z) - ring);
@@ -1042,7 +1045,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         load_coef(t0, nxt, UNI);
         load_plane(t0 + 1);
         if (tid < 2) *wg_ovf(tid) = 0;
-        for (int t = t0; t <= t1; ++t) {
+        auto step = [&](int t, bool all_tag) {
             store_plane(t + 1);
 #pragma unroll
             for (int i = K - 1; i > 0; --i) Q[i] = Q[i - 1];
@@ -1050,7 +1053,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
             step_cols(t);
             // order of issue: fractional faces, write-back, prefetches, so
             // that waiting for the faces waits for nothing younger
-            if (!UNI) specials_issue(t);
+            if (!UNI) specials_issue(t, all_tag);
             write_back(t - K);
             if (t < t1) {
                 load_coef(t + 1, nxt, UNI);
@@ -1060,11 +1063,11 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                 __syncthreads();
 #pragma unroll
                 for (int k = 1; k <= K; ++k) {
-                    stage(k, t, Fast{});
+                    stage(k, t, Fast{}, all_tag);
                     __syncthreads();
                 }
                 fold_finish();
-                continue;
+                return;
             }
             specials_store();
             // overflow flags alternate between two words: the one for step
@@ -1077,24 +1080,28 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
             if (__builtin_amdgcn_readfirstlane(*wg_ovf(t & 1))) {
 #pragma unroll
                 for (int k = 1; k <= K; ++k) {
-                    stage(k, t, Slow{});
+                    stage(k, t, Slow{}, all_tag);
                     __syncthreads();
                 }
             } else if (nospec) {
 #pragma unroll
                 for (int k = 1; k <= K; ++k) {
-                    stage(k, t, NoSpec{});
+                    stage(k, t, NoSpec{}, all_tag);
                     __syncthreads();
                 }
             } else {
 #pragma unroll
                 for (int k = 1; k <= K; ++k) {
-                    stage(k, t, Fast{});
+                    stage(k, t, Fast{}, all_tag);
                     __syncthreads();
                 }
             }
             fold_finish();
-        }
+        };
+        // steps ts .. te have all six half-sweeps in range: those run without
+        // the per-half-sweep range tests
+        const int ts = max(t0, max(K, z0 + K - 1)), te = min(t1, min(nz - 2, z1 + K - 2));
+        for (int t = t0; t <= t1; ++t) step(t, t >= ts && t <= te);
         // the grid's last plane (never updated, still in the ring) follows
         // the last step
         if (wz1 > z1) {
