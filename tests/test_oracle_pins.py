@@ -652,3 +652,121 @@ def test_pb_residue_terms_far_apart_closed_form(oracle):
     assert scale > 0.1
     assert np.abs(tr - er).max() < 0.02 * scale and np.abs(tl - el).max() < 0.02 * scale
     assert abs(tr.sum() + tl.sum() - d) < 1e-9 * max(1.0, abs(d))
+
+
+def _gb_from_radii(oracle, top, x, p, R):
+    L = oracle.lib()
+    v = top.view()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    return L.orc_gb_energy_from_radii(C.byref(v), P(x), C.byref(p), P(R))
+
+
+def test_gb_pair_distance_still_formula(oracle):
+    """The GB pair term at intermediate distances (gb_energy.rs:322-352).  The
+    effective distance the oracle uses is read back from its energy at fixed
+    Born radii (no salt: pair = -q_i q_j (1/eps_in - 1/eps_out) / f) and
+    compared with Still et al., JACS 112:6127 (1990):
+    f = sqrt(r^2 + R_i R_j exp(-r^2 / (4 R_i R_j))), which is sqrt(R_i R_j) at
+    r = 0 and tends to r for r >> R.  With salt the same f enters
+    exp(-0.73 kappa f) (the screening factor of the OBC papers)."""
+    Ri, Rj = 1.3, 2.1
+    top = make_topology([0.5, -0.3], [1.5, 1.5])
+    qa = np.asarray(top.charges_amber)
+    R = np.array([Ri, Rj])
+    rr = Ri * Rj
+    p0 = oracle.gb_params(salt_concentration=0.0)
+    ps = oracle.gb_params(salt_concentration=0.15)
+    k = oracle.compute_kappa(0.15, 78.3, 300.0)
+    far = np.array([[0.0, 0, 0], [5000.0, 0, 0]])  # inside the default 9999 A cutoff
+    self0 = _gb_from_radii(oracle, top, far, p0, R) + qa[0] * qa[1] * (1 - 1 / 78.3) / 5000.0
+    selfs = sum(-0.5 * q * q * (1 - math.exp(-0.73 * k * r) / 78.3) / r for q, r in zip(qa, R))
+    for r in (0.0, 0.6, math.sqrt(rr), 2.5, Ri + Rj, 6.0, 15.0):
+        x = np.array([[0.0, 0, 0], [r * 0.6, r * 0.8, 0.0]])
+        pair0 = _gb_from_radii(oracle, top, x, p0, R) - self0
+        f_seen = -qa[0] * qa[1] * (1 - 1 / 78.3) / pair0
+        f_still = math.sqrt(r * r + rr * math.exp(-r * r / (4 * rr)))
+        assert abs(f_seen - f_still) < 1e-9 * f_still, (r, f_seen, f_still)
+        pairs = _gb_from_radii(oracle, top, x, ps, R) - selfs
+        want = -qa[0] * qa[1] * (1 - math.exp(-0.73 * k * f_still) / 78.3) / f_still
+        assert abs(pairs - want) < 1e-10 * abs(want), (r, pairs, want)
+    # r >> R: f -> r
+    x = np.array([[0.0, 0, 0], [40.0, 0, 0]])
+    f40 = -qa[0] * qa[1] * (1 - 1 / 78.3) / (_gb_from_radii(oracle, top, x, p0, R) - self0)
+    assert abs(f40 - 40.0) < 1e-9
+
+
+@pytest.mark.parametrize("a", [2.0, 3.0])
+def test_pb_salt_born_ion_debye_huckel(oracle, a):
+    """The kappa-bar^2 = eps_s kappa^2 map (pb_grid.rs:951-985) in isolation:
+    one ion of radius a in a uniform eps = 80 medium (eps_in = eps_out, so the
+    solvated and reference problems differ only by the salt), ion radius 0 so
+    the salt is excluded from the sphere r < a.  Debye-Hueckel for a charge at
+    the centre of a salt-free sphere gives the extra potential at the charge
+    -C q kappa / (eps (1 + kappa a)), so dG_PB = -(C q^2 / 2) kappa /
+    (eps (1 + kappa a)), C = 332.0522.  Dropping eps_s from the map (kappa_eff
+    = kappa / sqrt(80)) or doubling it moves dG by far more than the 2% bar."""
+    top = make_topology([1.0], [a])
+    p = oracle.pb_params(grid_spacing=0.5, grid_buffer=16.0 if a == 2.0 else 20.0,
+                         solute_dielectric=80.0, solvent_dielectric=80.0,
+                         salt_concentration=0.15, probe_radius=0.0, ion_radius=0.0,
+                         tolerance=1e-9, max_iterations=20000, fillratio=0.0)
+    r = oracle.pb_energy(top, np.zeros((1, 3)), p)
+    k = oracle.compute_kappa(0.15, 80.0, 300.0)
+    want = -0.5 * 332.0522 * k / (80.0 * (1.0 + k * a))
+    assert r.solv.converged and r.ref.converged
+    assert abs(r.total - want) < 0.02 * abs(want), (r.total, want)
+
+
+def test_coarse_grid_invariants(oracle):
+    """build_coarse_grid (pb_energy.rs:92-125) on point clouds, through what a
+    correct coarse grid must satisfy on each axis: spacing h * fscale, an odd
+    node count, centred on the bounding box, spanning at least fillratio times
+    the extent, and the fewest odd nodes that do ((n - 3) cs < F ext)."""
+    L = oracle.lib()
+    rng = np.random.default_rng(23)
+    for trial in range(60):
+        n = int(rng.integers(2, 80))
+        x = rng.normal(0, rng.uniform(0.5, 30), (n, 3)) + rng.uniform(-80, 80, 3)
+        h = float(rng.choice([0.3, 0.5, 0.6, 1.0]))
+        F = float(rng.choice([1.5, 2.0, 4.0, 6.0]))
+        s = int(rng.choice([2, 4, 8]))
+        g = oracle.OrcGrid()
+        L.orc_coarse_grid(n, P(np.ascontiguousarray(x)), h, F, s, C.byref(g))
+        mn, mx = x.min(0), x.max(0)
+        cs = h * s
+        for k in range(3):
+            nk, o = g.n[k], g.o[k]
+            assert g.h[k] == cs and nk % 2 == 1
+            span = (nk - 1) * cs
+            ext = F * (mx[k] - mn[k])
+            assert abs((o + 0.5 * span) - 0.5 * (mn[k] + mx[k])) <= 1e-12 * (1 + abs(mn[k]) + abs(mx[k]))
+            assert span >= ext - 1e-9 * max(1.0, ext)
+            assert (nk - 3) * cs < ext
+
+
+def test_decomposition_gb_weight_charge_power(oracle):
+    """The GB share of a residue is proportional to sum |q^2 (1/R_complex -
+    1/R_alone)| over its atoms (decomposition.rs:309-351).  Born radii do not
+    depend on charges, so scaling one interface residue's charges by c scales
+    its share ratio to any other residue by exactly c^2 (whatever dGB does);
+    a weight linear in |q|, or one taken from pair terms, scales by c or not
+    at all."""
+    rng = np.random.default_rng(8)
+    centres = np.array([[0.0, 0, 0], [0, 4.2, 0], [4.2, 0, 0], [0, 0, 4.4]])
+    x = np.concatenate([c + rng.normal(0, 0.9, (4, 3)) for c in centres])
+    q = rng.normal(0, 0.5, 16)
+    radii = rng.choice([1.2, 1.5, 1.7], 16)
+    rec, lig = [0, 1, 2], [3]
+    shares = []
+    for c in (1.0, 2.5):
+        qq = q.copy()
+        qq[0:4] *= c
+        top = make_topology(qq, radii, residue_pointers=(0, 4, 8, 12))
+        rr, ll = oracle.decompose(top, x, rec, lig)
+        shares.append({d["residue_index"]: d["polar"] for d in rr + ll})
+    for other in (1, 2, 3):
+        r1 = shares[0][0] / shares[0][other]
+        r2 = shares[1][0] / shares[1][other]
+        assert abs(shares[0][other]) > 1e-6
+        assert abs(r2 / r1 - 2.5 ** 2) < 1e-9 * 6.25, (other, r2 / r1)
