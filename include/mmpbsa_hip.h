@@ -199,6 +199,37 @@ int mmpbsa_sa_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_sa_params 
 int mmpbsa_pb_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_pb_params *p,
                      mmpbsa_pb_result *out);
 
+/* compute_pb_energy_solvated (pb_energy.rs:442-482): with solute_res (0-based
+ * residue indices, n_solute of them) the topology is stripped to those
+ * residues (strip_to_solute: build_selection + extract_subtopology, the
+ * coordinates sliced alike) before compute_pb_energy; solute_res == NULL is
+ * mmpbsa_pb_energy.  A residue index >= n_residues is MMPBSA_ERR_BAD_RESIDUE
+ * (prmtop.rs:167-175).  The stripped topology stays resident on the context
+ * while later calls name the same residues. */
+int mmpbsa_pb_energy_solvated(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_pb_params *p,
+                              const int32_t *solute_res, int32_t n_solute, mmpbsa_pb_result *out);
+
+/* compute_gb_energy_from_radii (gb_energy.rs:283-355): the GB energy of the
+ * given Born radii [n_atoms] without computing radii (the step that
+ * compute_gb_energy_cached, gb_energy.rs:372-387, takes on a cache hit). */
+int mmpbsa_gb_energy_from_radii(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_gb_params *p,
+                                const double *born_radii, double *total_out);
+
+/* GbCache (gb_energy.rs:66-120): Born radii of the last frame computed
+ * through mmpbsa_gb_energy_cached, with its coordinates (host memory).  A
+ * cache may serve any context; it is externally synchronised. */
+typedef struct mmpbsa_gb_cache mmpbsa_gb_cache;
+int mmpbsa_gb_cache_create(double rmsd_threshold, mmpbsa_gb_cache **out);
+int mmpbsa_gb_cache_destroy(mmpbsa_gb_cache *cache);
+/* compute_gb_energy_cached (gb_energy.rs:372-387): when the cache holds as
+ * many atoms as the context and the RMSD to its coordinates (summed in atom
+ * order, GbCache::try_reuse) is below the threshold, its radii are reused
+ * unchanged; otherwise radii are computed and the cache takes them and xyz.
+ * *reused (nullable) is 1 on reuse; born_radii_out [n_atoms] may be NULL. */
+int mmpbsa_gb_energy_cached(mmpbsa_ctx *ctx, mmpbsa_gb_cache *cache, const double *xyz,
+                            const mmpbsa_gb_params *p, double *born_radii_out, double *total_out,
+                            int32_t *reused);
+
 /* ---- receptor / ligand split (binding.rs:142-211, subsystem.rs:22-169) -- */
 typedef struct mmpbsa_split mmpbsa_split;
 

@@ -40,6 +40,10 @@ static inline void kadd(ksum *k, double v)
 static inline double kval(const ksum *k) { return k->s + k->c; }
 
 static int g_face_scan = 1;
+/* > 0: the multigrid divergence test fires at this cycle (a test hook that
+ * mirrors the library's MMPBSA_FORCE_DIVERGENCE_CYCLE) */
+static int g_force_div = 0;
+void orc_set_force_divergence(int cycle) { g_force_div = cycle > 0 ? cycle : 0; }
 void orc_set_face_scan(int mode) { g_face_scan = mode; }
 void orc_set_threads(int n)
 {
@@ -1523,7 +1527,8 @@ void orc_solve_multigrid(const orc_grid *g, const double *charge, const double *
         } else {
             growing++;
         }
-        const int div = !isfinite(r) || r > best * 1e4 || (growing >= 5 && r > prev);
+        const int div = (g_force_div > 0 && cyc + 1 == g_force_div) || !isfinite(r) ||
+                        r > best * 1e4 || (growing >= 5 && r > prev);
         if (div) {
             /* fall back to Chebyshev SOR from the partial potential, BC re-applied */
             apply_bc(g, bc, n, xyz, q, kappa_bulk, eps_out, phi, saved);

@@ -112,6 +112,7 @@ void orc_decomp_pairs(const mmpbsa_topology_view *t, const double *xyz, int32_t 
  * 1 = restrict to atoms in nearby cells (same interval set, same result). */
 void orc_set_face_scan(int mode);
 void orc_set_threads(int n);
+void orc_set_force_divergence(int cycle);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,7 @@ def lib():
             "orc_decomp_pairs": (None, [TV, dp, C.c_int32, ip, C.c_int32, ip, ip, dp, dp]),
             "orc_set_face_scan": (None, [C.c_int]),
             "orc_set_threads": (None, [C.c_int]),
+            "orc_set_force_divergence": (None, [C.c_int]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -264,8 +265,13 @@ def frame_energy(top: AmberTopology, xyz, rec_res, lig_res, method=0, gb=None, p
     lx = extract_coords(cx, la)
     out = {}
     for name, t, x in (("complex", ctop, cx), ("receptor", rtop, rx), ("ligand", ltop, lx)):
-        out[name] = {"mm": mm_energy(t, x)["total"], "polar": polar(t, x, method, gb, pb),
-                     "sa": sa_energy(t, x, sa)[0]}
+        out[name] = {"mm": mm_energy(t, x)["total"], "sa": sa_energy(t, x, sa)[0]}
+        if method == 1:
+            r = pb_energy(t, x, pb)
+            out[name]["polar"] = r.total
+            out[name]["pb"] = r  # PbResultC: per-medium energies, cycles, grids
+        else:
+            out[name]["polar"] = gb_energy(t, x, gb)[0]
     c, r, l = out["complex"], out["receptor"], out["ligand"]
     out["delta_mm"] = c["mm"] - r["mm"] - l["mm"]
     out["delta_polar"] = c["polar"] - r["polar"] - l["polar"]
@@ -340,3 +346,64 @@ def decompose(top: AmberTopology, xyz, rec_res, lig_res, gb=None, sa=None):
                  "polar": float(pol[m]), "sa": float(sav[m])} for o, m in zip(orig, rem)]
 
     return mk(rec_res, rmap), mk(lig_res, lmap)
+
+
+def reimage(xyz, bonds, box):
+    """reimage_complex (binding.rs:234-308), restated on a copy of (N, 3) A
+    coordinates: breadth-first walks over the bond graph (adjacency in bond
+    order, components started from the lowest unvisited atom) put each
+    neighbour at the minimum image of the atom it is reached from; then every
+    component after the first moves by one box vector per axis on which its
+    geometric centre (binding.rs:311-323) lies more than half a box from the
+    first component's."""
+    from collections import deque
+    x = np.array(xyz, dtype=np.float64, copy=True)
+    n = x.shape[0]
+    if n == 0:
+        return x
+    adj = [[] for _ in range(n)]
+    for a, b in np.asarray(bonds).reshape(-1, np.shape(bonds)[-1])[:, :2]:
+        a, b = int(a), int(b)
+        if a < n and b < n:
+            adj[a].append(b)
+            adj[b].append(a)
+    seen = [False] * n
+    comps = []
+    for s in range(n):
+        if seen[s]:
+            continue
+        comp, q = [], deque([s])
+        seen[s] = True
+        while q:
+            c = q.popleft()
+            comp.append(c)
+            for nb in adj[c]:
+                if not seen[nb]:
+                    seen[nb] = True
+                    for d in range(3):
+                        diff = x[nb, d] - x[c, d]
+                        if diff > box[d] * 0.5:
+                            x[nb, d] -= box[d]
+                        elif diff < -box[d] * 0.5:
+                            x[nb, d] += box[d]
+                    q.append(nb)
+        comps.append(comp)
+    if len(comps) <= 1:
+        return x
+
+    def centre(idx):
+        c = [0.0, 0.0, 0.0]
+        for i in idx:
+            for d in range(3):
+                c[d] += x[i, d]
+        return [v / float(len(idx)) for v in c]
+
+    ref = centre(comps[0])
+    for comp in comps[1:]:
+        cc = centre(comp)
+        for d in range(3):
+            diff = cc[d] - ref[d]
+            sh = -box[d] if diff > box[d] * 0.5 else (box[d] if diff < -box[d] * 0.5 else 0.0)
+            for a in comp:
+                x[a, d] += sh
+    return x

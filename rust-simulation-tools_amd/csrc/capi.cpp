@@ -10,6 +10,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <set>
 #include <thread>
 #include <unordered_map>
@@ -25,6 +26,16 @@ struct mmpbsa_ctx {
     std::unique_ptr<Worker> w;
     std::string err;
     int device = 0;
+    // the solute of the last mmpbsa_pb_energy_solvated call, kept resident
+    std::vector<int> solute_res, solute_sel;
+    std::unique_ptr<DevTopo> solute;
+};
+
+// GbCache (gb_energy.rs:66-120): the coordinates and Born radii of the last
+// frame whose radii were computed, on the host as in the reference.
+struct mmpbsa_gb_cache {
+    double threshold = 0.5;
+    std::vector<double> xyz, radii;
 };
 
 struct mmpbsa_split {
@@ -559,6 +570,112 @@ int mmpbsa_pb_energy(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_pb_params 
         DBuf<double> d(w.pool, 3 * (size_t)n + 3);
         HIP_OK(hipMemcpyAsync(d.get(), xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, w.stream));
         run_pb(w, ctx->top, d.get(), xyz, *p, out);
+    });
+}
+
+// compute_pb_energy_solvated (pb_energy.rs:442-482): strip_to_solute, then
+// compute_pb_energy of the solute.  The solute's topology stays resident on
+// the context until a call names other residues.
+int mmpbsa_pb_energy_solvated(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_pb_params *p,
+                              const int32_t *solute_res, int32_t n_solute, mmpbsa_pb_result *out)
+{
+    if (!ctx || !p || !out || n_solute < 0 || (n_solute > 0 && !solute_res))
+        return fail(ctx, MMPBSA_ERR_INVALID_ARG, "invalid argument");
+    if (!solute_res) return mmpbsa_pb_energy(ctx, xyz, p, out);
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        const std::vector<int> res(solute_res, solute_res + n_solute);
+        if (!ctx->solute || res != ctx->solute_res) {
+            std::vector<int> sel = ctx->top.h.build_selection(res);  // prmtop.rs:165-204 errors
+            std::unique_ptr<DevTopo> t(new DevTopo());
+            t->build(ctx->top.h.extract(sel));
+            ctx->solute = std::move(t);
+            ctx->solute_sel = std::move(sel);
+            ctx->solute_res = res;
+        }
+        Worker &w = *ctx->w;
+        const std::vector<double> hx = gather_host(xyz, ctx->solute_sel);
+        const int n = (int)ctx->solute_sel.size();
+        DBuf<double> d(w.pool, 3 * (size_t)n + 3);
+        if (n)
+            HIP_OK(hipMemcpyAsync(d.get(), hx.data(), sizeof(double) * 3 * n,
+                                  hipMemcpyHostToDevice, w.stream));
+        run_pb(w, *ctx->solute, d.get(), hx.data(), *p, out);
+    });
+}
+
+int mmpbsa_gb_energy_from_radii(mmpbsa_ctx *ctx, const double *xyz, const mmpbsa_gb_params *p,
+                                const double *born_radii, double *total_out)
+{
+    if (!ctx || !p || !born_radii || !total_out)
+        return fail(ctx, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        Worker &w = *ctx->w;
+        const int n = ctx->top.h.n;
+        DBuf<double> d(w.pool, 3 * (size_t)n + 3), R(w.pool, (size_t)n + 1);
+        HIP_OK(hipMemcpyAsync(d.get(), xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, w.stream));
+        HIP_OK(hipMemcpyAsync(R.get(), born_radii, sizeof(double) * n, hipMemcpyHostToDevice,
+                              w.stream));
+        *total_out = run_gb_from_radii(w, ctx->top, d.get(), *p, R.get());
+    });
+}
+
+int mmpbsa_gb_cache_create(double rmsd_threshold, mmpbsa_gb_cache **out)
+{
+    if (!out) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    *out = new (std::nothrow) mmpbsa_gb_cache();
+    if (!*out) return fail(nullptr, MMPBSA_ERR_OOM, "host allocation failed");
+    (*out)->threshold = rmsd_threshold;
+    return MMPBSA_OK;
+}
+
+int mmpbsa_gb_cache_destroy(mmpbsa_gb_cache *c)
+{
+    delete c;
+    return MMPBSA_OK;
+}
+
+// compute_gb_energy_cached (gb_energy.rs:372-387): radii are reused when the
+// cache holds as many atoms and the RMSD (GbCache::try_reuse, the reference's
+// summation order) is below the threshold; otherwise they are computed and
+// stored together with these coordinates.
+int mmpbsa_gb_energy_cached(mmpbsa_ctx *ctx, mmpbsa_gb_cache *cache, const double *xyz,
+                            const mmpbsa_gb_params *p, double *born_radii_out, double *total_out,
+                            int32_t *reused)
+{
+    if (!ctx || !cache || !p || !total_out) return fail(ctx, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        check_xyz(xyz);
+        const int n = ctx->top.h.n;
+        bool hit = false;
+        if (n > 0 && cache->xyz.size() == 3 * (size_t)n) {
+            double ss = 0.0;
+            for (int i = 0; i < n; ++i) {
+                const double dx = cache->xyz[3 * i] - xyz[3 * i];
+                const double dy = cache->xyz[3 * i + 1] - xyz[3 * i + 1];
+                const double dz = cache->xyz[3 * i + 2] - xyz[3 * i + 2];
+                ss += dx * dx + dy * dy + dz * dz;
+            }
+            hit = std::sqrt(ss / (double)n) < cache->threshold;
+        }
+        Worker &w = *ctx->w;
+        DBuf<double> d(w.pool, 3 * (size_t)n + 3);
+        HIP_OK(hipMemcpyAsync(d.get(), xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, w.stream));
+        if (hit) {
+            DBuf<double> R(w.pool, (size_t)n + 1);
+            HIP_OK(hipMemcpyAsync(R.get(), cache->radii.data(), sizeof(double) * n,
+                                  hipMemcpyHostToDevice, w.stream));
+            *total_out = run_gb_from_radii(w, ctx->top, d.get(), *p, R.get());
+            if (born_radii_out) std::memcpy(born_radii_out, cache->radii.data(), sizeof(double) * n);
+        } else {
+            std::vector<double> born;
+            *total_out = run_gb(w, ctx->top, d.get(), *p, nullptr, &born);
+            cache->xyz.assign(xyz, xyz + 3 * (size_t)n);
+            cache->radii = born;
+            if (born_radii_out && n) std::memcpy(born_radii_out, born.data(), sizeof(double) * n);
+        }
+        if (reused) *reused = hit ? 1 : 0;
     });
 }
 

@@ -78,6 +78,24 @@ double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_
     return w.pin[0];
 }
 
+// compute_gb_energy_from_radii (gb_energy.rs:283-355): the energy kernel alone
+// over given device Born radii.
+double run_gb_from_radii(Worker &w, const DevTopo &t, const double *d_xyz,
+                         const mmpbsa_gb_params &p, const double *d_born)
+{
+    const PairTopo v = t.view();
+    if (v.n == 0) return 0.0;
+    const int nbk = gb_energy_blocks(v.n);
+    DBuf<double> part(w.pool, nbk), res(w.pool, 1);
+    const double kappa = compute_kappa(p.salt_concentration, p.solvent_dielectric, p.temperature);
+    launch_gb_energy(d_xyz, v, d_born, kappa, p.solute_dielectric, p.solvent_dielectric, p.cutoff,
+                     part.get(), w.stream);
+    launch_sum_partials(part.get(), nbk, 1, 1, res.get(), w.stream);
+    HIP_OK(hipMemcpyAsync(w.pin, res.get(), sizeof(double), hipMemcpyDeviceToHost, w.stream));
+    w.sync();
+    return w.pin[0];
+}
+
 // ---- geometry helpers (host, FP64 exactly as the reference) -----------------
 void bbox(int n, const double *xyz, double *mn, double *mx)
 {

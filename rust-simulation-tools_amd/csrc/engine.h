@@ -152,6 +152,8 @@ struct SolveInfo {
 void run_mm(Worker &w, const DevTopo &t, const double *d_xyz, mmpbsa_mm_terms *out);
 double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_params &p,
               double *d_born /* nullable: pooled scratch used */, std::vector<double> *born_out);
+double run_gb_from_radii(Worker &w, const DevTopo &t, const double *d_xyz,
+                         const mmpbsa_gb_params &p, const double *d_born);
 // h_xyz (host copy of d_xyz) may be null: the bounding box is then taken on
 // the device.
 void run_sa(Worker &w, const DevTopo &t, const double *d_xyz, const double *h_xyz,

@@ -24,6 +24,8 @@ struct MgStatus {
     double rms;
     double tol;
     double count;    // interior nodes (RMS denominator)
+    int diverge_at;  // > 0: the divergence test fires at this cycle (set from the host)
+    int pad_;
 };
 
 // One FP32 level of the correction V-cycle.  nx == 0 marks "no work for this

@@ -1801,7 +1801,9 @@ __global__ __launch_bounds__(256) void mg_check_kernel(const MgOuterDev *__restr
     } else {
         st->growing += 1;
     }
-    const bool div = !isfinite(rms) || rms > st->best * 1e4 || (st->growing >= 5 && rms > st->prev);
+    const bool forced = st->diverge_at > 0 && cycle == st->diverge_at;
+    const bool div = forced || !isfinite(rms) || rms > st->best * 1e4 ||
+                     (st->growing >= 5 && rms > st->prev);
     if (div) {
         st->state = 2;
         return;

@@ -241,7 +241,19 @@ static MgStatus status_template(double tol, double count, int max_cycles)
     s.rms = 0.0;
     s.tol = tol;
     s.count = count;
+    s.diverge_at = 0;
+    s.pad_ = 0;
     return s;
+}
+
+// MMPBSA_FORCE_DIVERGENCE_CYCLE=k (k > 0) makes the divergence test of every
+// multigrid solve fire at cycle k, so that the fallback of
+// pb_solver.rs:1110-1138 (Chebyshev SOR from the partial potential) can be
+// exercised from the host; unset or 0 leaves the reference's test alone.
+static int forced_divergence_cycle()
+{
+    const char *e = std::getenv("MMPBSA_FORCE_DIVERGENCE_CYCLE");
+    return e ? std::max(0, std::atoi(e)) : 0;
 }
 
 // solve_lpbe_initial (pb_solver.rs:373-447): Chebyshev-ramped FP64 SOR from phi.
@@ -589,10 +601,12 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     MgOuterDev *dO0 = (MgOuterDev *)(dbase + off);
     off += szO;
     std::vector<MgStatus> init(S);
+    const int div_at = forced_divergence_cycle();
     for (int s = 0; s < S; ++s) {
         const GridDesc &g = W[s].t->g;
         init[s] = status_template(W[s].t->tol, (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2),
                                   W[s].maxv);
+        init[s].diverge_at = div_at;
     }
     std::memcpy(hb.data() + off, init.data(), S * sizeof(MgStatus));
     MgStatus *dst = (MgStatus *)(dbase + off);

@@ -330,12 +330,22 @@ void *Pool::get(size_t bytes)
         free_.erase(it);
         return p;
     }
+    // A fresh allocation waits for any graph capture in another thread to
+    // end (see capture_gate); no capture region allocates, so the capturing
+    // thread never gets here holding the gate shared.
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e;
+    {
+        std::unique_lock<std::shared_mutex> g(capture_gate());
+        e = hipMalloc(&p, bytes);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         trim();
-        e = hipMalloc(&p, bytes);
+        {
+            std::unique_lock<std::shared_mutex> g(capture_gate());
+            e = hipMalloc(&p, bytes);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             throw DeviceError(MMPBSA_ERR_OOM, "device allocation of " + std::to_string(bytes) +
@@ -365,6 +375,7 @@ void Pool::trim()
 
 Worker::Worker()
 {
+    std::unique_lock<std::shared_mutex> g(capture_gate());
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_OK(hipHostMalloc((void **)&pin, sizeof(double) * 64, hipHostMallocDefault));
     HIP_OK(hipHostMalloc((void **)&pin_i, sizeof(int) * 64, hipHostMallocDefault));

@@ -26,7 +26,8 @@ __all__ = [
     "AmberTopology", "GbModel", "GbParams", "PbParams", "SaParams", "MmEnergy", "GbEnergy",
     "SaEnergy", "PbEnergy", "FrameEnergy", "BindingResult", "ResidueContribution",
     "DecompositionResult", "compute_mm_energy", "compute_gb_energy", "compute_sa_energy",
-    "compute_pb_energy", "compute_binding_energy_single_frame", "compute_binding_energy_frames",
+    "compute_pb_energy", "compute_gb_energy_from_radii", "compute_gb_energy_cached", "GbCache",
+    "compute_binding_energy_single_frame", "compute_binding_energy_frames",
     "decompose_binding_energy", "decompose_binding_energy_pb", "extract_subtopology",
 ]
 
@@ -164,6 +165,10 @@ class PbEnergy:
         self.final_residual = (r.solv.final_residual, r.ref.final_residual)
         self.fine_dims = tuple(r.fine_dims)
         self.coarse_dims = tuple(r.coarse_dims)
+        # extensions: whether each fine solve took its boundary from the
+        # coarse focusing solve, and that solve's cycle count
+        self.focused = (bool(r.solv.focused), bool(r.ref.focused))
+        self.coarse_iterations = (r.solv.coarse_iterations, r.ref.coarse_iterations)
 
 
 class FrameEnergy:
@@ -355,18 +360,77 @@ def compute_sa_energy(topology: AmberTopology, coords, sa_params: SaParams | Non
 
 def compute_pb_energy(topology: AmberTopology, coords, pb_params: PbParams | None = None,
                       solute_residues=None) -> PbEnergy:
-    """compute_pb_energy_solvated (pb_energy.rs:469-482)."""
+    """compute_pb_energy_solvated (pb_energy.rs:469-482): with solute_residues
+    the topology is stripped to those residues on the library side
+    (mmpbsa_pb_energy_solvated) before the PB solve."""
     p = (pb_params or PbParams())._c
-    if solute_residues is not None:
-        sel = topology.build_selection(solute_residues)
-        sub = _extract_subtopology(topology, sel)
-        x_nm = np.asarray(coords, dtype=np.float64)[sel]
-        return compute_pb_energy(sub, x_nm, pb_params, None)
     c = _ctx(topology)
     x = _angstrom(coords, topology.n_atoms)
     out = PbResultC()
-    _check(c.lib.mmpbsa_pb_energy(c.h, _dp(x), C.byref(p), C.byref(out)), c.lib, c.h)
+    if solute_residues is None:
+        _check(c.lib.mmpbsa_pb_energy(c.h, _dp(x), C.byref(p), C.byref(out)), c.lib, c.h)
+    else:
+        res = np.ascontiguousarray([int(r) for r in solute_residues], dtype=np.int32)
+        if res.size and res.min() < 0:
+            raise ValueError(f"Residue index {int(res.min())} out of range "
+                             f"(0-{topology.n_residues - 1})")
+        _check(c.lib.mmpbsa_pb_energy_solvated(c.h, _dp(x), C.byref(p),
+                                               res.ctypes.data_as(_abi.c_int32_p), len(res),
+                                               C.byref(out)), c.lib, c.h)
     return PbEnergy(out)
+
+
+def compute_gb_energy_from_radii(topology: AmberTopology, coords, born_radii,
+                                 gb_params: GbParams | None = None) -> float:
+    """compute_gb_energy_from_radii (gb_energy.rs:283-355): the GB energy of
+    given Born radii (Angstrom), with no radii computation."""
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    R = np.ascontiguousarray(born_radii, dtype=np.float64)
+    if R.shape != (topology.n_atoms,):
+        raise ValueError(f"born_radii has shape {R.shape}, topology has {topology.n_atoms} atoms")
+    p = (gb_params or GbParams())._c
+    tot = C.c_double()
+    _check(c.lib.mmpbsa_gb_energy_from_radii(c.h, _dp(x), C.byref(p), _dp(R), C.byref(tot)),
+           c.lib, c.h)
+    return tot.value
+
+
+class GbCache:
+    """GbCache (gb_energy.rs:66-120): Born radii reused across frames whose
+    RMSD (Angstrom) to the last computed frame is below rmsd_threshold.
+    last_reused tells whether the last compute_gb_energy_cached call reused
+    them."""
+
+    def __init__(self, rmsd_threshold: float = 0.5):
+        self.lib = load_library()
+        self.rmsd_threshold = float(rmsd_threshold)
+        self.h = C.c_void_p()
+        _check(self.lib.mmpbsa_gb_cache_create(self.rmsd_threshold, C.byref(self.h)), self.lib)
+        self.last_reused = False
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.mmpbsa_gb_cache_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+def compute_gb_energy_cached(topology: AmberTopology, coords, gb_params: GbParams | None,
+                             cache: GbCache) -> GbEnergy:
+    """compute_gb_energy_cached (gb_energy.rs:372-387)."""
+    c = _ctx(topology)
+    x = _angstrom(coords, topology.n_atoms)
+    p = (gb_params or GbParams())._c
+    radii = np.zeros(topology.n_atoms)
+    tot = C.c_double()
+    hit = C.c_int32()
+    _check(c.lib.mmpbsa_gb_energy_cached(c.h, cache.h, _dp(x), C.byref(p), _dp(radii),
+                                         C.byref(tot), C.byref(hit)), c.lib, c.h)
+    cache.last_reused = bool(hit.value)
+    return GbEnergy(tot.value, radii)
 
 
 def _method(gb_params, pb_params):

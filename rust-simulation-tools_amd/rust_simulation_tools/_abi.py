@@ -214,6 +214,14 @@ SIGNATURES = {
     "mmpbsa_sa_energy": (C.c_int, [P, c_double_p, C.POINTER(SaParamsC), c_double_p, c_double_p,
                                    c_double_p]),
     "mmpbsa_pb_energy": (C.c_int, [P, c_double_p, C.POINTER(PbParamsC), C.POINTER(PbResultC)]),
+    "mmpbsa_pb_energy_solvated": (C.c_int, [P, c_double_p, C.POINTER(PbParamsC), c_int32_p,
+                                            C.c_int32, C.POINTER(PbResultC)]),
+    "mmpbsa_gb_energy_from_radii": (C.c_int, [P, c_double_p, C.POINTER(GbParamsC), c_double_p,
+                                              c_double_p]),
+    "mmpbsa_gb_cache_create": (C.c_int, [C.c_double, C.POINTER(P)]),
+    "mmpbsa_gb_cache_destroy": (C.c_int, [P]),
+    "mmpbsa_gb_energy_cached": (C.c_int, [P, P, c_double_p, C.POINTER(GbParamsC), c_double_p,
+                                          c_double_p, c_int32_p]),
     "mmpbsa_split_create": (C.c_int, [P, c_int32_p, C.c_int32, c_int32_p, C.c_int32, C.POINTER(P)]),
     "mmpbsa_split_destroy": (C.c_int, [P]),
     "mmpbsa_split_atoms": (C.c_int, [P, C.c_int32, c_int32_p, C.c_int32, c_int32_p]),

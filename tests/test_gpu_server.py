@@ -60,20 +60,33 @@ def test_alone_vs_mix_identity(systems):
 
 
 def test_direct_calls_while_server_captures(systems):
-    """Direct calls (topology uploads: synchronous copies and frees) while the
-    server thread captures its cycle graphs.  HIP rejects such calls while a
+    """Direct calls that create and destroy device state while the server
+    thread captures its cycle graphs: every direct call uses a topology built
+    afresh (context and split creation: stream, pinned buffers, synchronous
+    uploads; destruction: frees), and device arrays are allocated, filled,
+    read back and freed in between.  HIP rejects synchronising calls while a
     capture is open; the library's capture gate keeps them apart, so neither
     side fails and the results stay equal."""
+    import gc
     (ta, fa, ra, la), (tb, fb, rb, lb), _ = systems
     pb = rst.PbParams(**PB_SMALL)
+    ref = rst.compute_binding_energy_frames(tb, fb, rb, lb, pb_params=pb, batched=True)
     srv = rst.Server(max_round_frames=16)
     try:
-        tickets = [srv.submit(ta, fa, ra, la, pb_params=pb) for _ in range(4)]
-        direct = [rst.compute_binding_energy_frames(tb, fb, rb, lb, pb_params=pb, batched=True)
-                  for _ in range(4)]
+        tickets = [srv.submit(ta, fa, ra, la, pb_params=pb) for _ in range(6)]
+        direct = []
+        for k in range(4):
+            fresh, _, rr, ll = synthetic.build_complex(1400, 120, None, synthetic.seed_for(3, 1))
+            direct.append(rst.compute_binding_energy_frames(fresh, fb, rr, ll, pb_params=pb,
+                                                            batched=True))
+            del fresh            # context and split destroyed while tickets are pending
+            gc.collect()
+            arr = rst.DeviceArray(fb * (k + 1.0))
+            assert np.array_equal(arr.numpy(), fb * (k + 1.0))
+            del arr
         served = [srv.wait(t)[0] for t in tickets]
-        for r in direct[1:]:
-            _same(direct[0], r)
+        for r in direct:
+            _same(ref, r)
         for r in served[1:]:
             _same(served[0], r)
     finally:

@@ -64,3 +64,39 @@ def test_mdcrd_trajectory(system, tmp_path):
         rst.compute_binding_energy(top, p, rec, lig, start_frame=10)
     with pytest.raises(ValueError, match="Failed to open mdcrd"):
         rst.compute_binding_energy(top, tmp_path / "none.mdcrd", rec, lig)
+
+
+def test_dcd_reimaging_in_a_real_box(system, oracle, tmp_path):
+    """Frames wrapped into a 40 A box (the 700-atom complex spans ~28 A and
+    sits on a box corner, so its chains cross the faces): the trajectory
+    driver reimages each frame (binding.rs:234-308, :630-645) before the
+    batched PB path.  Its energies equal those of the same frames reimaged
+    by the oracle's restatement and run in memory, and the complex
+    coordinates it kept of the last frame equal the oracle's bit for bit."""
+    import ctypes as C
+    top, frames, rec, lig = system
+    L = 40.0
+    c = 0.5 * (frames[0].min(0) + frames[0].max(0))
+    wrapped = np.mod(frames - c, L)
+    p = tmp_path / "box.dcd"
+    _write_dcd(p, wrapped, cells=[[L, L, L, 90.0, 90.0, 90.0]] * len(frames))
+    pb = rst.PbParams(**PB)
+    res = rst.compute_binding_energy(top, p, rec, lig, pb_params=pb, trajectory_format="dcd",
+                                     stride=2)
+    traj, _ = rst.DcdReader(p).read_all()          # nm, float32-rounded positions
+    bonds = np.asarray(top.bonds)[:, :2]
+    box = np.array([L, L, L])
+    fixed = np.stack([oracle.reimage(traj[k] * 10.0, bonds, box) for k in (0, 2, 4)])
+    assert np.abs(fixed - traj[[0, 2, 4]] * 10.0).max() > 10.0   # reimaging moved atoms
+    ref = rst.compute_binding_energy_frames(top, fixed / 10.0, rec, lig, pb_params=pb)
+    assert len(res.frames()) == 3
+    for a, b in zip(res.frames(), ref.frames()):
+        for k in KEYS:
+            assert getattr(a, k) == pytest.approx(getattr(b, k), rel=1e-12, abs=1e-9), k
+    # the complex coordinates the driver kept of the last frame it processed
+    s = rst._ctx(top).split(rec, lig)
+    lib = rst._abi.load_library()
+    out = (rst._abi.FrameTermsC * 3)()
+    last = np.zeros((top.n_atoms, 3))
+    assert lib.mmpbsa_trajectory_frames(s, out, 3, last.ctypes.data_as(rst._abi.c_double_p)) == 0
+    assert np.array_equal(last, fixed[-1])
