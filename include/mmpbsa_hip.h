@@ -230,6 +230,20 @@ int mmpbsa_gb_energy_cached(mmpbsa_ctx *ctx, mmpbsa_gb_cache *cache, const doubl
                             const mmpbsa_gb_params *p, double *born_radii_out, double *total_out,
                             int32_t *reused);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mmpbsa_gb_energy with every temporary (Born-radius
+ * partial sums, the radii unless d_born_radii_out is given, energy partials)
+ * in one fixed 16 MiB static device array per GPU instead of the library's
+ * pool; the call allocates nothing.  Inputs and outputs are device pointers
+ * on the context's device: d_xyz [n_atoms][3] Angstrom, d_born_radii_out
+ * [n_atoms] (nullable), d_total_out [1].  Results equal mmpbsa_gb_energy's.
+ * MMPBSA_ERR_TOO_LARGE when the temporaries exceed the array (about 200,000
+ * atoms).  All calls on one GPU share the array: calls that overlap on one
+ * GPU race. */
+int mmpbsa_gb_energy_noalloc(mmpbsa_ctx *ctx, const double *d_xyz, const mmpbsa_gb_params *p,
+                             double *d_born_radii_out, double *d_total_out);
+
 /* ---- receptor / ligand split (binding.rs:142-211, subsystem.rs:22-169) -- */
 typedef struct mmpbsa_split mmpbsa_split;
 

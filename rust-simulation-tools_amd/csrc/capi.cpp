@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
+#include <map>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -419,6 +420,8 @@ void mmpbsa_server::loop()
 
 namespace {
 
+unsigned char *scratch_on(int device);  // the allocation-free GB entry point's array
+
 void check_xyz(const double *xyz) {
     if (!xyz) throw DeviceError(MMPBSA_ERR_INVALID_ARG, "null coordinates");
 }
@@ -439,6 +442,15 @@ int mmpbsa_init(int32_t *n_devices)
         }
         if (n_devices) *n_devices = n;
         require_device();
+        // set-up of mmpbsa_gb_energy_noalloc's scratch on every gfx950 device
+        for (int d = 0; d < n; ++d) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, d) != hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
+            if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) scratch_on(d);
+        }
     });
 }
 
@@ -1365,6 +1377,106 @@ int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes,
     if (alg_bytes) *alg_bytes = p.alg_bytes;
     if (grid_threads) *grid_threads = p.grid_threads;
     return MMPBSA_OK;
+}
+
+}  // extern "C"
+
+// ---- GB energy without device allocation ------------------------------------
+// Every temporary of mmpbsa_gb_energy (Born-radius partial sums, the radii
+// when the caller does not want them, energy partials) lives in one fixed
+// device array per GPU.  Host code never uses the array's name as an
+// address: hipGetSymbolAddress gives each device's own copy once, and only
+// that pointer is passed to kernels.
+namespace {
+
+constexpr size_t kNoallocScratchBytes = (size_t)16 << 20;
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char *> g_scratch_ptr;  // device -> address of its scratch copy
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+// Looks up (once per device, with that device current) the device address of
+// `scratch`; throws on failure.
+unsigned char *scratch_on(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) return it->second;
+    int prev = 0;
+    HIP_OK(hipGetDevice(&prev));
+    HIP_OK(hipSetDevice(device));
+    unsigned char *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        throw DeviceError(MMPBSA_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                                 hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = ptr;
+    return ptr;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of scratch a GB energy of n atoms needs (radii included when the
+// caller passes no output for them).
+size_t gb_noalloc_bytes(int n, bool radii_in_scratch)
+{
+    const size_t nb = (size_t)std::max(1, ceil_div(n, kTile));
+    return align256(sizeof(double) * ((size_t)gb_chunks(n) * n + 1)) +
+           (radii_in_scratch ? align256(sizeof(double) * ((size_t)n + 1)) : 0) +
+           align256(sizeof(double) * nb * nb);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mmpbsa_gb_energy_noalloc(mmpbsa_ctx *ctx, const double *d_xyz, const mmpbsa_gb_params *p,
+                             double *d_born_radii_out, double *d_total_out)
+{
+    if (!ctx || !p || !d_xyz || !d_total_out)
+        return fail(ctx, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        const PairTopo v = ctx->top.view();
+        const size_t need = gb_noalloc_bytes(v.n, d_born_radii_out == nullptr);
+        if (need > kNoallocScratchBytes)
+            throw DeviceError(MMPBSA_ERR_TOO_LARGE,
+                              "GB energy of " + std::to_string(v.n) + " atoms needs " +
+                                  std::to_string(need) + " bytes of scratch, above the " +
+                                  std::to_string(kNoallocScratchBytes) + "-byte maximum");
+        unsigned char *base = scratch_on(ctx->device);
+        Worker &w = *ctx->w;
+        if (v.n == 0) {
+            launch_fill_f64(d_total_out, 0.0, 1, w.stream);
+            w.sync();
+            return;
+        }
+        size_t off = 0;
+        double *psi = (double *)(base + off);
+        off += align256(sizeof(double) * ((size_t)gb_chunks(v.n) * v.n + 1));
+        double *born = d_born_radii_out;
+        if (!born) {
+            born = (double *)(base + off);
+            off += align256(sizeof(double) * ((size_t)v.n + 1));
+        }
+        double *part = (double *)(base + off);
+        // the same launches as mmpbsa_gb_energy (run_gb): every scratch slot
+        // read is written first in this call
+        launch_gb_radii(d_xyz, v, p->model, p->offset, p->rgbmax, p->cutoff, psi, born, w.stream);
+        const double kappa = compute_kappa(p->salt_concentration, p->solvent_dielectric,
+                                           p->temperature);
+        launch_gb_energy(d_xyz, v, born, kappa, p->solute_dielectric, p->solvent_dielectric,
+                         p->cutoff, part, w.stream);
+        launch_sum_partials(part, gb_energy_blocks(v.n), 1, 1, d_total_out, w.stream);
+        w.sync();
+    });
 }
 
 }  // extern "C"

@@ -18,6 +18,7 @@ c_int64_p = C.POINTER(C.c_int64)
 
 MMPBSA_OK = 0
 MMPBSA_ERR_NOT_READY = 6
+MMPBSA_ERR_TOO_LARGE = 7
 ERRORS = {
     1: "invalid argument",
     2: "residue index out of range",
@@ -222,6 +223,8 @@ SIGNATURES = {
     "mmpbsa_gb_cache_destroy": (C.c_int, [P]),
     "mmpbsa_gb_energy_cached": (C.c_int, [P, P, c_double_p, C.POINTER(GbParamsC), c_double_p,
                                           c_double_p, c_int32_p]),
+    "mmpbsa_gb_energy_noalloc": (C.c_int, [P, C.c_void_p, C.POINTER(GbParamsC), C.c_void_p,
+                                           C.c_void_p]),
     "mmpbsa_split_create": (C.c_int, [P, c_int32_p, C.c_int32, c_int32_p, C.c_int32, C.POINTER(P)]),
     "mmpbsa_split_destroy": (C.c_int, [P]),
     "mmpbsa_split_atoms": (C.c_int, [P, C.c_int32, c_int32_p, C.c_int32, c_int32_p]),
