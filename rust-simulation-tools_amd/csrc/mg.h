@@ -96,8 +96,12 @@ int smooth_zc(int nz);             // z-chunk length for a level of nz planes
 // max_blocks: the largest smooth_tiles * ceil(nz / zc) among the solves
 // C (one coarse level per solve, or null): add the coarse correction's
 // prolongation to the source while loading it (first post-smoothing pass).
+// Every batched launcher takes an optional work list wl (device memory,
+// {n, pre[0..n], id[0..n-1]}: solve id[k] owns blocks pre[k] .. pre[k+1]-1);
+// with a list, the block-count argument is the list's total pre[n] and the
+// grid is one-dimensional, without one it is that count times n_solves.
 void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
-               int max_blocks, const MgStatus *st, hipStream_t s);
+               int max_blocks, const MgStatus *st, hipStream_t s, const uint32_t *wl = nullptr);
 // Temporally blocked smoother: one pass = three red-black sweeps (a whole
 // smoothing phase of the V-cycle), same buffers and flags as mg_smooth, with
 // smooth3_blocks workgroups per solve, L.zc from smooth3_zc.  The result
@@ -106,29 +110,29 @@ int smooth3_tiles(int nx, int ny);
 int smooth3_blocks(int nx, int ny, int nz, int zc);
 int smooth3_zc(int nx, int ny, int nz, int n_solves);
 void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
-                int max_blocks, const MgStatus *st, hipStream_t s);
+                int max_blocks, const MgStatus *st, hipStream_t s, const uint32_t *wl = nullptr);
 // z-streaming version of prolong_e2 (same results bit for bit): one thread
 // per in-plane node over L.zc planes, mg_stream_blocks workgroups per solve.
 int mg_stream_blocks(int nx, int ny, int nz, int zc);
 void mg_prolong_e2s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
-                    const MgStatus *st, hipStream_t s);
+                    const MgStatus *st, hipStream_t s, const uint32_t *wl = nullptr);
 // Residual of e2 and its restriction into C.f in one pass (no residual
 // array), equal bit for bit to mg_residual32 then mg_restrict at the interior
 // coarse nodes (the boundary's right-hand sides are never read, nor written
 // here); mg_rr_blocks(coarse dims, C.zc) workgroups per solve.
 int mg_rr_blocks(int cnx, int cny, int cnz, int czc);
 void mg_residual_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
-                          const MgStatus *st, hipStream_t s);
+                          const MgStatus *st, hipStream_t s, const uint32_t *wl = nullptr);
 // e2 += prolongation of C's correction (post-smoothing input of smooth3)
 void mg_prolong_e2(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_nodes,
-                   const MgStatus *st, hipStream_t s);
+                   const MgStatus *st, hipStream_t s, const uint32_t *wl = nullptr);
 void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
-                   hipStream_t s);
+                   hipStream_t s, const uint32_t *wl = nullptr);
 void mg_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_coarse,
-                 const MgStatus *st, hipStream_t s);
+                 const MgStatus *st, hipStream_t s, const uint32_t *wl = nullptr);
 void mg_tail(const MgTailDev *T, int n_solves, const MgStatus *st, hipStream_t s);
 void mg_outer64(const MgOuterDev *O, int n_solves, int64_t max_nodes, const MgStatus *st,
-                hipStream_t s);
+                hipStream_t s, const uint32_t *wl = nullptr);
 void mg_check(const MgOuterDev *O, int n_solves, int count_cycle, MgStatus *st, hipStream_t s);
 void mg_status_init(MgStatus *st, const MgStatus *init_host, int n, hipStream_t s);
 void mg_inject(const MgLevelDev &F, const MgLevelDev &C, hipStream_t s);  // host-side structs

@@ -28,17 +28,35 @@ namespace mmpbsa {
 
 __device__ __forceinline__ bool stopped(const MgStatus *st) { return st->state != 0; }
 
-// (solve, block) of a batched launch (grid = blocks x solves), XCD-ordered so
-// that each XCD walks contiguous blocks of one solve and z-neighbour planes
-// are still in its L2.
+// (solve, block) of a batched launch, XCD-ordered so that each XCD walks
+// contiguous blocks of one solve and z-neighbour planes are still in its L2.
+// Without a work list the grid is blocks x solves (every solve gets the
+// largest solve's block count; the surplus returns at once).  With a work
+// list wl = {n, pre[0..n], id[0..n-1]} the grid is one-dimensional with
+// pre[n] blocks: blocks pre[k] .. pre[k+1]-1 belong to solve id[k], found by a
+// binary search that is uniform across the workgroup (scalar loads).
 struct BatchBlock {
     int s, b;
 };
-__device__ __forceinline__ BatchBlock batch_block()
+__device__ __forceinline__ BatchBlock batch_block(const uint32_t *__restrict__ wl)
 {
-    const uint32_t flat = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const uint32_t s = flat / gridDim.x;
-    return {(int)s, (int)(flat - s * gridDim.x)};
+    if (!wl) {
+        const uint32_t flat =
+            xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+        const uint32_t s = flat / gridDim.x;
+        return {(int)s, (int)(flat - s * gridDim.x)};
+    }
+    const uint32_t flat = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t n = wl[0];
+    uint32_t lo = 0, hi = n;  // wl[1 + lo] <= flat < wl[1 + hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (wl[1 + mid] <= flat)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return {(int)wl[n + 2 + lo], (int)(flat - wl[1 + lo])};
 }
 
 // ---- node operations shared by the batched kernels and the tail --------------
@@ -283,13 +301,13 @@ __device__ __forceinline__ float face_sel(uint32_t w, int bit, uint32_t cin, uin
 template <bool FIRST, bool PROLONG>
 __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restrict__ Ls,
                                                      const MgLevelDev *__restrict__ Cs, int dir,
-                                                     const MgStatus *__restrict__ st)
+                                                     const MgStatus *__restrict__ st,
+                                                     const uint32_t *__restrict__ wl)
 {
     using namespace sm;
     __shared__ float ring[RING * RN];
-    const uint32_t flat = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int s = (int)(flat / gridDim.x);
-    const int b = (int)(flat - (uint32_t)s * gridDim.x);
+    const BatchBlock bb = batch_block(wl);
+    const int s = bb.s, b = bb.b;
     if (stopped(st + s)) return;
     const MgLevelDev L = Ls[s];
     if (L.nx == 0) return;
@@ -498,17 +516,24 @@ __global__ __launch_bounds__(256) void smooth_kernel(const MgLevelDev *__restric
     }
 }
 
+// Batched launch grid: blocks x solves, or the work list's total as a 1-D grid.
+static inline dim3 batch_grid(int64_t blocks, int n_solves, const uint32_t *wl)
+{
+    return wl ? dim3((unsigned)blocks) : dim3((unsigned)blocks, (unsigned)n_solves);
+}
+
 void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
-               int max_blocks, const MgStatus *st, hipStream_t s)
+               int max_blocks, const MgStatus *st, hipStream_t s, const uint32_t *wl)
 {
     if (n_solves <= 0 || max_blocks <= 0) return;
-    const dim3 grid(max_blocks, n_solves);
+    const dim3 grid = batch_grid(max_blocks, n_solves, wl);
     if (first)
-        hipLaunchKernelGGL((smooth_kernel<true, false>), grid, dim3(256), 0, s, L, C, dir, st);
+        hipLaunchKernelGGL((smooth_kernel<true, false>), grid, dim3(256), 0, s, L, C, dir, st, wl);
     else if (C)
-        hipLaunchKernelGGL((smooth_kernel<false, true>), grid, dim3(256), 0, s, L, C, dir, st);
+        hipLaunchKernelGGL((smooth_kernel<false, true>), grid, dim3(256), 0, s, L, C, dir, st, wl);
     else
-        hipLaunchKernelGGL((smooth_kernel<false, false>), grid, dim3(256), 0, s, L, C, dir, st);
+        hipLaunchKernelGGL((smooth_kernel<false, false>), grid, dim3(256), 0, s, L, C, dir, st,
+                           wl);
     LAUNCH_OK();
 }
 
@@ -619,7 +644,8 @@ __device__ __forceinline__ void bstore64(__amdgpu_buffer_rsrc_t r, double v, uin
 template <bool FIRST, bool PROLONG>
 __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__restrict__ Ls,
                                                       const MgLevelDev *__restrict__ Cs, int dir,
-                                                      const MgStatus *__restrict__ st)
+                                                      const MgStatus *__restrict__ st,
+                                                      const uint32_t *__restrict__ wl)
 {
     using namespace s3;
     __shared__ float ring[RING * RN];
@@ -628,9 +654,8 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     __shared__ float4 fac[2 * (TAB + NW * SLOTS)];
     float4 *const tab = fac;
     float4 *const spc = fac + 2 * TAB;
-    const uint32_t flat = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int s = (int)(flat / gridDim.x);
-    const int b = (int)(flat - (uint32_t)s * gridDim.x);
+    const BatchBlock bb = batch_block(wl);
+    const int s = bb.s, b = bb.b;
     if (stopped(st + s)) return;
     const MgLevelDev L = Ls[s];
     if (L.nx == 0) return;
@@ -1118,9 +1143,10 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
 // e2 += prolongation of the coarse correction (the start of post-smoothing)
 __global__ __launch_bounds__(256) void prolong_e2_kernel(const MgLevelDev *__restrict__ Fs,
                                                          const MgLevelDev *__restrict__ Cs,
-                                                         const MgStatus *__restrict__ st)
+                                                         const MgStatus *__restrict__ st,
+                                                         const uint32_t *__restrict__ wl)
 {
-    const BatchBlock bb = batch_block();
+    const BatchBlock bb = batch_block(wl);
     const int s = bb.s;
     if (stopped(st + s)) return;
     MgLevelDev F = Fs[s];
@@ -1133,25 +1159,27 @@ __global__ __launch_bounds__(256) void prolong_e2_kernel(const MgLevelDev *__res
 }
 
 void mg_prolong_e2(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_nodes,
-                   const MgStatus *st, hipStream_t s)
+                   const MgStatus *st, hipStream_t s, const uint32_t *wl)
 {
     if (n_solves <= 0 || max_nodes <= 0) return;
-    const dim3 grid(ceil_div(max_nodes, 256), n_solves);
-    hipLaunchKernelGGL(prolong_e2_kernel, grid, dim3(256), 0, s, F, C, st);
+    const dim3 grid = batch_grid(wl ? max_nodes : ceil_div(max_nodes, 256), n_solves, wl);
+    hipLaunchKernelGGL(prolong_e2_kernel, grid, dim3(256), 0, s, F, C, st, wl);
     LAUNCH_OK();
 }
 
 void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
-                int max_blocks, const MgStatus *st, hipStream_t s)
+                int max_blocks, const MgStatus *st, hipStream_t s, const uint32_t *wl)
 {
     if (n_solves <= 0 || max_blocks <= 0) return;
-    const dim3 grid(max_blocks, n_solves);
+    const dim3 grid = batch_grid(max_blocks, n_solves, wl);
     // the prolongation runs as its own kernel (mg_prolong_e2) before this pass
     if (C) throw DeviceError(1, "mg_smooth3: no fused prolongation");  // invalid argument
     if (first)
-        hipLaunchKernelGGL((smooth3_kernel<true, false>), grid, dim3(s3::NT), 0, s, L, C, dir, st);
+        hipLaunchKernelGGL((smooth3_kernel<true, false>), grid, dim3(s3::NT), 0, s, L, C, dir, st,
+                           wl);
     else
-        hipLaunchKernelGGL((smooth3_kernel<false, false>), grid, dim3(s3::NT), 0, s, L, C, dir, st);
+        hipLaunchKernelGGL((smooth3_kernel<false, false>), grid, dim3(s3::NT), 0, s, L, C, dir, st,
+                           wl);
     LAUNCH_OK();
 }
 
@@ -1194,9 +1222,10 @@ __device__ __forceinline__ bool stream_pos(int nx, int ny, int nz, int zc, int b
 // prolong_e2_kernel: e2 += prolongation of the coarse correction
 __global__ __launch_bounds__(256) void prolong_e2s_kernel(const MgLevelDev *__restrict__ Fs,
                                                           const MgLevelDev *__restrict__ Cs,
-                                                          const MgStatus *__restrict__ st)
+                                                          const MgStatus *__restrict__ st,
+                                                          const uint32_t *__restrict__ wl)
 {
-    const BatchBlock bb = batch_block();
+    const BatchBlock bb = batch_block(wl);
     if (stopped(st + bb.s)) return;
     const MgLevelDev F = Fs[bb.s];
     if (F.nx == 0) return;
@@ -1312,14 +1341,15 @@ int mg_rr_blocks(int cnx, int cny, int cnz, int czc)
 
 __global__ __launch_bounds__(256, 4) void resrestrict_kernel(const MgLevelDev *__restrict__ Fs,
                                                           const MgLevelDev *__restrict__ Cs,
-                                                          const MgStatus *__restrict__ st)
+                                                          const MgStatus *__restrict__ st,
+                                                          const uint32_t *__restrict__ wl)
 {
     using namespace rr;
     using s3::kOob;
     __shared__ float ering[4 * ER * ER];
     __shared__ float rpl[2 * FR * RS];
     __shared__ float4 tabr[2 * 128];  // code byte bits 0-6 -> {a0..a3}, {a4, a5, diag, -}
-    const BatchBlock bb = batch_block();
+    const BatchBlock bb = batch_block(wl);
     if (stopped(st + bb.s)) return;
     const MgLevelDev C = Cs[bb.s];
     if (C.nx == 0) return;
@@ -1553,26 +1583,29 @@ __global__ __launch_bounds__(256, 4) void resrestrict_kernel(const MgLevelDev *_
 }
 
 void mg_residual_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
-                          const MgStatus *st, hipStream_t s)
+                          const MgStatus *st, hipStream_t s, const uint32_t *wl)
 {
     if (n_solves <= 0 || max_blocks <= 0) return;
-    hipLaunchKernelGGL(resrestrict_kernel, dim3(max_blocks, n_solves), dim3(256), 0, s, F, C, st);
+    hipLaunchKernelGGL(resrestrict_kernel, batch_grid(max_blocks, n_solves, wl), dim3(256), 0, s,
+                       F, C, st, wl);
     LAUNCH_OK();
 }
 
 void mg_prolong_e2s(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int max_blocks,
-                    const MgStatus *st, hipStream_t s)
+                    const MgStatus *st, hipStream_t s, const uint32_t *wl)
 {
     if (n_solves <= 0 || max_blocks <= 0) return;
-    hipLaunchKernelGGL(prolong_e2s_kernel, dim3(max_blocks, n_solves), dim3(256), 0, s, F, C, st);
+    hipLaunchKernelGGL(prolong_e2s_kernel, batch_grid(max_blocks, n_solves, wl), dim3(256), 0, s,
+                       F, C, st, wl);
     LAUNCH_OK();
 }
 int mg_stream_blocks(int nx, int ny, int nz, int zc) { return stream_blocks(nx, ny, nz, zc); }
 
 __global__ __launch_bounds__(256) void residual32_kernel(const MgLevelDev *__restrict__ Ls,
-                                                         const MgStatus *__restrict__ st)
+                                                         const MgStatus *__restrict__ st,
+                                                         const uint32_t *__restrict__ wl)
 {
-    const BatchBlock bb = batch_block();
+    const BatchBlock bb = batch_block(wl);
     const int s = bb.s;
     if (stopped(st + s)) return;
     MgLevelDev L = Ls[s];
@@ -1585,9 +1618,10 @@ __global__ __launch_bounds__(256) void residual32_kernel(const MgLevelDev *__res
 
 __global__ __launch_bounds__(256) void restrict_kernel(const MgLevelDev *__restrict__ Fs,
                                                        const MgLevelDev *__restrict__ Cs,
-                                                       const MgStatus *__restrict__ st)
+                                                       const MgStatus *__restrict__ st,
+                                                       const uint32_t *__restrict__ wl)
 {
-    const BatchBlock bb = batch_block();
+    const BatchBlock bb = batch_block(wl);
     const int s = bb.s;
     if (stopped(st + s)) return;
     const MgLevelDev C = Cs[s];
@@ -1679,10 +1713,11 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
 // ---- FP64 outer step: phi_out = phi_in + e, f0 = b - A phi_out (stencil_at,
 // pb_solver.rs:159-194), block partial sums of r^2.
 __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restrict__ Os,
-                                                      const MgStatus *__restrict__ st)
+                                                      const MgStatus *__restrict__ st,
+                                                      const uint32_t *__restrict__ wl)
 {
     __shared__ double red[4];
-    const BatchBlock bb = batch_block();
+    const BatchBlock bb = batch_block(wl);
     const int s = bb.s;
     if (stopped(st + s)) return;
     const MgOuterDev O = Os[s];
@@ -1964,18 +1999,20 @@ __global__ __launch_bounds__(256) void sor_check_kernel(const double *__restrict
 static inline unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_div(n, 256)); }
 
 void mg_residual32(const MgLevelDev *L, int n_solves, int64_t max_nodes, const MgStatus *st,
-                   hipStream_t s)
+                   hipStream_t s, const uint32_t *wl)
 {
     if (max_nodes <= 0 || n_solves <= 0) return;
-    hipLaunchKernelGGL(residual32_kernel, dim3(nblk(max_nodes), n_solves), dim3(256), 0, s, L, st);
+    hipLaunchKernelGGL(residual32_kernel, batch_grid(wl ? max_nodes : nblk(max_nodes), n_solves, wl),
+                       dim3(256), 0, s, L, st, wl);
     LAUNCH_OK();
 }
 
 void mg_restrict(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64_t max_coarse,
-                 const MgStatus *st, hipStream_t s)
+                 const MgStatus *st, hipStream_t s, const uint32_t *wl)
 {
     if (max_coarse <= 0 || n_solves <= 0) return;
-    hipLaunchKernelGGL(restrict_kernel, dim3(nblk(max_coarse), n_solves), dim3(256), 0, s, F, C, st);
+    hipLaunchKernelGGL(restrict_kernel, batch_grid(wl ? max_coarse : nblk(max_coarse), n_solves, wl),
+                       dim3(256), 0, s, F, C, st, wl);
     LAUNCH_OK();
 }
 
@@ -1987,10 +2024,11 @@ void mg_tail(const MgTailDev *T, int n_solves, const MgStatus *st, hipStream_t s
 }
 
 void mg_outer64(const MgOuterDev *O, int n_solves, int64_t max_nodes, const MgStatus *st,
-                hipStream_t s)
+                hipStream_t s, const uint32_t *wl)
 {
-    if (n_solves <= 0) return;
-    hipLaunchKernelGGL(outer64_kernel, dim3(nblk(max_nodes), n_solves), dim3(256), 0, s, O, st);
+    if (n_solves <= 0 || (wl && max_nodes <= 0)) return;
+    hipLaunchKernelGGL(outer64_kernel, batch_grid(wl ? max_nodes : nblk(max_nodes), n_solves, wl),
+                       dim3(256), 0, s, O, st, wl);
     LAUNCH_OK();
 }
 

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 
 #include "engine.h"
 
@@ -612,8 +613,72 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     MgStatus *dst = (MgStatus *)(dbase + off);
     HIP_OK(hipMemcpyAsync(dbase, hb.data(), total, hipMemcpyHostToDevice, w.stream));
 
+    // ---- work lists: per launch kind and level, the solves that still run
+    // and their block counts (a launch's grid is their sum, not the largest
+    // solve's count times S).  kinds: 0 smoothing pass, 1 prolongation,
+    // 2 residual + restriction (per-node residual without streaming), 3
+    // per-node restriction; list WL_OUT: the FP64 outer step.
+    constexpr int kKinds = 4;
+    const int WL_OUT = kKinds * maxT;
+    auto kind_blocks = [&](int kind, int l, int s) -> int64_t {
+        const TaskWork &tw = W[s];
+        if (kind < 0) return ceil_div(lv_nodes(tw.t->g), 256);
+        if (l >= tw.tail_start) return 0;
+        const MgLevelDev &d = tw.L[l].d;
+        const GridDesc &g = tw.L[l].g;
+        switch (kind) {
+        case 0:
+            return phase_pass ? smooth3_blocks(g.nx, g.ny, g.nz, d.zc)
+                              : (int64_t)smooth_tiles(g.nx, g.ny) * ceil_div(g.nz, d.zc);
+        case 1:
+            return stream ? mg_stream_blocks(g.nx, g.ny, g.nz, d.zc) : ceil_div(lv_nodes(g), 256);
+        case 2: {
+            const GridDesc &gc = tw.L[l + 1].g;
+            return stream ? mg_rr_blocks(gc.nx, gc.ny, gc.nz, tw.L[l + 1].d.zc)
+                          : ceil_div(lv_nodes(g), 256);
+        }
+        default:
+            return stream ? 0 : ceil_div(lv_nodes(tw.L[l + 1].g), 256);
+        }
+    };
+    struct WlRef {
+        size_t off = 0;
+        int64_t total = 0;
+    };
+    std::vector<WlRef> wref(WL_OUT + 1);
+    std::vector<uint32_t> hwl;
+    DBuf<uint32_t> dwl(w.pool, (size_t)(WL_OUT + 1) * (2 * (size_t)S + 2));
+    std::vector<char> active(S, 1);
+    auto build_lists = [&]() {
+        hwl.clear();
+        for (int li = 0; li <= WL_OUT; ++li) {
+            const int kind = li == WL_OUT ? -1 : li / std::max(1, maxT);
+            const int l = li == WL_OUT ? 0 : li % std::max(1, maxT);
+            std::vector<uint32_t> ids, pre(1, 0);
+            int64_t acc = 0;
+            for (int s = 0; s < S; ++s) {
+                const int64_t b = active[s] ? kind_blocks(kind, l, s) : 0;
+                if (b <= 0) continue;
+                ids.push_back((uint32_t)s);
+                acc += b;
+                pre.push_back((uint32_t)acc);
+            }
+            if (acc >= ((int64_t)1 << 31)) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "launch above 2^31 blocks");
+            wref[li].off = hwl.size();
+            wref[li].total = acc;
+            hwl.push_back((uint32_t)ids.size());
+            hwl.insert(hwl.end(), pre.begin(), pre.end());
+            hwl.insert(hwl.end(), ids.begin(), ids.end());
+        }
+        HIP_OK(hipMemcpyAsync(dwl.get(), hwl.data(), sizeof(uint32_t) * hwl.size(),
+                              hipMemcpyHostToDevice, w.stream));
+    };
+    build_lists();
+    auto wl_of = [&](int li) { return dwl.get() + wref[li].off; };
+    auto tot_of = [&](int li) { return wref[li].total; };
+
     // cycle 0: residual of the boundary-only potential (no test)
-    mg_outer64(dO0, S, maxN0, dst, w.stream);
+    mg_outer64(dO0, S, tot_of(WL_OUT), dst, w.stream, wl_of(WL_OUT));
     mg_check(dO0, S, 0, dst, w.stream);
 
     // profiled level-0 smoother passes: events, cycle number, pass reads e,
@@ -634,10 +699,11 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             // pre: from zero into e2; post: e2 plus the prolongated coarse
             // correction (added by its own kernel) into e
             if (!pre) {
+                const int li = 1 * maxT + l;
                 if (stream)
-                    mg_prolong_e2s(F, C, S, strblk[l], dst, w.stream);
+                    mg_prolong_e2s(F, C, S, (int)tot_of(li), dst, w.stream, wl_of(li));
                 else
-                    mg_prolong_e2(F, C, S, maxnodes[l], dst, w.stream);
+                    mg_prolong_e2(F, C, S, tot_of(li), dst, w.stream, wl_of(li));
             }
             // the events bracket the smoother launch alone
             Ev ev{nullptr, nullptr, cycle_counter, !pre, false};
@@ -646,7 +712,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 HIP_OK(hipEventCreate(&ev.b));
                 HIP_OK(hipEventRecord(ev.a, w.stream));
             }
-            mg_smooth3(F, nullptr, S, pre ? 1 : 0, pre ? 1 : 0, sblocks[l], dst, w.stream);
+            mg_smooth3(F, nullptr, S, pre ? 1 : 0, pre ? 1 : 0, (int)tot_of(l), dst, w.stream,
+                       wl_of(l));
             if (profiling && l == 0) {
                 HIP_OK(hipEventRecord(ev.b, w.stream));
                 evs.push_back(ev);
@@ -663,7 +730,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 HIP_OK(hipEventCreate(&ev.b));
                 HIP_OK(hipEventRecord(ev.a, w.stream));
             }
-            mg_smooth(F, prolong ? C : nullptr, S, first ? 1 : 0, dir, sblocks[l], dst, w.stream);
+            mg_smooth(F, prolong ? C : nullptr, S, first ? 1 : 0, dir, (int)tot_of(l), dst,
+                      w.stream, wl_of(l));
             if (profiling && l == 0) {
                 HIP_OK(hipEventRecord(ev.b, w.stream));
                 evs.push_back(ev);
@@ -676,11 +744,12 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const MgLevelDev *F = D[p].F + (size_t)l * S;
             const MgLevelDev *C = D[p].C + (size_t)l * S;
             smooth_phase(F, C, l, true);
+            const int li = 2 * maxT + l;
             if (stream) {
-                mg_residual_restrict(F, C, S, rrblk[l], dst, w.stream);
+                mg_residual_restrict(F, C, S, (int)tot_of(li), dst, w.stream, wl_of(li));
             } else {
-                mg_residual32(F, S, maxnodes[l], dst, w.stream);
-                mg_restrict(F, C, S, maxcoarse[l], dst, w.stream);
+                mg_residual32(F, S, tot_of(li), dst, w.stream, wl_of(li));
+                mg_restrict(F, C, S, tot_of(3 * maxT + l), dst, w.stream, wl_of(3 * maxT + l));
             }
         }
         mg_tail(D[p].T, S, dst, w.stream);
@@ -689,7 +758,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const MgLevelDev *C = D[p].C + (size_t)l * S;
             smooth_phase(F, C, l, false);
         }
-        mg_outer64(D[p].O, S, maxN0, dst, w.stream);
+        mg_outer64(D[p].O, S, tot_of(WL_OUT), dst, w.stream, wl_of(WL_OUT));
         mg_check(D[p].O, S, 1, dst, w.stream);
     };
 
@@ -698,7 +767,13 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     std::vector<MgStatus> hs(S);
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    if (use_graph) {
+    auto drop_graph = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    };
+    auto capture = [&]() {
         std::shared_lock<std::shared_mutex> gate(capture_gate());
         HIP_OK(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
         try {
@@ -712,7 +787,15 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         }
         HIP_OK(hipStreamEndCapture(w.stream, &graph));
         HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
+    };
+    struct GraphGuard {
+        std::function<void()> f;
+        ~GraphGuard() { f(); }
+    } graph_guard{drop_graph};
+    if (use_graph) capture();
+    // blocks of the FP64 outer step of the running solves when the lists were
+    // last built (a proxy for the level-0 work of every launch)
+    int64_t listed = tot_of(WL_OUT);
     const int chunk_cycles = 8;
     int done_cycles = 0;
     while (done_cycles < maxv) {
@@ -729,9 +812,26 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         bool all = true;
         for (int s = 0; s < S; ++s) all = all && hs[s].state != 0;
         if (all) break;
+        // Once finished solves leave a quarter of the listed level-0 work
+        // idle, rebuild the lists without them (and the graph with the
+        // smaller grids).  The stream is idle here.  Profiled runs keep one
+        // launch size throughout.
+        if (!profiling) {
+            int64_t running = 0;
+            for (int s = 0; s < S; ++s)
+                if (active[s] && hs[s].state == 0) running += kind_blocks(-1, 0, s);
+            if (4 * running <= 3 * listed) {
+                for (int s = 0; s < S; ++s) active[s] = active[s] && hs[s].state == 0;
+                build_lists();
+                listed = tot_of(WL_OUT);
+                if (use_graph) {
+                    drop_graph();
+                    capture();
+                }
+            }
+        }
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     if (profiling && !evs.empty()) {
         // algorithmic bytes of one pass over a level-0 solve, per node: the
         // correction read (unless starting from zero) and written (the
@@ -758,7 +858,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         prof.total_ms += ms_tot;
         prof.launches += (int64_t)evs.size();
         prof.alg_bytes += bytes;
-        prof.grid_threads = (int64_t)sblocks[0] * S * 256;
+        prof.grid_threads = tot_of(0) * 256;
     }
     for (int s = 0; s < S; ++s) {
         TaskWork &tw = W[s];
