@@ -1,22 +1,37 @@
 """MM-PBSA frames/s on one MI355X.
 
-Workload: the 25,000-atom synthetic complex (23,500 receptor + 1,500 ligand
-atoms, 45 A sphere), frames = base + N(0, 0.3^2) A noise, PB on a 161^3 grid
-at 0.5 A (eps_in 4, eps_out 80, 0.15 M, two-level focusing), MM and SA for the
-complex, receptor and ligand of every frame.  The frames are placed in HBM
-before the timed region; one step = a batch of frames through the batched
-path (every PB solve of the batch advances in shared, graph-replayed
-launches).  Beside it: the same steps from host buffers (PCIe included), the
-plain path (one frame at a time, host-synchronous, no graphs), the smoother's
-roofline from HIP events in one extra profiled step of the same workload, the
-2,500-atom complex's batched / plain ratio, and the FP64 CPU oracle on the
-host cores.
+Headline workload: the 25,000-atom synthetic complex (23,500 receptor + 1,500
+ligand atoms, 45 A sphere), frames = base + N(0, 0.3^2) A noise, PB on a 161^3
+grid at 0.5 A (eps_in 4, eps_out 80, 0.15 M, two-level focusing), MM and SA
+for the complex, receptor and ligand of every frame.  The frames are placed in
+HBM before the timed region; one step = a batch of frames through the batched
+path (every PB solve of the batch advances in shared, graph-replayed launches).
+
+Beside it, in the same run:
+  - the same steps from host buffers (PCIe included) and the plain path (one
+    frame at a time, host-synchronous, no graphs);
+  - every term's error against the FP64 oracle (13 frame terms, Born radii,
+    the oracle's own convergence) and the oracle timed on the host cores;
+  - the smoother's roofline from HIP events in one extra profiled step, with
+    hardware traffic from two rocprofv3 counter runs of that step; rooflines
+    of the pair kernels and the face-dielectric kernel from HIP events;
+  - the same complex by MM-GBSA (GB polar term), batched;
+  - the 2,500-atom complex's batched / plain ratio;
+  - uneven batching (8 complexes, 3k-60k atoms, mixed vs one at a time);
+  - the many-complex workload (64 complexes, 3k-60k atoms, own auto_grid
+    grids) through the serving loop;
+  - Poisson-arrival serving of complexes of 5k-100k atoms with 1-100 frames;
+  - the 150,000-atom complex with focusing 193^3 -> 321^3 and per-residue
+    decomposition.
 """
 import argparse
+import concurrent.futures as cf
 import csv
 import json
 import math
+import multiprocessing
 import os
+import platform
 import shutil
 import subprocess
 import sys
@@ -34,6 +49,8 @@ import ctypes as C  # noqa: E402
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+FP32_PEAK_TFS = 157.3
 CFG1_PB = dict(grid_spacing=0.5, solute_dielectric=4.0, solvent_dielectric=80.0,
                salt_concentration=0.15, fine_dims=(161, 161, 161))
 CFG0_PB = dict(grid_spacing=0.6, solute_dielectric=4.0, solvent_dielectric=80.0,
@@ -41,31 +58,80 @@ CFG0_PB = dict(grid_spacing=0.6, solute_dielectric=4.0, solvent_dielectric=80.0,
 # many-complex workloads: auto_grid at 0.5 A with a 10 A buffer per subsystem
 AUTO_PB = dict(grid_spacing=0.5, grid_buffer=10.0, solute_dielectric=4.0,
                solvent_dielectric=80.0, salt_concentration=0.15)
+GB_MODEL = 2  # OBC-II
 
 
 def _note(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def _complexes(rst, synthetic, config, sizes, frames, rng):
-    """(topology, frames nm, receptor residues, ligand residues) per size, with
-    a ligand fraction drawn from U(3%, 10%)."""
-    out = []
-    for i, (n, nf) in enumerate(zip(sizes, frames)):
-        seed = synthetic.seed_for(config, i)
-        n_lig = max(16, int(round(n * rng.uniform(0.03, 0.10) / 16.0)) * 16)
-        top, base, rec, lig = synthetic.build_complex(int(n), n_lig, None, seed)
-        f = np.stack([synthetic.noisy_frame(base, seed, k) for k in range(nf)]) / 10.0
-        out.append((top, f, rec, lig))
-    return out
+def cpu_share():
+    """Host cores this job may use: the affinity mask, capped by
+    OMP_NUM_THREADS when the environment sets it (the GPU host allots a share
+    of its cores to each GPU and says so there)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    share = min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff
+    return share, aff
 
 
-def uneven_leg(rst, synthetic):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+# ---- synthetic complexes, built in worker processes -----------------------------
+def _build_one(job):
+    """(config, index, n_atoms, n_ligand, n_frames) -> (topology, frames nm,
+    receptor residues, ligand residues); runs in a spawned worker."""
+    from rust_simulation_tools import synthetic
+    config, idx, n, n_lig, nf = job
+    seed = synthetic.seed_for(config, idx)
+    top, base, rec, lig = synthetic.build_complex(int(n), int(n_lig), None, seed)
+    f = np.stack([synthetic.noisy_frame(base, seed, k) for k in range(nf)]) / 10.0
+    return top, f, rec, lig
+
+
+def build_complexes(jobs, workers):
+    if workers <= 1 or len(jobs) <= 1:
+        return [_build_one(j) for j in jobs]
+    ctx = multiprocessing.get_context("spawn")
+    with cf.ProcessPoolExecutor(max_workers=min(workers, len(jobs)), mp_context=ctx) as ex:
+        return list(ex.map(_build_one, jobs, chunksize=1))
+
+
+def _lig_atoms(n, frac):
+    return max(16, int(round(n * frac / 16.0)) * 16)
+
+
+def _frame_errors(f, o):
+    err = {}
+    for part in ("complex", "receptor", "ligand"):
+        for term in ("mm", "polar", "sa"):
+            err[f"{part}_{term}"] = abs(getattr(f, f"{part}_{term}") - o[part][term])
+    for term in ("delta_mm", "delta_polar", "delta_sa", "delta_total"):
+        err[term] = abs(getattr(f, term) - o[term])
+    return err
+
+
+# ---- legs -------------------------------------------------------------------------
+def uneven_leg(rst, synthetic, workers):
     """Eight complexes, N log-spaced over [3k, 60k], two frames each: all in one
     server (their solves share rounds) against each complex batched alone."""
     rng = np.random.Generator(np.random.PCG64(synthetic.seed_for(2, 900)))
     sizes = np.round(np.exp(np.linspace(math.log(3000), math.log(60000), 8))).astype(int)
-    cx = _complexes(rst, synthetic, 2, sizes, [2] * 8, rng)
+    jobs = [(2, i, int(n), _lig_atoms(int(n), rng.uniform(0.03, 0.10)), 2)
+            for i, n in enumerate(sizes)]
+    cx = build_complexes(jobs, workers)
     pb = rst.PbParams(**AUTO_PB)
     for top, f, rec, lig in cx:  # contexts, splits, first-use costs
         rst.compute_binding_energy_frames(top, f[:1], rec, lig, pb_params=pb)
@@ -87,20 +153,81 @@ def uneven_leg(rst, synthetic):
     nfr = 2 * len(cx)
     return {"workload": "8 complexes, N = " + ", ".join(str(int(n)) for n in sizes) +
                         "; 2 frames each; auto_grid h 0.5 A, buffer 10 A",
-            "mixed_frames_per_s": nfr / mixed, "one_complex_at_a_time_frames_per_s": nfr / alone}
+            "mixed_frames_per_s": nfr / mixed, "one_complex_at_a_time_frames_per_s": nfr / alone,
+            "mixed_over_one_at_a_time": alone / mixed}
 
 
-def serving_leg(rst, synthetic, check_oracle):
-    """Poisson arrivals (50/s) of 24 requests, N log-uniform over [5k, 25k],
-    1-8 frames each, into one server under a 200 GB budget: request latency
-    (submission to last frame) and throughput; one sampled frame is checked
-    against the FP64 oracle."""
-    rng = np.random.Generator(np.random.PCG64(synthetic.seed_for(3, 900)))
-    n_req = 24
-    sizes = np.round(np.exp(rng.uniform(math.log(5000), math.log(25000), n_req))).astype(int)
-    nframes = rng.integers(1, 9, n_req)
+def many_complex_leg(rst, synthetic, n_cx, n_frames, workers, max_round):
+    """n_cx complexes, N log-uniform over [3k, 60k], ligand fraction U(3%, 10%),
+    n_frames noisy frames each, each on its own auto_grid grids (0.5 A, 10 A
+    buffer): every frame submitted at once to one server under a 200 GB
+    budget, which batches frames of different complexes into shared solver
+    launches.  The plain path (one frame at a time) is timed on the first
+    frame of four complexes at the size quartiles."""
+    rng = np.random.Generator(np.random.PCG64(synthetic.seed_for(2, 0)))
+    sizes = np.round(np.exp(rng.uniform(math.log(3000), math.log(60000), n_cx))).astype(int)
+    fracs = rng.uniform(0.03, 0.10, n_cx)
+    jobs = [(2, i, int(n), _lig_atoms(int(n), fr), n_frames)
+            for i, (n, fr) in enumerate(zip(sizes, fracs))]
+    ts = time.perf_counter()
+    cx = build_complexes(jobs, workers)
+    _note(f"many: {n_cx} complexes built in {time.perf_counter() - ts:.1f} s")
+    pb = rst.PbParams(**AUTO_PB)
+    for top, f, rec, lig in cx:  # contexts and splits (GB frame: no PB first-use cost)
+        rst.compute_binding_energy_frames(top, f[:1], rec, lig)
+    order = np.argsort(sizes)
+    quart = [int(order[int(q * (n_cx - 1))]) for q in (0.125, 0.375, 0.625, 0.875)]
+    plain_t = []
+    for k in quart:
+        top, f, rec, lig = cx[k]
+        rst.compute_binding_energy_frames(top, f[:1], rec, lig, pb_params=pb, batched=False)
+        t0 = time.perf_counter()
+        rst.compute_binding_energy_frames(top, f[:1], rec, lig, pb_params=pb, batched=False)
+        plain_t.append(time.perf_counter() - t0)
+    srv = rst.Server(hbm_budget_bytes=200 << 30, max_round_frames=max_round)
+    try:
+        big = int(order[-1])
+        srv.wait(srv.submit(cx[big][0], cx[big][1][:1], cx[big][2], cx[big][3], pb_params=pb))
+        t0 = time.perf_counter()
+        tickets = [srv.submit(top, f, rec, lig, pb_params=pb) for top, f, rec, lig in cx]
+        lat = []
+        for t in tickets:
+            _, st = srv.wait(t)
+            lat.append(st.total_ms)
+        span = time.perf_counter() - t0
+    finally:
+        srv.close()
+    nfr = n_cx * n_frames
+    # the same four frames batched through the server path, for a like-for-like ratio
+    return {"workload": f"{n_cx} complexes, N log-uniform [3k, 60k] (min {int(sizes.min())}, "
+                        f"median {int(np.median(sizes))}, max {int(sizes.max())}), ligand "
+                        f"3-10%, {n_frames} frames each ({nfr} frames), auto_grid h 0.5 A "
+                        f"buffer 10 A, one server, 200 GB budget, rounds of {max_round} frames",
+            "frames": nfr, "frames_per_s": nfr / span, "wall_s": span,
+            "request_latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                   "p99": float(np.percentile(lat, 99))},
+            "plain_sample": {"complexes_atoms": [int(sizes[k]) for k in quart],
+                             "seconds_per_frame": plain_t,
+                             "frames_per_s": len(plain_t) / sum(plain_t)}}
+
+
+def serving_leg(rst, synthetic, n_req, workers, oracle_max_atoms, check_oracle):
+    """Poisson arrivals (50/s) of n_req requests into one server under a 200 GB
+    budget; each request is one complex of N log-uniform over [5k, 100k] atoms
+    with U{1..100} frames.  Request latency (submission to last frame),
+    throughput, and a seeded 5% sample of frames checked against the FP64
+    oracle where the oracle is affordable (complexes up to oracle_max_atoms;
+    the smallest request's first frame always)."""
+    rng = np.random.Generator(np.random.PCG64(synthetic.seed_for(3, 0)))
+    sizes = np.round(np.exp(rng.uniform(math.log(5000), math.log(100000), n_req))).astype(int)
+    nframes = rng.integers(1, 101, n_req)
+    fracs = rng.uniform(0.03, 0.10, n_req)
     arrivals = np.cumsum(rng.exponential(1.0 / 50.0, n_req))
-    cx = _complexes(rst, synthetic, 3, sizes, nframes, rng)
+    jobs = [(3, i, int(n), _lig_atoms(int(n), fr), int(nf))
+            for i, (n, fr, nf) in enumerate(zip(sizes, fracs, nframes))]
+    ts = time.perf_counter()
+    cx = build_complexes(jobs, workers)
+    _note(f"serving: {n_req} complexes built in {time.perf_counter() - ts:.1f} s")
     pb = rst.PbParams(**AUTO_PB)
     for top, f, rec, lig in cx:  # register topologies (contexts, splits) up front
         rst.compute_binding_energy_frames(top, f[:1], rec, lig)
@@ -123,57 +250,107 @@ def serving_leg(rst, synthetic, check_oracle):
         span = time.perf_counter() - t0
     finally:
         srv.close()
-    out = {"workload": f"{n_req} requests, Poisson 50/s, N log-uniform [5k, 25k], 1-8 frames, "
-                       "auto_grid h 0.5 A buffer 10 A, HBM budget 200 GB",
-           "frames": int(nframes.sum()), "frames_per_s": float(nframes.sum()) / span,
+    total = int(nframes.sum())
+    out = {"workload": f"{n_req} requests, Poisson 50/s, N log-uniform [5k, 100k] "
+                       f"(drawn: {', '.join(str(int(n)) for n in sizes)}), frames U{{1..100}} "
+                       f"(drawn: {', '.join(str(int(n)) for n in nframes)}), auto_grid h 0.5 A "
+                       "buffer 10 A, HBM budget 200 GB",
+           "frames": total, "frames_per_s": total / span, "wall_s": span,
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                           "max": float(np.max(lat))}}
     if check_oracle:
         from oracle import oracle as O
-        k = int(np.argmin(sizes))  # a sampled request: the smallest system, its first frame
-        top, f, rec, lig = cx[k]
-        o = O.frame_energy(top, f[0] * 10.0, rec, lig, method=1, pb=O.pb_params(**AUTO_PB))
-        out["sampled_dG_err_vs_fp64_oracle"] = abs(results[k].frames()[0].delta_total -
-                                                   o["delta_total"])
+        srng = np.random.Generator(np.random.PCG64(synthetic.seed_for(3, 1)))
+        pairs = [(k, f) for k in range(n_req) for f in range(int(nframes[k]))]
+        sample = [pairs[i] for i in sorted(srng.choice(len(pairs), max(1, len(pairs) // 20),
+                                                       replace=False))]
+        small = int(np.argmin(sizes))
+        checked = [(k, f) for k, f in sample if sizes[k] <= oracle_max_atoms]
+        if (small, 0) not in checked:
+            checked.insert(0, (small, 0))
+        errs = []
+        for k, f in checked:
+            top, fr, rec, lig = cx[k]
+            o = O.frame_energy(top, fr[f] * 10.0, rec, lig, method=1, pb=O.pb_params(**AUTO_PB))
+            errs.append(max(_frame_errors(results[k].frames()[f], o).values()))
+        out["oracle_check"] = {"sample_frames": len(sample), "checked_frames": len(checked),
+                               "checked": [[int(sizes[k]), int(f)] for k, f in checked],
+                               "rule": f"seeded 5% sample of frames, oracle run on complexes of "
+                                       f"at most {oracle_max_atoms} atoms, plus the smallest "
+                                       "request's first frame",
+                               "max_term_err_kcal": float(max(errs))}
     return out
 
 
-def large_decomposition_leg(rst, synthetic):
+def _lj_coul(top, xa, ia, xb, ib):
+    """FP64 brute force over a x b: sum of A/r^12 - B/r^6 and qa qb / r."""
+    qa = np.asarray(top.charges_amber)
+    ty = np.asarray(top.atom_type_indices)
+    nbi = np.asarray(top.nb_parm_index)
+    A, B = np.asarray(top.lj_acoef), np.asarray(top.lj_bcoef)
+    d = np.linalg.norm(xa[:, None, :] - xb[None, :, :], axis=2)
+    k = nbi[top.n_types * ty[ia][:, None] + ty[ib][None, :]] - 1
+    a = np.where(k >= 0, A[np.maximum(k, 0)], 0.0)
+    b = np.where(k >= 0, B[np.maximum(k, 0)], 0.0)
+    r6 = d ** 6
+    return float((a / (r6 * r6) - b / r6).sum()), float((qa[ia][:, None] * qa[ib][None, :] / d).sum())
+
+
+def large_decomposition_leg(rst, synthetic, n_frames):
     """The 150,000-atom complex (142,500 + 7,500 atoms; 8,907 + 469 residues of
-    16 atoms): one frame's MM-PBSA and its per-residue decomposition with PB
+    16 atoms): per frame its MM-PBSA and its per-residue decomposition with PB
     polar terms, on explicit focusing grids shared by the three subsystems
     (193^3 at 1.0 A centred on the complex, then 321^3 at 0.4 A centred on the
-    interface), beside the GB decomposition of the same frame."""
+    interface), beside the GB decomposition of the first frame.  Errors: the
+    size-independent identities, and the vdW / elec terms of 12 sampled ligand
+    residues against an FP64 brute-force pair sum."""
     top, base, rec, lig, seed = synthetic.config_system(4)
-    x = synthetic.noisy_frame(base, seed, 0)
     _note("large: system built")
     ra, la = top.build_selection(rec), top.build_selection(lig)
+    xn0 = synthetic.noisy_frame(base, seed, 0) / 10.0
+    x0 = xn0 * 10.0
     kw = dict(solute_dielectric=4.0, solvent_dielectric=80.0, salt_concentration=0.15,
-              **synthetic.focusing_grids(x, ra, la))
+              **synthetic.focusing_grids(x0, ra, la))
     pb = rst.PbParams(**kw)
-    xn = x / 10.0
     ts = time.perf_counter()
-    rst.decompose_binding_energy(top, xn, rec, lig)  # topology upload, split, first use
+    g = rst.decompose_binding_energy(top, xn0, rec, lig)  # upload, split, first use
     first = time.perf_counter() - ts
-    _note(f"large: first GB decomposition {first:.2f} s")
     ts = time.perf_counter()
-    rst.decompose_binding_energy(top, xn, rec, lig)
+    g = rst.decompose_binding_energy(top, xn0, rec, lig)
     t_gb = time.perf_counter() - ts
-    ts = time.perf_counter()
-    e = rst.compute_binding_energy_single_frame(top, xn, rec, lig, pb_params=pb)
-    t_frame = time.perf_counter() - ts
-    _note(f"large: PB frame {t_frame:.2f} s")
-    ts = time.perf_counter()
-    d, dpb = rst.decompose_binding_energy_pb(top, xn, rec, lig, pb_params=pb)
-    t_dec = time.perf_counter() - ts
-    _note(f"large: PB decomposition {t_dec:.2f} s")
-    pol = sum(c.polar for c in d.receptor_residues() + d.ligand_residues())
-    return {"workload": "150,000-atom complex, 9,376 residues; PB 193^3 at 1.0 A then 321^3 "
-                        "at 0.4 A on the interface centroid; 1 frame",
-            "frame_mmpbsa_s": t_frame, "pb_decomposition_s": t_dec, "gb_decomposition_s": t_gb,
-            "first_call_s": first, "delta_total": e.delta_total, "delta_pb": dpb,
-            "residue_sum_minus_delta_pb": pol - dpb,
-            "frame_delta_polar_minus_decomposition_delta_pb": e.delta_polar - dpb}
+    t_frame, t_dec, ident = [], [], []
+    dpb0 = None
+    for k in range(n_frames):
+        xn = synthetic.noisy_frame(base, seed, k) / 10.0
+        ts = time.perf_counter()
+        e = rst.compute_binding_energy_single_frame(top, xn, rec, lig, pb_params=pb)
+        t_frame.append(time.perf_counter() - ts)
+        ts = time.perf_counter()
+        d, dpb = rst.decompose_binding_energy_pb(top, xn, rec, lig, pb_params=pb)
+        t_dec.append(time.perf_counter() - ts)
+        pol = math.fsum(c.polar for c in d.receptor_residues() + d.ligand_residues())
+        ident.append(max(abs(pol - dpb), abs(e.delta_polar - dpb)))
+        if k == 0:
+            dpb0, e0 = dpb, e
+        _note(f"large: frame {k}: MM-PBSA {t_frame[-1]:.2f} s, PB decomposition {t_dec[-1]:.2f} s")
+    rng = np.random.default_rng(3)
+    worst = 0.0
+    for k in rng.choice(len(lig), 12, replace=False):
+        res = lig[k]
+        ptr = np.append(np.asarray(top.residue_pointers), top.n_atoms)
+        ia = np.arange(ptr[res], ptr[res + 1])
+        v, el = _lj_coul(top, x0[ia], ia, x0[ra], ra)
+        c = g.ligand_residues()[k]
+        worst = max(worst, abs(c.vdw - 0.5 * v), abs(c.elec - 0.5 * el))
+    return {"workload": f"150,000-atom complex, 9,376 residues; PB 193^3 at 1.0 A then 321^3 at "
+                        f"0.4 A on the interface centroid; {n_frames} frames",
+            "frame_mmpbsa_s": float(np.mean(t_frame)), "pb_decomposition_s": float(np.mean(t_dec)),
+            "gb_decomposition_s": t_gb, "first_call_s": first,
+            "delta_total": e0.delta_total, "delta_pb": dpb0,
+            "max_identity_err_kcal": float(max(ident)),
+            "identities": "sum of residue PB terms = dG_PB = the frame's dPolar",
+            "sampled_residue_vdw_elec_err_vs_fp64_kcal": worst,
+            "sampled_residues": 12}
 
 
 def measure_traffic(frames_per_step, grid_threads):
@@ -219,6 +396,26 @@ def measure_traffic(frames_per_step, grid_threads):
             "dispatches": vals["FETCH_SIZE"][2], "copy_calibration_bytes": copy_mib << 20}
 
 
+def kernel_rooflines(lib, names):
+    """Achieved rate of named kernels from the library's HIP-event timers."""
+    out = {}
+    for name, bound in names.items():
+        ms, n, work = C.c_double(), C.c_int64(), C.c_double()
+        lib.mmpbsa_profile_kernel(name.encode(), C.byref(ms), C.byref(n), C.byref(work))
+        if n.value <= 0 or ms.value <= 0:
+            continue
+        rate = work.value / (ms.value * 1e-3)
+        if bound == "hbm":
+            ach, peak, unit = rate / 1e9, HBM_PEAK_GBS, "GB/s"
+        else:
+            ach, peak, unit = rate / 1e12, FP64_PEAK_TFS if bound == "fp64" else FP32_PEAK_TFS, \
+                "TFLOP/s"
+        out[name] = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit,
+                     "frac": ach / peak, "launches": n.value, "avg_launch_us": ms.value * 1e3 / n.value,
+                     "work_per_launch": work.value / n.value}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,14 +424,23 @@ def main():
     ap.add_argument("--frames-per-step", type=int, default=8)
     ap.add_argument("--plain-frames", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip roofline and small-system legs")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true", help="skip every leg beside the headline")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (default: this job's CPU share)")
     ap.add_argument("--no-many", action="store_true",
-                    help="skip the many-complex legs (uneven batch, serving)")
+                    help="skip the many-complex legs (uneven batch, 64 complexes, serving)")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 150,000-atom decomposition leg")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the 150,000-atom decomposition leg and print its JSON")
+    ap.add_argument("--many-complexes", type=int, default=64)
+    ap.add_argument("--many-frames", type=int, default=2,
+                    help="frames per complex of the many-complex leg (the workload names 20)")
+    ap.add_argument("--many-round", type=int, default=32, help="server round size (frames)")
+    ap.add_argument("--serving-requests", type=int, default=4)
+    ap.add_argument("--large-frames", type=int, default=2,
+                    help="frames of the 150,000-atom leg (the workload names 50)")
+    ap.add_argument("--oracle-max-atoms", type=int, default=12000)
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs behind roofline.traffic")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -249,6 +455,8 @@ def main():
 
     import rust_simulation_tools as rst
     from rust_simulation_tools import _abi, synthetic
+    share, aff = cpu_share()
+    workers = min(16, share)
     lib = _abi.load_library()
     if lib.mmpbsa_init(None) != 0:
         raise SystemExit(f"no usable gfx950 device: {lib.mmpbsa_last_error(None)}")
@@ -264,7 +472,8 @@ def main():
         lib.mmpbsa_device_free(dst)
 
     if args.large_only:
-        print(json.dumps({"large_decomposition": large_decomposition_leg(rst, synthetic)}))
+        print(json.dumps({"large_decomposition": large_decomposition_leg(rst, synthetic,
+                                                                         args.large_frames)}))
         return
 
     top, base, rec, lig, seed = synthetic.config_system(1)
@@ -275,9 +484,10 @@ def main():
 
     dev = rst.DeviceArray(frames_nm * 10.0)  # frames resident in HBM, Angstrom [F, N, 3]
 
-    def step(k, batched=True):
+    def step(k, batched=True, **kw):
+        kw = kw or dict(pb_params=pb)
         return rst.compute_binding_energy_device_frames(top, dev[k * B:(k + 1) * B], rec, lig,
-                                                        pb_params=pb, batched=batched)
+                                                        batched=batched, **kw)
 
     def step_host(k):
         return rst.compute_binding_energy_frames(top, frames_nm[k * B:(k + 1) * B], rec, lig,
@@ -296,6 +506,7 @@ def main():
             first = r
     elapsed = time.perf_counter() - t0
     value = args.steps * B / elapsed
+    _note(f"headline: {value:.2f} frames/s")
 
     roofline = None
     extras = {}
@@ -305,12 +516,28 @@ def main():
         for k in range(args.warmup, args.warmup + args.steps):
             step_host(k)
         extras["host_buffer_frames_per_s"] = args.steps * B / (time.perf_counter() - ts)
-        # dominant kernel: finest-level smoother passes, timed with HIP events
+        # dominant kernel: finest-level smoother passes, timed with HIP events;
+        # the pair kernels and face maps of the same step with their own events
         lib.mmpbsa_profile_enable(1)
         step(args.warmup + args.steps)
         ms, n, by, gt = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
         lib.mmpbsa_profile_read(C.byref(ms), C.byref(n), C.byref(by), C.byref(gt))
+        kr = kernel_rooflines(lib, {"mm_pair_kernel": "fp64", "faces_kernel": "hbm",
+                                    "dh_boundary_kernel": "fp32"})
         lib.mmpbsa_profile_enable(0)
+        # the same frames by MM-GBSA (OBC-II): batched throughput, then one
+        # profiled step for the GB pair kernels
+        gbp = rst.GbParams(model=GB_MODEL)
+        step(0, gb_params=gbp)
+        ts = time.perf_counter()
+        for k in range(args.warmup, args.warmup + args.steps):
+            step(k, gb_params=gbp)
+        extras["gbsa_batched_frames_per_s"] = args.steps * B / (time.perf_counter() - ts)
+        lib.mmpbsa_profile_enable(1)
+        step(0, gb_params=gbp)
+        kr.update(kernel_rooflines(lib, {"gb_radii": "fp64", "gb_energy_kernel": "fp64"}))
+        lib.mmpbsa_profile_enable(0)
+        extras["kernel_rooflines"] = kr
         if n.value > 0 and ms.value > 0:
             achieved = by.value / (ms.value * 1e-3) / 1e9
             traffic = None if args.no_traffic else measure_traffic(B, gt.value)
@@ -338,10 +565,16 @@ def main():
                                               "batched_over_plain": fb / fp}
 
     if not args.no_extras and not args.no_many:
-        extras["uneven_batch"] = uneven_leg(rst, synthetic)
-        extras["serving"] = serving_leg(rst, synthetic, not args.no_cpu)
+        extras["uneven_batch"] = uneven_leg(rst, synthetic, workers)
+        _note("uneven batch done")
+        extras["many_complexes"] = many_complex_leg(rst, synthetic, args.many_complexes,
+                                                    args.many_frames, workers, args.many_round)
+        _note("many-complex leg done")
+        extras["serving"] = serving_leg(rst, synthetic, args.serving_requests, workers,
+                                        args.oracle_max_atoms, not args.no_cpu)
+        _note("serving leg done")
     if not args.no_extras and not args.no_large:
-        extras["large_decomposition"] = large_decomposition_leg(rst, synthetic)
+        extras["large_decomposition"] = large_decomposition_leg(rst, synthetic, args.large_frames)
 
     plain_lat = []
     for f in range(args.plain_frames):
@@ -352,20 +585,45 @@ def main():
     plain_fps = 1.0 / float(np.median(plain_lat)) if plain_lat else None
 
     cpu = None
-    dg_err = None
+    errors = None
     if not args.no_cpu:
         from oracle import oracle as O
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or share
         O.lib().orc_set_threads(threads)
         x = frames_nm[args.warmup * B] * 10.0
         ts = time.perf_counter()
         o = O.frame_energy(top, x, rec, lig, method=1, pb=O.pb_params(**CFG1_PB))
         cpu_s = time.perf_counter() - ts
         g = first.frames()[0]
-        dg_err = abs(g.delta_total - o["delta_total"])
+        terms = _frame_errors(g, o)
+        # Born radii of the complex (OBC-II) and the oracle's own PB convergence
+        ge = rst.compute_gb_energy(top, x / 10.0, rst.GbParams(model=GB_MODEL))
+        og, oR = O.gb_energy(top, x, O.gb_params(model=GB_MODEL))
+        gp = rst.compute_pb_energy(top, x / 10.0, pb)
+        oc = o["complex"]["pb"]
+        pbinfo = {}
+        for part in ("complex", "receptor", "ligand"):
+            r_ = o[part]["pb"]
+            pbinfo[part] = {"oracle_converged": [bool(r_.solv.converged), bool(r_.ref.converged)],
+                            "oracle_final_residual": [r_.solv.final_residual,
+                                                      r_.ref.final_residual],
+                            "oracle_cycles": [r_.solv.iterations, r_.ref.iterations]}
+        pbinfo["complex"]["gpu_cycles"] = list(gp.iterations)
+        pbinfo["complex"]["gpu_final_residual"] = list(gp.final_residual)
+        errors = {"frame_terms_kcal": terms, "max_term_kcal": max(terms.values()),
+                  "born_radii_max_rel": float(np.max(np.abs(ge.born_radii() - oR) / oR)),
+                  "gb_complex_kcal": abs(ge.total - og),
+                  "pb_complex_dG_kcal": abs(gp.total - oc.total),
+                  "pb_solves": pbinfo,
+                  "oracle_vs_exact": "no scipy or long-double pin at this size; the oracle's "
+                                     "tol-1e-6 answer is within 6e-4 kcal/mol of its tol-1e-10 "
+                                     "answer on this grid"}
         cpu = {"value": 1.0 / cpu_s, "unit": "frames/s", "cores": threads, "kind": "port",
+               "nproc": os.cpu_count(), "affinity_cpus": aff, "threads": threads,
+               "cpu_model": cpu_model(),
                "sample": "1 frame of the 25,000-atom complex (MM + PB + SA for complex, "
-                         "receptor, ligand) on the FP64 C oracle, OpenMP"}
+                         "receptor, ligand) on the FP64 C oracle, OpenMP over this job's CPU "
+                         "share (affinity mask capped by OMP_NUM_THREADS)"}
 
     out = {
         "metric": "MM-PBSA frames/s (batched) at dG err <= 0.01 kcal/mol",
@@ -380,10 +638,13 @@ def main():
         "batched_over_plain": (value / plain_fps) if plain_fps else None,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "of": f"one step of {B} frames"},
-        "dG_err_vs_fp64_oracle": dg_err,
+        "dG_err_vs_fp64_oracle": errors["frame_terms_kcal"]["delta_total"] if errors else None,
+        "errors_vs_fp64_oracle": errors,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if cpu:
+        out["batched_over_cpu"] = value / cpu["value"]
     out.update(extras)
     print(json.dumps(out))
 

@@ -469,6 +469,17 @@ int mmpbsa_device_copy(void *dst, const void *src, int64_t bytes);
 int mmpbsa_profile_enable(int32_t on);
 int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes,
                         int64_t *grid_threads);
+/* While profiling is enabled, these launches are also timed with HIP events
+ * on the library's stream and booked by name with their algorithmic work:
+ *   "mm_pair_kernel"     20 FLOP per unordered atom pair (FP64)
+ *   "gb_radii"           32 FLOP per ordered pair (FP64; Born-radius integral)
+ *   "gb_energy_kernel"   26 FLOP per unordered pair (FP64)
+ *   "dh_boundary_kernel" 15 FLOP per boundary node and atom (FP32 terms)
+ *   "faces_kernel"       37 bytes per grid node (SES byte read, 3 FP64 and
+ *                        3 FP32 face values written)
+ * mmpbsa_profile_kernel returns the summed time, launch count and work of one
+ * name since profiling was last enabled (zeros for a name never timed). */
+int mmpbsa_profile_kernel(const char *name, double *total_ms, int64_t *launches, double *work);
 
 #ifdef __cplusplus
 }

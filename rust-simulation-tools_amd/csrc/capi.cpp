@@ -1358,6 +1358,7 @@ int mmpbsa_device_copy(void *dst, const void *src, int64_t bytes)
 
 int mmpbsa_profile_enable(int32_t on)
 {
+    kernel_profile_reset();
     std::lock_guard<std::mutex> lk(profile_mutex());
     ProfileStats &p = profile_stats();
     p.enabled = on != 0;
@@ -1365,6 +1366,12 @@ int mmpbsa_profile_enable(int32_t on)
     p.launches = 0;
     p.alg_bytes = 0.0;
     return MMPBSA_OK;
+}
+
+int mmpbsa_profile_kernel(const char *name, double *total_ms, int64_t *launches, double *work)
+{
+    if (!name) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    return guarded(nullptr, [&] { kernel_profile_read(name, total_ms, launches, work); });
 }
 
 int mmpbsa_profile_read(double *total_ms, int64_t *launches, double *alg_bytes,

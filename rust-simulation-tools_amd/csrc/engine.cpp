@@ -29,7 +29,11 @@ void run_mm(Worker &w, const DevTopo &t, const double *d_xyz, mmpbsa_mm_terms *o
     launch_bonded(d_xyz, v, pb.get(), w.stream);
     launch_sum_partials(pb.get(), nbb, 3, 3, res.get(), w.stream);
     if (v.n >= 2) {
-        launch_mm_pairs(d_xyz, v, pm.get(), w.stream);
+        {
+            // 20 FLOP per unordered pair (Coulomb + LJ + exclusion and 1-4 rules)
+            KernelTimer kt(w, "mm_pair_kernel", 10.0 * (double)v.n * (v.n - 1));
+            launch_mm_pairs(d_xyz, v, pm.get(), w.stream);
+        }
         launch_sum_partials_dd(pm.get(), nmm, 4, res.get() + 3, w.stream);
     } else {
         launch_fill_f64(res.get() + 3, 0.0, 4, w.stream);
@@ -63,10 +67,20 @@ double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_
     }
     const int nbk = gb_energy_blocks(v.n);
     DBuf<double> part(w.pool, nbk), res(w.pool, 1);
-    launch_gb_radii(d_xyz, v, p.model, p.offset, p.rgbmax, p.cutoff, psi.get(), d_born, w.stream);
+    {
+        // 32 FLOP per ordered pair (the HCT integral); the O(N) radius map
+        // launched with it is not counted
+        KernelTimer kt(w, "gb_radii", 32.0 * (double)v.n * (v.n - 1));
+        launch_gb_radii(d_xyz, v, p.model, p.offset, p.rgbmax, p.cutoff, psi.get(), d_born,
+                        w.stream);
+    }
     const double kappa = compute_kappa(p.salt_concentration, p.solvent_dielectric, p.temperature);
-    launch_gb_energy(d_xyz, v, d_born, kappa, p.solute_dielectric, p.solvent_dielectric, p.cutoff,
-                     part.get(), w.stream);
+    {
+        // 26 FLOP per unordered pair
+        KernelTimer kt(w, "gb_energy_kernel", 13.0 * (double)v.n * (v.n - 1));
+        launch_gb_energy(d_xyz, v, d_born, kappa, p.solute_dielectric, p.solvent_dielectric,
+                         p.cutoff, part.get(), w.stream);
+    }
     launch_sum_partials(part.get(), nbk, 1, 1, res.get(), w.stream);
     HIP_OK(hipMemcpyAsync(w.pin, res.get(), sizeof(double), hipMemcpyDeviceToHost, w.stream));
     if (born_out) {

@@ -212,4 +212,25 @@ struct ProfileStats {
 ProfileStats &profile_stats();
 std::mutex &profile_mutex();
 
+// Named-kernel timing (measurement only, no counterpart in the reference):
+// while profiling is enabled, a KernelTimer brackets the launches made in its
+// scope with HIP events on the worker's stream and books them under `name`
+// with `work` algorithmic units (FLOP or bytes, per the kernel's own rule).
+class KernelTimer {
+public:
+    KernelTimer(Worker &w, const char *name, double work);
+    ~KernelTimer();
+    KernelTimer(const KernelTimer &) = delete;
+    KernelTimer &operator=(const KernelTimer &) = delete;
+
+private:
+    Worker &w_;
+    const char *name_;
+    double work_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+// Totals of one name (events of finished launches are read and released).
+void kernel_profile_read(const char *name, double *ms, int64_t *launches, double *work);
+void kernel_profile_reset();
+
 }  // namespace mmpbsa

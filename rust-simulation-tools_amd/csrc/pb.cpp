@@ -7,6 +7,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <string>
 
 #include "engine.h"
 
@@ -37,6 +39,90 @@ std::mutex &profile_mutex()
 {
     static std::mutex m;
     return m;
+}
+
+namespace {
+struct KernelBook {
+    double ms = 0.0, work = 0.0;
+    int64_t launches = 0;
+};
+struct PendingTimer {
+    std::string name;
+    double work;
+    hipEvent_t a, b;
+};
+std::map<std::string, KernelBook> &kernel_books()
+{
+    static std::map<std::string, KernelBook> m;
+    return m;
+}
+std::vector<PendingTimer> &pending_timers()
+{
+    static std::vector<PendingTimer> v;
+    return v;
+}
+// with profile_mutex held
+void drain_timers()
+{
+    for (PendingTimer &t : pending_timers()) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            KernelBook &k = kernel_books()[t.name];
+            k.ms += ms;
+            k.work += t.work;
+            k.launches += 1;
+        } else {
+            (void)hipGetLastError();
+        }
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    pending_timers().clear();
+}
+}  // namespace
+
+KernelTimer::KernelTimer(Worker &w, const char *name, double work)
+    : w_(w), name_(name), work_(work)
+{
+    if (!profile_stats().enabled) return;
+    if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess ||
+        hipEventRecord(a_, w_.stream) != hipSuccess) {
+        (void)hipGetLastError();
+        if (a_) (void)hipEventDestroy(a_);
+        if (b_) (void)hipEventDestroy(b_);
+        a_ = b_ = nullptr;
+    }
+}
+
+KernelTimer::~KernelTimer()
+{
+    if (!a_) return;
+    if (hipEventRecord(b_, w_.stream) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipEventDestroy(a_);
+        (void)hipEventDestroy(b_);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(profile_mutex());
+    pending_timers().push_back(PendingTimer{name_, work_, a_, b_});
+}
+
+void kernel_profile_read(const char *name, double *ms, int64_t *launches, double *work)
+{
+    std::lock_guard<std::mutex> lk(profile_mutex());
+    drain_timers();
+    const auto it = kernel_books().find(name);
+    const KernelBook k = it == kernel_books().end() ? KernelBook{} : it->second;
+    if (ms) *ms = k.ms;
+    if (launches) *launches = k.launches;
+    if (work) *work = k.work;
+}
+
+void kernel_profile_reset()
+{
+    std::lock_guard<std::mutex> lk(profile_mutex());
+    drain_timers();
+    kernel_books().clear();
 }
 
 struct MapSet {
@@ -152,6 +238,9 @@ static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
         for (size_t f = 0; f < faces.size(); ++f) {
             MapReq &r = rq[faces[f]];
             MapSet &m = *r.m;
+            // algorithmic bytes: the SES byte read, three FP64 and three FP32
+            // face values written per node
+            KernelTimer kt(w, "faces_kernel", 37.0 * (double)r.g.size());
             launch_faces(r.g, tmp[faces[f]].inside.get(), r.d_xyz, r.t->radii.p, r.t->max_radius,
                          r.c->cl, r.ein, r.eout, m.ex.get(), m.ey.get(), m.ez.get(), m.fx.get(),
                          m.fy.get(), m.fz.get(), ovf.get() + f, w.stream);
@@ -913,6 +1002,11 @@ static void dh_sides(Worker &w, PbJob *j, const GridDesc &g, bool want0, bool wa
     const auto &s0 = S.side[0], &s1 = S.side[1];
     const int n = j->t->h.n;
     DBuf<double> dhw(w.pool, dh_work_size(g, n));
+    // 15 FP32 FLOP per boundary node and atom (one pass serves both media
+    // when the reference medium is salt-free)
+    const double bnodes = (double)g.size() - (double)std::max(0, g.nx - 2) *
+                                                 std::max(0, g.ny - 2) * std::max(0, g.nz - 2);
+    KernelTimer kt(w, "dh_boundary_kernel", 15.0 * bnodes * n);
     const double k0 = s0.ionic ? S.kappa : 0.0, k1 = s1.ionic ? S.kappa : 0.0;
     if (want0 && want1 && k1 == 0.0) {
         launch_dh_boundary(g, n, j->d_xyz, j->t->q.p, k0, s0.eout, phi0, s1.eout, phi1, dhw.get(),

@@ -287,6 +287,7 @@ SIGNATURES = {
     "mmpbsa_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "mmpbsa_profile_enable": (C.c_int, [C.c_int32]),
     "mmpbsa_profile_read": (C.c_int, [c_double_p, c_int64_p, c_double_p, c_int64_p]),
+    "mmpbsa_profile_kernel": (C.c_int, [C.c_char_p, c_double_p, c_int64_p, c_double_p]),
 }
 
 _lib = None
