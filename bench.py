@@ -157,13 +157,34 @@ def uneven_leg(rst, synthetic, workers):
             "mixed_over_one_at_a_time": alone / mixed}
 
 
-def many_complex_leg(rst, synthetic, n_cx, n_frames, workers, max_round):
+def _serve_all(rst, cx, pb, round_frames, frames=None):
+    """Submit every complex's frames at once to one server; (wall s, request
+    latencies ms)."""
+    srv = rst.Server(hbm_budget_bytes=200 << 30, max_round_frames=round_frames)
+    try:
+        t0 = time.perf_counter()
+        tickets = [srv.submit(top, f if frames is None else f[:frames], rec, lig, pb_params=pb)
+                   for top, f, rec, lig in cx]
+        lat = []
+        for t in tickets:
+            _, st = srv.wait(t)
+            lat.append(st.total_ms)
+        return time.perf_counter() - t0, lat
+    finally:
+        srv.close()
+
+
+def many_complex_leg(rst, synthetic, n_cx, n_frames, workers, tune):
     """n_cx complexes, N log-uniform over [3k, 60k], ligand fraction U(3%, 10%),
     n_frames noisy frames each, each on its own auto_grid grids (0.5 A, 10 A
     buffer): every frame submitted at once to one server under a 200 GB
     budget, which batches frames of different complexes into shared solver
-    launches.  The plain path (one frame at a time) is timed on the first
-    frame of four complexes at the size quartiles."""
+    launches.  Run twice: with the built-in launch choices and the server's
+    default round size, then with the autotuner's choices (smoother z-chunks
+    per level bucket, check interval, round size), tuned on one frame of the
+    eight complexes at the size octiles.  The plain path (one frame at a time)
+    is timed on the first frame of four complexes at the size quartiles."""
+    from rust_simulation_tools import autotune
     rng = np.random.Generator(np.random.PCG64(synthetic.seed_for(2, 0)))
     sizes = np.round(np.exp(rng.uniform(math.log(3000), math.log(60000), n_cx))).astype(int)
     fracs = rng.uniform(0.03, 0.10, n_cx)
@@ -184,31 +205,58 @@ def many_complex_leg(rst, synthetic, n_cx, n_frames, workers, max_round):
         t0 = time.perf_counter()
         rst.compute_binding_energy_frames(top, f[:1], rec, lig, pb_params=pb, batched=False)
         plain_t.append(time.perf_counter() - t0)
-    srv = rst.Server(hbm_budget_bytes=200 << 30, max_round_frames=max_round)
-    try:
-        big = int(order[-1])
-        srv.wait(srv.submit(cx[big][0], cx[big][1][:1], cx[big][2], cx[big][3], pb_params=pb))
-        t0 = time.perf_counter()
-        tickets = [srv.submit(top, f, rec, lig, pb_params=pb) for top, f, rec, lig in cx]
-        lat = []
-        for t in tickets:
-            _, st = srv.wait(t)
-            lat.append(st.total_ms)
-        span = time.perf_counter() - t0
-    finally:
-        srv.close()
+    big = int(order[-1])
+    _serve_all(rst, [cx[big]], pb, 16, frames=1)  # the server path's first-use costs
     nfr = n_cx * n_frames
-    # the same four frames batched through the server path, for a like-for-like ratio
-    return {"workload": f"{n_cx} complexes, N log-uniform [3k, 60k] (min {int(sizes.min())}, "
-                        f"median {int(np.median(sizes))}, max {int(sizes.max())}), ligand "
-                        f"3-10%, {n_frames} frames each ({nfr} frames), auto_grid h 0.5 A "
-                        f"buffer 10 A, one server, 200 GB budget, rounds of {max_round} frames",
-            "frames": nfr, "frames_per_s": nfr / span, "wall_s": span,
-            "request_latency_ms": {"p50": float(np.percentile(lat, 50)),
-                                   "p99": float(np.percentile(lat, 99))},
-            "plain_sample": {"complexes_atoms": [int(sizes[k]) for k in quart],
-                             "seconds_per_frame": plain_t,
-                             "frames_per_s": len(plain_t) / sum(plain_t)}}
+    autotune.apply(None)
+    default_round = 16
+    span0, lat0 = _serve_all(rst, cx, pb, default_round)
+    _note(f"many: built-in {nfr / span0:.2f} frames/s")
+    out = {"workload": f"{n_cx} complexes, N log-uniform [3k, 60k] (min {int(sizes.min())}, "
+                       f"median {int(np.median(sizes))}, max {int(sizes.max())}), ligand "
+                       f"3-10%, {n_frames} frames each ({nfr} frames), auto_grid h 0.5 A "
+                       "buffer 10 A, one server, 200 GB budget",
+           "frames": nfr,
+           "builtin": {"frames_per_s": nfr / span0, "wall_s": span0, "round_frames": default_round,
+                       "request_latency_ms": {"p50": float(np.percentile(lat0, 50)),
+                                              "p99": float(np.percentile(lat0, 99))}},
+           "plain_sample": {"complexes_atoms": [int(sizes[k]) for k in quart],
+                            "seconds_per_frame": plain_t,
+                            "frames_per_s": len(plain_t) / sum(plain_t)}}
+    if tune:
+        octiles = [cx[int(order[int(q * (n_cx - 1))])] for q in np.linspace(0.0625, 0.9375, 8)]
+        rounds = {"n": default_round}
+        ts = time.perf_counter()
+        table, rep = autotune.tune(lambda: _serve_all(rst, octiles, pb, rounds["n"], frames=1),
+                                   max_buckets=3, candidates=(1, 2, 3, 4), repeats=1,
+                                   check_cycles=(8, 4, 16), log=_note)
+
+        def with_round(r):
+            rounds["n"] = r
+            _serve_all(rst, octiles, pb, r, frames=1)
+
+        best_round, tried = autotune.choose(with_round, (16, 8, 32, 64), repeats=1, log=_note,
+                                            name="round_frames")
+        table.round_frames = best_round
+        t_tune = time.perf_counter() - ts
+        try:
+            path = autotune.save(table, workload="many-complex")
+        except OSError:
+            path = None
+        span1, lat1 = _serve_all(rst, cx, pb, best_round)
+        _note(f"many: tuned {nfr / span1:.2f} frames/s")
+        autotune.apply(None)
+        out["tuned"] = {"frames_per_s": nfr / span1, "wall_s": span1, "round_frames": best_round,
+                        "request_latency_ms": {"p50": float(np.percentile(lat1, 50)),
+                                               "p99": float(np.percentile(lat1, 99))},
+                        "tuned_over_builtin": span0 / span1,
+                        "table": table.to_dict(), "search": rep,
+                        "round_tried_ms": {str(k): v * 1e3 for k, v in tried.items()},
+                        "tuning_s": t_tune, "persisted_to": path,
+                        "tuned_on": "first frame of the 8 complexes at the size octiles"}
+    best = out.get("tuned", out["builtin"])
+    out["frames_per_s"] = best["frames_per_s"]
+    return out
 
 
 def serving_leg(rst, synthetic, n_req, workers, oracle_max_atoms, check_oracle):
@@ -436,7 +484,8 @@ def main():
     ap.add_argument("--many-complexes", type=int, default=64)
     ap.add_argument("--many-frames", type=int, default=2,
                     help="frames per complex of the many-complex leg (the workload names 20)")
-    ap.add_argument("--many-round", type=int, default=32, help="server round size (frames)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner runs (many-complex leg, GB chunking)")
     ap.add_argument("--serving-requests", type=int, default=4)
     ap.add_argument("--large-frames", type=int, default=2,
                     help="frames of the 150,000-atom leg (the workload names 50)")
@@ -537,6 +586,21 @@ def main():
         step(0, gb_params=gbp)
         kr.update(kernel_rooflines(lib, {"gb_radii": "fp64", "gb_energy_kernel": "fp64"}))
         lib.mmpbsa_profile_enable(0)
+        if not args.no_tune:
+            # pair-kernel chunking of the Born radii, tuned on the same GB step
+            from rust_simulation_tools import autotune
+            gtab, gtried = autotune.tune_gb(lambda: step(0, gb_params=gbp), log=_note)
+            ts = time.perf_counter()
+            for k in range(args.warmup, args.warmup + args.steps):
+                step(k, gb_params=gbp)
+            extras["gbsa_tuned"] = {"frames_per_s": args.steps * B / (time.perf_counter() - ts),
+                                    "gb_chunks": gtab.gb_chunks, "tried_ms": gtried}
+            lib.mmpbsa_profile_enable(1)
+            step(0, gb_params=gbp)
+            extras["gbsa_tuned"]["gb_radii"] = kernel_rooflines(lib, {"gb_radii": "fp64"}).get(
+                "gb_radii")
+            lib.mmpbsa_profile_enable(0)
+            autotune.apply(None)
         extras["kernel_rooflines"] = kr
         if n.value > 0 and ms.value > 0:
             achieved = by.value / (ms.value * 1e-3) / 1e9
@@ -568,7 +632,7 @@ def main():
         extras["uneven_batch"] = uneven_leg(rst, synthetic, workers)
         _note("uneven batch done")
         extras["many_complexes"] = many_complex_leg(rst, synthetic, args.many_complexes,
-                                                    args.many_frames, workers, args.many_round)
+                                                    args.many_frames, workers, not args.no_tune)
         _note("many-complex leg done")
         extras["serving"] = serving_leg(rst, synthetic, args.serving_requests, workers,
                                         args.oracle_max_atoms, not args.no_cpu)

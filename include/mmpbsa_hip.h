@@ -457,6 +457,43 @@ int mmpbsa_device_upload(void *dst, const void *src, int64_t bytes);
 int mmpbsa_device_download(void *dst, const void *src, int64_t bytes);
 int mmpbsa_device_copy(void *dst, const void *src, int64_t bytes);
 
+/* ---- launch tuning (no counterpart in the reference) -----------------------
+ * Launch shapes of the batched solver and the pair kernels, as chosen by an
+ * autotuner (rust_simulation_tools.autotune) for the workloads at hand.  The
+ * reference fixes its equivalents (8^3 SOR tiles and thread chunking,
+ * pb_solver.rs:214-327, pb_grid.rs:233).  Smoother z-chunking and the check
+ * interval never change a result bit; the Born-radius chunk count changes
+ * the order of its partial sums (results within rounding).  A level of a
+ * solve group uses the first entry whose node range holds its size and whose
+ * solve range holds the group's solve count; otherwise the built-in choice. */
+typedef struct mmpbsa_tune_entry {
+    int64_t nodes_min, nodes_max;   /* level size range, nodes                     */
+    int32_t solves_min, solves_max; /* solves sharing the launch                   */
+    int32_t smoother_chunks;        /* z-chunks per column of the phase smoother   */
+} mmpbsa_tune_entry;
+
+typedef struct mmpbsa_tuning {
+    int32_t check_cycles;  /* V-cycles between host status checks (default 8) */
+    int32_t gb_chunks;     /* Born-radius j-chunks per atom block (0: built-in) */
+    int32_t n_entries;     /* entries[0..n_entries) copied by mmpbsa_tuning_set */
+    const mmpbsa_tune_entry *entries;
+} mmpbsa_tuning;
+
+/* NULL restores the built-in choices.  At most 256 entries. */
+int mmpbsa_tuning_set(const mmpbsa_tuning *t);
+/* Current values; up to cap entries copied to entries (nullable), their
+ * total count in t->n_entries (t->entries is left NULL). */
+int mmpbsa_tuning_get(mmpbsa_tuning *t, mmpbsa_tune_entry *entries, int32_t cap);
+/* The smoother z-chunk length the solver would use for a level of nx*ny*nz
+ * nodes in a group of n_solves solves (for reports). */
+int32_t mmpbsa_tuning_smoother_zc(int32_t nx, int32_t ny, int32_t nz, int32_t n_solves);
+/* Observation for the autotuner: while on, every batched multigrid level the
+ * solver lays out is recorded as (level nodes, solves in its group) with a
+ * count.  Turning it on clears the record.  mmpbsa_tuning_observed copies up
+ * to cap records and returns how many there are. */
+int mmpbsa_tuning_observe(int32_t on);
+int32_t mmpbsa_tuning_observed(int64_t *nodes, int32_t *solves, int64_t *count, int32_t cap);
+
 /* ---- measurement (no counterpart in the reference) ----------------------
  * While enabled, each level-0 smoother pass of the fine focusing stage (the
  * dominant kernel: one red-black sweep per launch) is bracketed by HIP events

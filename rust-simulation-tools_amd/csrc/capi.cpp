@@ -1356,6 +1356,72 @@ int mmpbsa_device_copy(void *dst, const void *src, int64_t bytes)
     });
 }
 
+int mmpbsa_tuning_set(const mmpbsa_tuning *t)
+{
+    Tuning v = tuning();
+    if (!t) {
+        v.check_cycles = 8;
+        v.gb_chunks = 0;
+        v.entries.clear();
+        set_gb_chunks_override(0);
+        set_tuning(v);
+        return MMPBSA_OK;
+    }
+    if (t->n_entries < 0 || t->n_entries > 256 || (t->n_entries > 0 && !t->entries) ||
+        t->check_cycles < 0 || t->gb_chunks < 0)
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "invalid tuning");
+    v.check_cycles = t->check_cycles > 0 ? t->check_cycles : 8;
+    v.gb_chunks = t->gb_chunks;
+    v.entries.clear();
+    for (int k = 0; k < t->n_entries; ++k) {
+        const mmpbsa_tune_entry &e = t->entries[k];
+        TuneEntry q;
+        q.nodes_min = e.nodes_min;
+        q.nodes_max = e.nodes_max;
+        q.solves_min = e.solves_min;
+        q.solves_max = e.solves_max;
+        q.smoother_chunks = e.smoother_chunks;
+        v.entries.push_back(q);
+    }
+    set_gb_chunks_override(v.gb_chunks);
+    set_tuning(v);
+    return MMPBSA_OK;
+}
+
+int mmpbsa_tuning_get(mmpbsa_tuning *t, mmpbsa_tune_entry *entries, int32_t cap)
+{
+    if (!t) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null argument");
+    const Tuning v = tuning();
+    t->check_cycles = v.check_cycles;
+    t->gb_chunks = v.gb_chunks;
+    t->n_entries = (int32_t)v.entries.size();
+    t->entries = nullptr;
+    for (int k = 0; k < (int)v.entries.size() && k < cap && entries; ++k) {
+        entries[k].nodes_min = v.entries[k].nodes_min;
+        entries[k].nodes_max = v.entries[k].nodes_max;
+        entries[k].solves_min = v.entries[k].solves_min;
+        entries[k].solves_max = v.entries[k].solves_max;
+        entries[k].smoother_chunks = v.entries[k].smoother_chunks;
+    }
+    return MMPBSA_OK;
+}
+
+int32_t mmpbsa_tuning_smoother_zc(int32_t nx, int32_t ny, int32_t nz, int32_t n_solves)
+{
+    return tuned_smooth3_zc(tuning(), nx, ny, nz, n_solves);
+}
+
+int mmpbsa_tuning_observe(int32_t on)
+{
+    tuning_observe(on != 0);
+    return MMPBSA_OK;
+}
+
+int32_t mmpbsa_tuning_observed(int64_t *nodes, int32_t *solves, int64_t *count, int32_t cap)
+{
+    return tuning_observed(nodes, solves, count, cap);
+}
+
 int mmpbsa_profile_enable(int32_t on)
 {
     kernel_profile_reset();

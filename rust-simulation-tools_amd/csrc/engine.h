@@ -195,11 +195,32 @@ GridDesc coarse_grid(const double *mn, const double *mx, double h, double fillra
 void build_cells(Worker &w, const double *d_xyz, int n, const double *mn, const double *mx,
                  double cs, DevCells &c);
 
+// Launch-shape choices of the batched solver.  None of them changes a result
+// bit: the phase smoother, the streaming restriction and prolongation are
+// exact for any z-chunking, and the status checks only decide when the host
+// looks.  Entries come from the autotuner (mmpbsa_tuning_set).
+struct TuneEntry {
+    int64_t nodes_min = 0, nodes_max = 0;  // level size range
+    int solves_min = 0, solves_max = 0;    // solves sharing the launch
+    int smoother_chunks = 0;               // z-chunks per column (0: heuristic)
+};
 struct Tuning {
     int64_t tail_nodes = 16384;
     int batch_solves = 1 << 20;
+    int check_cycles = 8;   // V-cycles between host status checks
+    int gb_chunks = 0;      // Born-radius j-chunks per atom block (0: heuristic)
+    std::vector<TuneEntry> entries;
 };
-Tuning &tuning();
+// A snapshot of the current tuning (the table may be replaced concurrently).
+Tuning tuning();
+void set_tuning(const Tuning &t);
+// the smoother's z-chunk length for a level of these dimensions in a launch of
+// n_solves solves: the tuned chunk count when an entry matches, else the
+// built-in heuristic (smooth3_zc)
+int tuned_smooth3_zc(const Tuning &t, int nx, int ny, int nz, int n_solves);
+// record the (level nodes, group solves) of batched smoother levels
+void tuning_observe(bool on);
+int tuning_observed(int64_t *nodes, int32_t *solves, int64_t *count, int cap);
 
 // Kernel timing of the finest-grid smoother (no counterpart in the reference).
 struct ProfileStats {

@@ -37,6 +37,7 @@ constexpr int kTile = 256;
 
 // Born radii: psi partials then the OBC/HCT map.  work: [chunks * n] doubles.
 int gb_chunks(int n);
+void set_gb_chunks_override(int c);  // 0 restores the heuristic
 void launch_gb_radii(const double *xyz, const PairTopo &t, int model, double offset,
                      double rgbmax, double cutoff, double *psi_work, double *born_out,
                      hipStream_t s);

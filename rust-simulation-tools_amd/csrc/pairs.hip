@@ -6,6 +6,8 @@
 // Built with -ffp-contract=off: an FMA appears only where fma() is written.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 
 #include "kernels.h"
@@ -119,10 +121,16 @@ __global__ void gb_finalize_kernel(const double *__restrict__ part, int chunks, 
     born[i] = R;
 }
 
+// j-chunks of the Born-radius kernel: enough workgroups to fill the GPU
+// (about 4096) unless the tuning table fixes the count
+static std::atomic<int> g_gb_chunks_override{0};
+void set_gb_chunks_override(int c) { g_gb_chunks_override.store(c > 0 ? c : 0); }
+
 int gb_chunks(int n)
 {
     const int nb = std::max(1, ceil_div(n, kTile));
-    int c = std::max(1, std::min(64, ceil_div(4096, nb)));
+    const int o = g_gb_chunks_override.load();
+    int c = o > 0 ? std::min(o, 64) : std::max(1, std::min(64, ceil_div(4096, nb)));
     c = std::min(c, std::max(1, ceil_div(n, kTile)));
     return c;
 }

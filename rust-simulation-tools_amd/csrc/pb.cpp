@@ -18,7 +18,13 @@ static const double kFourPiEc = 4.0 * kPi * kCoulomb;
 // Launch-shape knobs of the batched solver (tuned per device, see bench.py):
 // levels with at most tail_nodes nodes run inside the one-workgroup tail;
 // at most batch_solves solves share a launch.
-Tuning &tuning()
+namespace {
+std::mutex &tuning_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+Tuning &tuning_store()
 {
     static Tuning t = [] {
         Tuning v;
@@ -27,6 +33,76 @@ Tuning &tuning()
         return v;
     }();
     return t;
+}
+}  // namespace
+
+Tuning tuning()
+{
+    std::lock_guard<std::mutex> lk(tuning_mutex());
+    return tuning_store();
+}
+
+void set_tuning(const Tuning &t)
+{
+    std::lock_guard<std::mutex> lk(tuning_mutex());
+    tuning_store() = t;
+}
+
+namespace {
+// (level nodes, solves in the group) -> level-solve occurrences, recorded while
+// observation is on (the autotuner's view of a workload)
+std::mutex &observe_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+bool g_observe = false;
+std::map<std::pair<int64_t, int>, int64_t> &observed()
+{
+    static std::map<std::pair<int64_t, int>, int64_t> m;
+    return m;
+}
+}  // namespace
+
+void tuning_observe(bool on)
+{
+    std::lock_guard<std::mutex> lk(observe_mutex());
+    g_observe = on;
+    observed().clear();
+}
+
+int tuning_observed(int64_t *nodes, int32_t *solves, int64_t *count, int cap)
+{
+    std::lock_guard<std::mutex> lk(observe_mutex());
+    int k = 0;
+    for (const auto &kv : observed()) {
+        if (k < cap) {
+            if (nodes) nodes[k] = kv.first.first;
+            if (solves) solves[k] = kv.first.second;
+            if (count) count[k] = kv.second;
+        }
+        ++k;
+    }
+    return k;
+}
+
+static void observe_level(int64_t nodes, int n_solves)
+{
+    std::lock_guard<std::mutex> lk(observe_mutex());
+    if (g_observe) observed()[{nodes, n_solves}] += 1;
+}
+
+int tuned_smooth3_zc(const Tuning &t, int nx, int ny, int nz, int n_solves)
+{
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    for (const TuneEntry &e : t.entries) {
+        if (nodes < e.nodes_min || nodes > e.nodes_max || n_solves < e.solves_min ||
+            n_solves > e.solves_max || e.smoother_chunks <= 0)
+            continue;
+        const int nch = std::min(e.smoother_chunks, std::max(1, nz - 2));
+        return ceil_div(std::max(1, nz - 2), nch);
+    }
+    return smooth3_zc(nx, ny, nz, n_solves);
 }
 
 ProfileStats &profile_stats()
@@ -452,7 +528,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
 static void mg_solve_batch(Worker &w, std::vector<MgTask *> &tasks, bool use_graph,
                            bool profile_level0)
 {
-    const size_t g = (size_t)std::max(1, tuning().batch_solves);
+    const size_t g = (size_t)std::max(1, tuning().batch_solves);  // one snapshot per batch
     for (size_t a = 0; a < tasks.size(); a += g) {
         std::vector<MgTask *> part(tasks.begin() + a, tasks.begin() + std::min(tasks.size(), a + g));
         mg_solve_group(w, part, use_graph, profile_level0);
@@ -462,6 +538,7 @@ static void mg_solve_batch(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
 static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_graph,
                            bool profile_level0)
 {
+    const Tuning tun = tuning();
     std::vector<MgTask *> mgt;
     for (MgTask *t : tasks) {
         const int md = std::min(t->g.nx, std::min(t->g.ny, t->g.nz));
@@ -511,7 +588,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         const int nl = (int)tw.L.size();
         tw.tail_start = nl - 1;
         for (int l = 0; l < nl; ++l)
-            if (lv_nodes(tw.L[l].g) <= tuning().tail_nodes) {
+            if (lv_nodes(tw.L[l].g) <= tun.tail_nodes) {
                 tw.tail_start = l;
                 break;
             }
@@ -534,7 +611,9 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const float ih = (float)(1.0 / (lv.g.h * lv.g.h));
             dv.ihx = dv.ihy = dv.ihz = ih;
             dv.omega = (float)estimate_omega(lv.g.nx, lv.g.ny, lv.g.nz);
-            dv.zc = phase_pass ? smooth3_zc(lv.g.nx, lv.g.ny, lv.g.nz, S) : smooth_zc(lv.g.nz);
+            dv.zc = phase_pass ? tuned_smooth3_zc(tun, lv.g.nx, lv.g.ny, lv.g.nz, S)
+                               : smooth_zc(lv.g.nz);
+            if (l < tw.tail_start) observe_level(N, S);
             dv.e = lv.e.get();
             dv.e2 = lv.e2.get();
             dv.f = lv.f.get();
@@ -885,7 +964,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     // blocks of the FP64 outer step of the running solves when the lists were
     // last built (a proxy for the level-0 work of every launch)
     int64_t listed = tot_of(WL_OUT);
-    const int chunk_cycles = 8;
+    const int chunk_cycles = std::max(2, tun.check_cycles);
     int done_cycles = 0;
     while (done_cycles < maxv) {
         const int n = std::min(chunk_cycles, maxv - done_cycles + (maxv - done_cycles) % 2);

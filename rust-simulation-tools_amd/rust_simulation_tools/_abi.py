@@ -198,6 +198,16 @@ class RequestStatsC(C.Structure):
                 ("status", C.c_int32)]
 
 
+class TuneEntryC(C.Structure):
+    _fields_ = [("nodes_min", C.c_int64), ("nodes_max", C.c_int64), ("solves_min", C.c_int32),
+                ("solves_max", C.c_int32), ("smoother_chunks", C.c_int32)]
+
+
+class TuningC(C.Structure):
+    _fields_ = [("check_cycles", C.c_int32), ("gb_chunks", C.c_int32), ("n_entries", C.c_int32),
+                ("entries", C.POINTER(TuneEntryC))]
+
+
 # Every symbol include/mmpbsa_hip.h declares, with its ctypes signature.
 P = C.c_void_p
 SIGNATURES = {
@@ -285,6 +295,11 @@ SIGNATURES = {
     "mmpbsa_device_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "mmpbsa_device_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     "mmpbsa_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "mmpbsa_tuning_set": (C.c_int, [C.POINTER(TuningC)]),
+    "mmpbsa_tuning_get": (C.c_int, [C.POINTER(TuningC), C.POINTER(TuneEntryC), C.c_int32]),
+    "mmpbsa_tuning_smoother_zc": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "mmpbsa_tuning_observe": (C.c_int, [C.c_int32]),
+    "mmpbsa_tuning_observed": (C.c_int32, [c_int64_p, c_int32_p, c_int64_p, C.c_int32]),
     "mmpbsa_profile_enable": (C.c_int, [C.c_int32]),
     "mmpbsa_profile_read": (C.c_int, [c_double_p, c_int64_p, c_double_p, c_int64_p]),
     "mmpbsa_profile_kernel": (C.c_int, [C.c_char_p, c_double_p, c_int64_p, c_double_p]),
