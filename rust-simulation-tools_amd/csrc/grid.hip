@@ -518,17 +518,20 @@ void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d
     HIP_OK(hipMemcpyAsync(d_table, ht.data(), sizeof(FloodJob) * n, hipMemcpyHostToDevice, s));
     launch_fill_i32(d_flags, 0, n, s);
     launch_fill_i32(d_flags + n, 1, n, s);
-    // rounds of each map until a round changes nothing.  kRounds rounds are
-    // queued per host check; a round after the fixpoint finds active = 0 and
-    // its launches exit at once.  Maps are flooded one after another: sharing
+    // rounds of each map until a round changes nothing.  kRounds rounds of
+    // every map still flooding are queued per host check (one check for all
+    // of them); a round after a map's fixpoint finds active = 0 and its
+    // launches exit at once.  Each map keeps launches of its own: sharing
     // launches between maps measured slower, as each map's x sweeps then
-    // compete for the caches and every map waited for the slowest one's rounds.
+    // compete for the caches.
     constexpr int kRounds = 4;
-    int act = 1;
-    for (int k = 0; k < n; ++k) {
-        const GridDesc &g = jobs[k].g;
-        const int64_t ln[3] = {(int64_t)g.ny * g.nz, (int64_t)g.nx * g.nz, (int64_t)g.nx * g.ny};
-        for (int round = 0; round < 100000; round += kRounds) {
+    std::vector<int> live(n), act(n, 1);
+    for (int k = 0; k < n; ++k) live[k] = k;
+    for (int round = 0; !live.empty() && round < 100000; round += kRounds) {
+        for (int k : live) {
+            const GridDesc &g = jobs[k].g;
+            const int64_t ln[3] = {(int64_t)g.ny * g.nz, (int64_t)g.nx * g.nz,
+                                   (int64_t)g.nx * g.ny};
             for (int r = 0; r < kRounds; ++r) {
                 for (int ax = 0; ax < 3; ++ax) {
                     const int len = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
@@ -547,10 +550,11 @@ void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d
                 hipLaunchKernelGGL(flood_round_end_kernel, dim3(1), dim3(64), 0, s, d_table + k);
                 LAUNCH_OK();
             }
-            HIP_OK(hipMemcpyAsync(&act, d_flags + n + k, sizeof(int), hipMemcpyDeviceToHost, s));
-            HIP_OK(hipStreamSynchronize(s));
-            if (!act) break;
         }
+        HIP_OK(hipMemcpyAsync(act.data(), d_flags + n, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        live.erase(std::remove_if(live.begin(), live.end(), [&](int k) { return act[k] == 0; }),
+                   live.end());
     }
 
     for (const SesJob &J : jobs) {

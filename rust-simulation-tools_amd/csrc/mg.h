@@ -136,6 +136,11 @@ void mg_outer64(const MgOuterDev *O, int n_solves, int64_t max_nodes, const MgSt
 void mg_check(const MgOuterDev *O, int n_solves, int count_cycle, MgStatus *st, hipStream_t s);
 void mg_status_init(MgStatus *st, const MgStatus *init_host, int n, hipStream_t s);
 void mg_inject(const MgLevelDev &F, const MgLevelDev &C, hipStream_t s);  // host-side structs
+// mg_inject then the code bytes (mg_pack_codes) of level C for n solves at once:
+// F and C are device arrays of n descriptors (C[k].nx == 0: skip solve k),
+// max_nodes the largest C level.
+void mg_inject_pack_batch(const MgLevelDev *F, const MgLevelDev *C, int n, int64_t max_nodes,
+                          hipStream_t s);
 
 // Code byte of a node: bits 0-5 = face x+, x-, y+, y-, z+, z- has the solute
 // dielectric (else the solvent one), bit 6 = kappa-bar^2 is c_k2 (else 0),

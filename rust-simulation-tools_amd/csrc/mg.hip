@@ -1911,6 +1911,67 @@ static void pack_codes(int nx, int ny, int nz, const T *ex, const T *ey, const T
     LAUNCH_OK();
 }
 
+// coarsen_dielectrics / coarsen_kappa for level C of every listed solve in one
+// launch (grid: blocks x solves; a solve whose C.nx is 0 has no such level)
+__global__ void inject_batch_kernel(const MgLevelDev *__restrict__ Fs,
+                                    const MgLevelDev *__restrict__ Cs)
+{
+    const MgLevelDev C = Cs[blockIdx.y];
+    if (C.nx == 0) return;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nodes(C)) return;
+    const MgLevelDev F = Fs[blockIdx.y];
+    const int64_t csz = (int64_t)C.nx * C.ny;
+    const int cz = (int)(idx / csz);
+    const int cy = (int)((idx - (int64_t)cz * csz) / C.nx);
+    const int cx = (int)(idx - (int64_t)cz * csz - (int64_t)cy * C.nx);
+    const int x = min(2 * cx, F.nx - 1), y = min(2 * cy, F.ny - 1), z = min(2 * cz, F.nz - 1);
+    const int64_t f = x + (int64_t)F.nx * (y + (int64_t)F.ny * z);
+    const_cast<float *>(C.ex)[idx] = F.ex[f];
+    const_cast<float *>(C.ey)[idx] = F.ey[f];
+    const_cast<float *>(C.ez)[idx] = F.ez[f];
+    const_cast<float *>(C.k2)[idx] = F.k2[f];
+}
+
+// code bytes of level L of every listed solve (pack_codes_kernel's rule)
+__global__ void pack_codes_batch_kernel(const MgLevelDev *__restrict__ Ls)
+{
+    const MgLevelDev L = Ls[blockIdx.y];
+    if (L.nx == 0) return;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nodes(L)) return;
+    const int64_t sy = L.nx, sz = (int64_t)L.nx * L.ny;
+    const float cin = L.c_in, cout = L.c_out;
+    uint32_t w = 0u;
+    auto face = [&](const float *a, int64_t i, int bit) {
+        if (i < 0) return;
+        const float v = a[i];
+        if (v == cin)
+            w |= 1u << bit;
+        else if (v != cout)
+            w |= kCodeSpecial;
+    };
+    face(L.ex, idx, 0);
+    face(L.ex, idx - 1, 1);
+    face(L.ey, idx, 2);
+    face(L.ey, idx - sy, 3);
+    face(L.ez, idx, 4);
+    face(L.ez, idx - sz, 5);
+    if (L.k2[idx] != 0.0f) w |= kCodeKappa;
+    const_cast<uint8_t *>(L.cw)[idx] = (uint8_t)w;
+}
+
+void mg_inject_pack_batch(const MgLevelDev *F, const MgLevelDev *C, int n, int64_t max_nodes,
+                          hipStream_t s)
+{
+    if (n <= 0 || max_nodes <= 0) return;
+    const dim3 grid((unsigned)std::max<int64_t>(1, ceil_div(max_nodes, 256)), (unsigned)n);
+    hipLaunchKernelGGL(inject_batch_kernel, grid, dim3(256), 0, s, F, C);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(pack_codes_batch_kernel, grid, dim3(256), 0, s, C);
+    LAUNCH_OK();
+}
+
 void mg_pack_codes(int nx, int ny, int nz, const double *ex, const double *ey, const double *ez,
                    const double *k2, double c_in, double c_out, uint8_t *cw, hipStream_t s)
 {

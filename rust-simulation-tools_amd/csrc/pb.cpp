@@ -638,14 +638,13 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                     dv.ey = lv.ey.get();
                     dv.ez = lv.ez.get();
                     dv.k2 = lv.k2.get();
-                    mg_inject(tw.L[l - 1].d, dv, w.stream);
                     lv.cw = DBuf<uint8_t>(w.pool, N);
                     dv.cw = lv.cw.get();
                     dv.c_in = (float)m.c_in;
                     dv.c_out = (float)m.c_out;
                     dv.c_k2 = (float)m.c_k2;
-                    mg_pack_codes(dv.nx, dv.ny, dv.nz, dv.ex, dv.ey, dv.ez, dv.k2, dv.c_in,
-                                  dv.c_out, lv.cw.get(), w.stream);
+                    // injected faces and code bytes: one launch per level for
+                    // every solve of the group, after this loop
                 }
             }
         }
@@ -678,6 +677,33 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 tw.O[p].phi_out = bufs[p];
                 tw.O[p].e = E[p];
             }
+        }
+    }
+
+    // coarsened dielectrics and kappa (coarsen_dielectrics / coarsen_kappa,
+    // pb_solver.rs:684-748) and their code bytes, level by level for all
+    // solves with non-uniform maps at once
+    std::vector<std::vector<MgLevelDev>> inj_host;
+    std::vector<DBuf<MgLevelDev>> inj_dev;
+    {
+        size_t maxL = 0;
+        for (const TaskWork &tw : W) maxL = std::max(maxL, tw.L.size());
+        for (size_t l = 1; l < maxL; ++l) {
+            std::vector<MgLevelDev> h(2 * (size_t)S, null_level());
+            int64_t mx = 0;
+            for (int s = 0; s < S; ++s) {
+                const TaskWork &tw = W[s];
+                if (tw.t->m->uni || l >= tw.L.size()) continue;
+                h[s] = tw.L[l - 1].d;
+                h[S + s] = tw.L[l].d;
+                mx = std::max(mx, lv_nodes(tw.L[l].g));
+            }
+            if (mx == 0) continue;
+            inj_dev.emplace_back(w.pool, 2 * (size_t)S);
+            HIP_OK(hipMemcpyAsync(inj_dev.back().get(), h.data(), sizeof(MgLevelDev) * h.size(),
+                                  hipMemcpyHostToDevice, w.stream));
+            mg_inject_pack_batch(inj_dev.back().get(), inj_dev.back().get() + S, S, mx, w.stream);
+            inj_host.push_back(std::move(h));
         }
     }
 
