@@ -484,6 +484,8 @@ def main():
     ap.add_argument("--many-complexes", type=int, default=64)
     ap.add_argument("--many-frames", type=int, default=2,
                     help="frames per complex of the many-complex leg (the workload names 20)")
+    ap.add_argument("--only", choices=("uneven", "many", "serving"),
+                    help="run only this many-complex leg after the headline (for profiling)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner runs (many-complex leg, GB chunking)")
     ap.add_argument("--serving-requests", type=int, default=4)
@@ -628,7 +630,17 @@ def main():
                                               "plain_frames_per_s": fp,
                                               "batched_over_plain": fb / fp}
 
-    if not args.no_extras and not args.no_many:
+    if args.only:
+        if args.only == "uneven":
+            extras["uneven_batch"] = uneven_leg(rst, synthetic, workers)
+        elif args.only == "many":
+            extras["many_complexes"] = many_complex_leg(rst, synthetic, args.many_complexes,
+                                                        args.many_frames, workers,
+                                                        not args.no_tune)
+        else:
+            extras["serving"] = serving_leg(rst, synthetic, args.serving_requests, workers,
+                                            args.oracle_max_atoms, not args.no_cpu)
+    elif not args.no_extras and not args.no_many:
         extras["uneven_batch"] = uneven_leg(rst, synthetic, workers)
         _note("uneven batch done")
         extras["many_complexes"] = many_complex_leg(rst, synthetic, args.many_complexes,
@@ -637,7 +649,7 @@ def main():
         extras["serving"] = serving_leg(rst, synthetic, args.serving_requests, workers,
                                         args.oracle_max_atoms, not args.no_cpu)
         _note("serving leg done")
-    if not args.no_extras and not args.no_large:
+    if not args.no_extras and not args.no_large and not args.only:
         extras["large_decomposition"] = large_decomposition_leg(rst, synthetic, args.large_frames)
 
     plain_lat = []
