@@ -1637,7 +1637,10 @@ __global__ __launch_bounds__(256) void restrict_kernel(const MgLevelDev *__restr
 __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict__ Ts,
                                                     const MgStatus *__restrict__ st)
 {
-    constexpr int kLds = 16384;  // floats: the coarsest levels live in LDS
+    // floats: 158 KB of gfx950's 160 KB of LDS per workgroup hold the
+    // coarsest levels and, on the 161^3 chain, the 21^3 entry level's
+    // correction, right-hand side and residual
+    constexpr int kLds = 40448;
     __shared__ float lds[kLds];
     const int s = blockIdx.x;
     if (stopped(st + s)) return;
@@ -1646,16 +1649,29 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
     const int tid = threadIdx.x, nt = blockDim.x;
     MgLevelDev lv[kMaxTail];
     for (int l = 0; l < n; ++l) lv[l] = Ts[s].lv[l];
+    float *entry_e = nullptr;  // global home of the entry level's correction, when in LDS
     {
-        // from the coarsest level upwards, move levels (except the entry level,
-        // whose correction is read back by the caller) into LDS while they fit
+        // from the coarsest level upwards, move levels into LDS while they
+        // fit: e, f and the residual, and the FP32 faces when those fit too.
+        // The entry level's right-hand side comes from the batched
+        // restriction and its correction is read back by the prolongation
+        // that follows, so they are copied in and out.
         int used = 0;
-        for (int l = n - 1; l >= 1; --l) {
+        for (int l = n - 1; l >= 0; --l) {
             const int N = (int)nodes(lv[l]);
-            const int need = (lv[l].uni ? 3 : 7) * N;
-            if (used + need > kLds) break;
+            const bool full = used + (lv[l].uni ? 3 : 7) * N <= kLds;
+            if (!full && used + 3 * N > kLds) break;
             float *base = lds + used;
-            if (!lv[l].uni) {
+            if (l == 0) {
+                for (int i = tid; i < N; i += nt) base[N + i] = lv[0].f[i];
+                entry_e = lv[0].e;
+            }
+            lv[l].e = base;
+            lv[l].f = base + N;
+            lv[l].res = base + 2 * N;
+            used += 3 * N;
+            if (!full || lv[l].uni) continue;
+            {
                 float *cx = base + 3 * N, *cy = base + 4 * N, *cz = base + 5 * N, *ck = base + 6 * N;
                 for (int i = tid; i < N; i += nt) {
                     cx[i] = lv[l].ex[i];
@@ -1667,11 +1683,8 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
                 lv[l].ey = cy;
                 lv[l].ez = cz;
                 lv[l].k2 = ck;
+                used += 4 * N;
             }
-            lv[l].e = base;
-            lv[l].f = base + N;
-            lv[l].res = base + 2 * N;
-            used += need;
         }
         __syncthreads();
     }
@@ -1707,6 +1720,10 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
         for (int64_t i = tid; i < nodes(L); i += nt) prolong_node(C, L, i);
         __syncthreads();
         sweeps(L, 3);
+    }
+    if (entry_e) {
+        const MgLevelDev L0 = lv[0];
+        for (int64_t i = tid; i < nodes(L0); i += nt) entry_e[i] = L0.e[i];
     }
 }
 
