@@ -709,18 +709,19 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         tab[2 * tid + 1] = make_float4(a4, a5, __builtin_amdgcn_rcpf(dg), 0.0f);
     }
 
-    // plane loads: value n = tid + NT j at region row (tid >> 5) + (NT / 32) j,
-    // column tid & 31
-    const int lx = xl + (tid & 31), ly = yl + (tid >> 5);
-    uint32_t pvo[PF];
-#pragma unroll
-    for (int j = 0; j < PF; ++j) {
-        const int y = ly + (NT / 32) * j;
-        pvo[j] = (lx >= 0 && lx < nx && y >= 0 && y < ny) ? 4u * (uint32_t)(lx + nx * y) : kOob;
-    }
     // column pairs m: region row ry = (tid >> 4) + PR m, columns rx0, rx0 + 1
     const int ry0 = tid >> 4, rx0 = 2 * (tid & 15);
     const int li0 = ry0 * RW + rx0;  // LDS index of column 0 of pair 0 (pair 1: + 16 RW)
+    // plane loads: a thread loads and stores the four nodes of its own column
+    // pairs, value j = 2 m + c at region row ry0 + PR m, column rx0 + c, so
+    // that within a step it alone reads the plane it stores
+    const int lx = xl + rx0, ly = yl + ry0;
+    uint32_t pvo[PF];
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+        const int x = lx + (j & 1), y = ly + PR * (j >> 1);
+        pvo[j] = (x >= 0 && x < nx && y >= 0 && y < ny) ? 4u * (uint32_t)(x + nx * y) : kOob;
+    }
     const int par = (xl + rx0 + yl + ry0) & 1;  // active column at step t: par ^ (t & 1)
     uint32_t cvo[PP];   // node index of column 0 (kOob when the pair leaves the grid)
     uint32_t rdpk = 0;  // byte 2m + c: ring distance of column c if it is updated, else 0
@@ -772,7 +773,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
             float x = bload(src, pvo[j], zo);
             if (PROLONG) {
                 const bool v = pvo[j] != kOob;
-                x += prolong_value(C, ceg, v ? lx : 0, v ? ly + (NT / 32) * j : 0, z);
+                x += prolong_value(C, ceg, v ? lx + (j & 1) : 0, v ? ly + PR * (j >> 1) : 0, z);
                 x = v ? x : 0.0f;
             }
             pf[j] = x;
@@ -781,7 +782,7 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     auto store_plane = [&](int z) {
         float *p = slot(z);
 #pragma unroll
-        for (int j = 0; j < PF; ++j) p[tid + NT * j] = pf[j];
+        for (int j = 0; j < PF; ++j) p[li0 + PR * RW * (j >> 1) + (j & 1)] = pf[j];
     };
     // finest level, post-smoothing: phi += e in FP64 instead of storing e (the
     // outer step's own addition).  At the write-back point the final values of
@@ -861,16 +862,12 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     using Fast = std::integral_constant<int, 0>;
     using Slow = std::integral_constant<int, 1>;
     using NoSpec = std::integral_constant<int, 2>;
-    // overflow flags of the two step parities live in the unused fourth
-    // word of table entries 0 and 1
-    int *const wg_ovf0 = reinterpret_cast<int *>(&tab[1]) + 3;
-    int *const wg_ovf1 = reinterpret_cast<int *>(&tab[3]) + 3;
-    auto wg_ovf = [&](int par_t) { return par_t ? wg_ovf1 : wg_ovf0; };
     uint32_t *req = reinterpret_cast<uint32_t *>(spc) + wv * SLOTS * 8;
 
     // the step loop, compiled separately for uniform and non-uniform levels
     auto run = [&](auto uni_tag) {
         constexpr bool UNI = decltype(uni_tag)::value;
+        constexpr bool ENT = FIRST;
         Coef Q[K];  // Q[i]: plane t - i
         Next nxt;
 
@@ -885,7 +882,18 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         bool nospec = false;  // no code word of the wave's six planes has the fractional bit
         float sa[6];
         uint32_t sreq = 0xffffffffu;
+        // face entries of the step's updates: byte j & 3 of ent[j >> 2] is the
+        // entry (table row, or 128 + the wave's slot) of update j = 2 (k - 1) +
+        // m, valid outside the SLOW mode
+        uint32_t ent[3] = {0u, 0u, 0u};
         auto specials_issue = [&](int t, bool all) {
+            // bytes c and 2 + c (the active column of both pairs) of the code
+            // words of two consecutive planes, kappa and face bits only
+            const uint32_t sel = 0x06040200u + 0x01010101u * (uint32_t)(par ^ (t & 1));
+            if (ENT)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    ent[i] = __builtin_amdgcn_perm(Q[2 * i + 1].w, Q[2 * i].w, sel) & 0x7f7f7f7fu;
             uint32_t anyw = 0;
 #pragma unroll
             for (int k = 0; k < K; ++k) anyw |= Q[k].w;
@@ -933,7 +941,18 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                     const int j = __ffs(rem) - 1;
                     rem &= rem - 1u;
                     const uint32_t g = ((j & 1) ? a1 : a0) - sxy * (uint32_t)((j >> 1) + 1);
-                    if (sl < SLOTS) req[sl] = (g << 1) | ((kb >> j) & 1u);
+                    if (sl < SLOTS) {
+                        req[sl] = (g << 1) | ((kb >> j) & 1u);
+                        if (ENT) {
+                            const uint32_t sh = 8u * (uint32_t)(j & 3);
+                            const uint32_t v = (uint32_t)(TAB + wv * SLOTS + sl) << sh;
+                            const uint32_t keep = ~(0xffu << sh);
+                            const int wi = j >> 2;
+                            ent[0] = wi == 0 ? (ent[0] & keep) | v : ent[0];
+                            ent[1] = wi == 1 ? (ent[1] & keep) | v : ent[1];
+                            ent[2] = wi == 2 ? (ent[2] & keep) | v : ent[2];
+                        }
+                    }
                     ++sl;
                 }
             }
@@ -958,8 +977,6 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                 make_float4(a4, a5, __builtin_amdgcn_rcpf(dg), 0.0f);
         };
 
-        // half-sweep k of step t on plane t - k + 1; SLOW: some wave of the
-        // workgroup has more fractional-face nodes this step than slots
         // per step: the active column's ring distance and the byte offset,
         // within a plane, of the node one row above it (so every in-plane
         // neighbour is a non-negative offset from one address)
@@ -975,8 +992,10 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                                             (RH - 3) * RW + RW - 1);
             }
         };
-        // MODE 0: fast path; 1: SLOW (a wave overflowed its slots); 2: the
-        // wave has no fractional-face node in its six planes (table only)
+        // half-sweep k of step t on plane t - k + 1.  MODE 0: fast path; 1:
+        // SLOW (the wave has more fractional-face updates this step than
+        // slots); 2: the wave has no fractional-face node in its six planes
+        // (table only)
         // all: every half-sweep of step t has its plane in range (no test;
         // uniform, so the scalar unit branches over the tests)
         auto stage = [&](int k, int t, auto mode_tag, bool all) {
@@ -1025,16 +1044,18 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                 } else {
                     const uint32_t w = (q.w >> (8 * jb)) & 0xffu;
                     const int j = 2 * (k - 1) + m;
-                    const int sl = NOSPEC ? 0 : spx + __popc(spm & ((1u << j) - 1u));
-                    // a lane that does not update reads its table row: its slot
-                    // number is not one of the wave's
-                    const bool sp = !NOSPEC && act[m] && w >= kCodeSpecial;
-                    // face entry: the wave's slot or the table row (both
-                    // computed, then selected: no branch); a workgroup on the
-                    // fast path has every slot below SLOTS
+                    // SLOW: a lane that does not update reads its table row
+                    // (its slot number is not one of the wave's); slots past
+                    // SLOTS load their faces below
+                    constexpr bool OLD = SLOW || !ENT;
+                    const int sl = OLD && !NOSPEC ? spx + __popc(spm & ((1u << j) - 1u)) : 0;
+                    const bool sp = OLD && !NOSPEC && act[m] && w >= kCodeSpecial;
                     const uint32_t se = (uint32_t)(TAB + wv * SLOTS) +
                                         (uint32_t)(SLOW ? min(sl, SLOTS - 1) : sl);
-                    const float4 *e = fac + 2u * (sp ? se : (w & 0x7fu));
+                    // otherwise the entry numbered at the start of the step
+                    const uint32_t ei =
+                        OLD ? (sp ? se : (w & 0x7fu)) : (ent[j >> 2] >> (8 * (j & 3))) & 0xffu;
+                    const float4 *e = fac + 2u * ei;
                     float4 A = e[0], B = e[1];
                     if (SLOW && sp && sl >= SLOTS) {
                         const uint32_t g = cvo[m] + (uint32_t)c + sxy * (uint32_t)z;
@@ -1069,7 +1090,8 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         store_plane(t0);
         load_coef(t0, nxt, UNI);
         load_plane(t0 + 1);
-        if (tid < 2) *wg_ovf(tid) = 0;
+        // the code table and the first two planes are read by every wave
+        __syncthreads();
         auto step = [&](int t, bool all_tag) {
             store_plane(t + 1);
 #pragma unroll
@@ -1084,43 +1106,37 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
                 load_coef(t + 1, nxt, UNI);
                 load_plane(t + 2);
             }
+            // One barrier per step.  Every half-sweep of step t updates only
+            // nodes with x + y + t even: their in-plane neighbours have the
+            // other parity and are not written during the step, and their z
+            // neighbours lie in the thread's own column, written by its own
+            // earlier half-sweeps or by its own store of plane t + 1.  The
+            // barrier follows the last half-sweep: the next step's in-plane
+            // neighbours and write-back read its values, and the next plane
+            // store refills the slot of plane t - 6, which this step's
+            // write-back and last half-sweep read.
             if (UNI) {
-                __syncthreads();
 #pragma unroll
-                for (int k = 1; k <= K; ++k) {
-                    stage(k, t, Fast{}, all_tag);
-                    __syncthreads();
-                }
+                for (int k = 1; k <= K; ++k) stage(k, t, Fast{}, all_tag);
+                __syncthreads();
                 fold_finish();
                 return;
             }
             specials_store();
-            // overflow flags alternate between two words: the one for step
-            // t + 1 is cleared here, after every wave has read the previous
-            // step's
-            if (total > SLOTS && lane == 0) *wg_ovf(t & 1) = 1;
-            if (tid == 0) *wg_ovf((t + 1) & 1) = 0;
-            __syncthreads();
-            // the branches are uniform per wave and each passes six barriers
-            if (__builtin_amdgcn_readfirstlane(*wg_ovf(t & 1))) {
+            // the branches are uniform per wave: a wave with more fractional-
+            // face updates than slots this step takes the SLOW path (its
+            // slots are its own, written and read by its own lanes)
+            if (total > SLOTS) {
 #pragma unroll
-                for (int k = 1; k <= K; ++k) {
-                    stage(k, t, Slow{}, all_tag);
-                    __syncthreads();
-                }
+                for (int k = 1; k <= K; ++k) stage(k, t, Slow{}, all_tag);
             } else if (nospec) {
 #pragma unroll
-                for (int k = 1; k <= K; ++k) {
-                    stage(k, t, NoSpec{}, all_tag);
-                    __syncthreads();
-                }
+                for (int k = 1; k <= K; ++k) stage(k, t, NoSpec{}, all_tag);
             } else {
 #pragma unroll
-                for (int k = 1; k <= K; ++k) {
-                    stage(k, t, Fast{}, all_tag);
-                    __syncthreads();
-                }
+                for (int k = 1; k <= K; ++k) stage(k, t, Fast{}, all_tag);
             }
+            __syncthreads();
             fold_finish();
         };
         // steps ts .. te have all six half-sweeps in range: those run without
