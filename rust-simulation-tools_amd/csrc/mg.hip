@@ -1649,7 +1649,9 @@ __global__ __launch_bounds__(256) void restrict_kernel(const MgLevelDev *__restr
     restrict_node(Fs[s], C, idx, false);
 }
 
-// One workgroup per solve runs the V-cycle over its tail levels.
+// One workgroup per solve runs the V-cycle over its tail levels.  Level
+// descriptors are read from the table with uniform (scalar) loads wherever
+// they are used, never copied into per-lane arrays.
 __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict__ Ts,
                                                     const MgStatus *__restrict__ st)
 {
@@ -1660,50 +1662,67 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
     __shared__ float lds[kLds];
     const int s = blockIdx.x;
     if (stopped(st + s)) return;
-    const int n = Ts[s].n;
+    const MgTailDev &T = Ts[s];
+    const int n = T.n;
     if (n == 0) return;
     const int tid = threadIdx.x, nt = blockDim.x;
-    MgLevelDev lv[kMaxTail];
-    for (int l = 0; l < n; ++l) lv[l] = Ts[s].lv[l];
-    float *entry_e = nullptr;  // global home of the entry level's correction, when in LDS
-    {
-        // from the coarsest level upwards, move levels into LDS while they
-        // fit: e, f and the residual, and the FP32 faces when those fit too.
-        // The entry level's right-hand side comes from the batched
-        // restriction and its correction is read back by the prolongation
-        // that follows, so they are copied in and out.
+    // LDS placement, from the coarsest level upwards while levels fit: e, f
+    // and the residual, plus the FP32 faces when those fit too (uniform)
+    struct Place {
+        int off;    // float offset of e (f at + N, res at + 2 N), or -1: global memory
+        bool full;  // faces at + 3 N .. + 6 N
+    };
+    auto place = [&](int l) {
         int used = 0;
-        for (int l = n - 1; l >= 0; --l) {
-            const int N = (int)nodes(lv[l]);
-            const bool full = used + (lv[l].uni ? 3 : 7) * N <= kLds;
+        for (int q = n - 1; q >= l; --q) {
+            const MgLevelDev &G = T.lv[q];
+            const int N = G.nx * G.ny * G.nz;
+            const bool full = used + (G.uni ? 3 : 7) * N <= kLds;
             if (!full && used + 3 * N > kLds) break;
-            float *base = lds + used;
-            if (l == 0) {
-                for (int i = tid; i < N; i += nt) base[N + i] = lv[0].f[i];
-                entry_e = lv[0].e;
-            }
-            lv[l].e = base;
-            lv[l].f = base + N;
-            lv[l].res = base + 2 * N;
-            used += 3 * N;
-            if (!full || lv[l].uni) continue;
-            {
-                float *cx = base + 3 * N, *cy = base + 4 * N, *cz = base + 5 * N, *ck = base + 6 * N;
-                for (int i = tid; i < N; i += nt) {
-                    cx[i] = lv[l].ex[i];
-                    cy[i] = lv[l].ey[i];
-                    cz[i] = lv[l].ez[i];
-                    ck[i] = lv[l].k2[i];
-                }
-                lv[l].ex = cx;
-                lv[l].ey = cy;
-                lv[l].ez = cz;
-                lv[l].k2 = ck;
-                used += 4 * N;
+            if (q == l) return Place{used, full && !G.uni};
+            used += (full && !G.uni) ? 7 * N : 3 * N;
+        }
+        return Place{-1, false};
+    };
+    auto level = [&](int l) {
+        MgLevelDev L = T.lv[l];
+        const Place p = place(l);
+        if (p.off >= 0) {
+            const int N = L.nx * L.ny * L.nz;
+            float *b = lds + p.off;
+            L.e = b;
+            L.f = b + N;
+            L.res = b + 2 * N;
+            if (p.full) {
+                L.ex = b + 3 * N;
+                L.ey = b + 4 * N;
+                L.ez = b + 5 * N;
+                L.k2 = b + 6 * N;
             }
         }
-        __syncthreads();
+        return L;
+    };
+    // copy in: the entry level's right-hand side (from the batched
+    // restriction) and the faces of levels placed whole; the entry level's
+    // correction goes back to global memory at the end, where the
+    // prolongation that follows reads it
+    for (int l = 0; l < n; ++l) {
+        const Place p = place(l);
+        if (p.off < 0) continue;
+        const MgLevelDev &G = T.lv[l];
+        const int N = G.nx * G.ny * G.nz;
+        float *b = lds + p.off;
+        if (l == 0)
+            for (int i = tid; i < N; i += nt) b[N + i] = G.f[i];
+        if (p.full)
+            for (int i = tid; i < N; i += nt) {
+                b[3 * N + i] = G.ex[i];
+                b[4 * N + i] = G.ey[i];
+                b[5 * N + i] = G.ez[i];
+                b[6 * N + i] = G.k2[i];
+            }
     }
+    __syncthreads();
     auto sweeps = [&](const MgLevelDev &L, int count) {
         const int64_t w = rb_work(L);
         for (int k = 0; k < count; ++k)
@@ -1716,30 +1735,43 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
             }
     };
     {
-        const MgLevelDev L0 = lv[0];
+        const MgLevelDev L0 = level(0);
         for (int64_t i = tid; i < nodes(L0); i += nt) L0.e[i] = 0.0f;
         __syncthreads();
     }
     for (int l = 0; l + 1 < n; ++l) {
-        const MgLevelDev L = lv[l];
-        const MgLevelDev C = lv[l + 1];
+        const MgLevelDev L = level(l);
+        const MgLevelDev C = level(l + 1);
         sweeps(L, 3);
         for (int64_t i = tid; i < nodes(L); i += nt) residual_node(L, i);
         __syncthreads();
         for (int64_t i = tid; i < nodes(C); i += nt) restrict_node(L, C, i, true);
         __syncthreads();
     }
-    sweeps(lv[n - 1], 50);
+    {
+        const MgLevelDev L = level(n - 1);
+        if (L.nx == 3 && L.ny == 3 && L.nz == 3) {
+            // one interior node, whose neighbours are boundary nodes that no
+            // sweep changes: its 50 updates (one per sweep, in the odd colour)
+            // need no other thread
+            if (tid == 0)
+                for (int k = 0; k < 50; ++k) rb_node(L, 13);
+            __syncthreads();
+        } else {
+            sweeps(L, 50);
+        }
+    }
     for (int l = n - 2; l >= 0; --l) {
-        const MgLevelDev L = lv[l];
-        const MgLevelDev C = lv[l + 1];
+        const MgLevelDev L = level(l);
+        const MgLevelDev C = level(l + 1);
         for (int64_t i = tid; i < nodes(L); i += nt) prolong_node(C, L, i);
         __syncthreads();
         sweeps(L, 3);
     }
-    if (entry_e) {
-        const MgLevelDev L0 = lv[0];
-        for (int64_t i = tid; i < nodes(L0); i += nt) entry_e[i] = L0.e[i];
+    if (place(0).off >= 0) {
+        const MgLevelDev L0 = level(0);
+        float *g = T.lv[0].e;
+        for (int64_t i = tid; i < nodes(L0); i += nt) g[i] = L0.e[i];
     }
 }
 
