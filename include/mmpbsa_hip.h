@@ -470,6 +470,8 @@ typedef struct mmpbsa_tune_entry {
     int64_t nodes_min, nodes_max;   /* level size range, nodes                     */
     int32_t solves_min, solves_max; /* solves sharing the launch                   */
     int32_t smoother_chunks;        /* z-chunks per column of the phase smoother   */
+    int32_t smoother_region;        /* workgroup region: 0 built-in, 1 32 x 32
+                                       nodes (20 x 20 core), 2 44 x 44 (32 x 32)  */
 } mmpbsa_tune_entry;
 
 typedef struct mmpbsa_tuning {
@@ -487,6 +489,9 @@ int mmpbsa_tuning_get(mmpbsa_tuning *t, mmpbsa_tune_entry *entries, int32_t cap)
 /* The smoother z-chunk length the solver would use for a level of nx*ny*nz
  * nodes in a group of n_solves solves (for reports). */
 int32_t mmpbsa_tuning_smoother_zc(int32_t nx, int32_t ny, int32_t nz, int32_t n_solves);
+/* The smoother region (1: 32 x 32, 2: 44 x 44) the solver would use for such
+ * a level (for reports). */
+int32_t mmpbsa_tuning_smoother_region(int32_t nx, int32_t ny, int32_t nz, int32_t n_solves);
 /* Observation for the autotuner: while on, every batched multigrid level the
  * solver lays out is recorded as (level nodes, solves in its group) with a
  * count.  Turning it on clears the record.  mmpbsa_tuning_observed copies up

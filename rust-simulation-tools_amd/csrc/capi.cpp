@@ -1381,6 +1381,9 @@ int mmpbsa_tuning_set(const mmpbsa_tuning *t)
         q.solves_min = e.solves_min;
         q.solves_max = e.solves_max;
         q.smoother_chunks = e.smoother_chunks;
+        if (e.smoother_region < 0 || e.smoother_region > 2)
+            return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "invalid smoother region");
+        q.smoother_region = e.smoother_region;
         v.entries.push_back(q);
     }
     set_gb_chunks_override(v.gb_chunks);
@@ -1402,13 +1405,20 @@ int mmpbsa_tuning_get(mmpbsa_tuning *t, mmpbsa_tune_entry *entries, int32_t cap)
         entries[k].solves_min = v.entries[k].solves_min;
         entries[k].solves_max = v.entries[k].solves_max;
         entries[k].smoother_chunks = v.entries[k].smoother_chunks;
+        entries[k].smoother_region = v.entries[k].smoother_region;
     }
     return MMPBSA_OK;
 }
 
 int32_t mmpbsa_tuning_smoother_zc(int32_t nx, int32_t ny, int32_t nz, int32_t n_solves)
 {
-    return tuned_smooth3_zc(tuning(), nx, ny, nz, n_solves);
+    const Tuning t = tuning();
+    return tuned_smooth3_zc(t, nx, ny, nz, n_solves, tuned_smooth3_region(t, nx, ny, nz, n_solves));
+}
+
+int32_t mmpbsa_tuning_smoother_region(int32_t nx, int32_t ny, int32_t nz, int32_t n_solves)
+{
+    return tuned_smooth3_region(tuning(), nx, ny, nz, n_solves) + 1;
 }
 
 int mmpbsa_tuning_observe(int32_t on)

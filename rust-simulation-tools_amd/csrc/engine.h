@@ -203,6 +203,7 @@ struct TuneEntry {
     int64_t nodes_min = 0, nodes_max = 0;  // level size range
     int solves_min = 0, solves_max = 0;    // solves sharing the launch
     int smoother_chunks = 0;               // z-chunks per column (0: heuristic)
+    int smoother_region = 0;               // 1: 32 x 32, 2: 44 x 44 (0: heuristic)
 };
 struct Tuning {
     int64_t tail_nodes = 16384;
@@ -217,7 +218,10 @@ void set_tuning(const Tuning &t);
 // the smoother's z-chunk length for a level of these dimensions in a launch of
 // n_solves solves: the tuned chunk count when an entry matches, else the
 // built-in heuristic (smooth3_zc)
-int tuned_smooth3_zc(const Tuning &t, int nx, int ny, int nz, int n_solves);
+int tuned_smooth3_zc(const Tuning &t, int nx, int ny, int nz, int n_solves, int rg);
+// the smoother region (0: 32 x 32, 1: 44 x 44) for such a level: the tuned
+// one when an entry with a region matches, else smooth3_region
+int tuned_smooth3_region(const Tuning &t, int nx, int ny, int nz, int n_solves);
 // record the (level nodes, group solves) of batched smoother levels
 void tuning_observe(bool on);
 int tuning_observed(int64_t *nodes, int32_t *solves, int64_t *count, int cap);

@@ -47,6 +47,7 @@ struct MgLevelDev {
     float *f;          // right-hand side
     float *res;        // residual scratch
     int zc;            // smoother: planes per workgroup (z-chunk length)
+    int rg;            // smoother region (host bookkeeping): 0 32 x 32, 1 44 x 44
     // finest level only (else null): the FP64 potential the phase smoother's
     // post-smoothing pass adds its final correction to, phi += e, in place of
     // storing e (the outer step then reads phi alone)
@@ -106,11 +107,15 @@ void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first
 // smoothing phase of the V-cycle), same buffers and flags as mg_smooth, with
 // smooth3_blocks workgroups per solve, L.zc from smooth3_zc.  The result
 // equals three mg_smooth passes bit for bit.  Needs 4 * nodes < 2^31.
-int smooth3_tiles(int nx, int ny);
-int smooth3_blocks(int nx, int ny, int nz, int zc);
-int smooth3_zc(int nx, int ny, int nz, int n_solves);
+// rg selects the (x, y) region of a workgroup: 0 = 32 x 32 nodes (a 20 x 20
+// core), 1 = 44 x 44 (32 x 32 core); every solve of a launch uses the same.
+int smooth3_region(int nx, int ny);  // the built-in choice for a level
+int smooth3_tiles(int nx, int ny, int rg);
+int smooth3_blocks(int nx, int ny, int nz, int zc, int rg);
+int smooth3_zc(int nx, int ny, int nz, int n_solves, int rg);
 void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
-                int max_blocks, const MgStatus *st, hipStream_t s, const uint32_t *wl = nullptr);
+                int max_blocks, int rg, const MgStatus *st, hipStream_t s,
+                const uint32_t *wl = nullptr);
 // z-streaming version of prolong_e2 (same results bit for bit): one thread
 // per in-plane node over L.zc planes, mg_stream_blocks workgroups per solve.
 int mg_stream_blocks(int nx, int ny, int nz, int zc);

@@ -564,55 +564,148 @@ void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first
 namespace s3 {
 constexpr int K = 6;                   // half-sweeps per pass
 constexpr int H = K;                   // halo
-// (A 32 x 40 region with 320 threads, 15% fewer updates per plane on 161^3,
-// measured 22% slower: the 5-wave workgroups lower the waves per CU.)
-constexpr int RW = 32, RH = 32, RN = RW * RH;  // region: 32 columns x 32 rows
-constexpr int NT = 256;                // threads (4 waves)
-constexpr int NW = NT / 64;            // waves
-constexpr int TX = RW - 2 * H;         // core width: 20
-constexpr int TY = RH - 2 * H;         // core height: 20
 constexpr int RING = 8;                // correction planes t-6 .. t+1
-constexpr int PF = RN / NT;            // plane values per thread: 4
-constexpr int PP = RN / 2 / NT;        // column pairs per thread: 2
-constexpr int PR = NT / 16;            // region rows between a thread's pairs: 16
-constexpr int WX = TX + 2, WY = TY + 2;      // written nodes: core plus grid-face nodes
-constexpr int WB = (WX * WY + NT - 1) / NT;  // values written back per thread: 2
 constexpr int SLOTS = 32;              // fractional-face slots per wave and step
 constexpr int TAB = 128;               // code-table entries
 constexpr uint32_t kOob = 0x80000000u; // offset outside every array
 constexpr int kRsrcWord3 = 0x00020000; // raw buffer, 32-bit data format
 static_assert(RING >= K + 2 && (RING & (RING - 1)) == 0, "ring holds planes t-K .. t+1");
-static_assert(PP * NT * 2 == RN && PF * NT == RN && RW == 32 && NT % 64 == 0 && TAB <= NT,
-              "region layout");
-// both column pairs of a thread share the red-black parity of their rows
-static_assert(PR % 2 == 0, "pair rows of one parity");
 // Tiles and z-chunks cover the interior nodes 1 .. n - 2 of each axis (on
 // 161^3, 8 x 8 tiles instead of 9 x 9); a tile on a face of the grid also
 // writes back the boundary nodes beside its core.
 __host__ __device__ constexpr int inner(int n) { return n > 2 ? n - 2 : 1; }
+
+// Region geometry: RW columns x RH rows, a core of (RW - 2 H) x (RH - 2 H).
+// Thread t < NP owns column pairs t and t + NP (rows t / PPR and t / PPR + PR,
+// columns 2 (t % PPR) and the next); NT rounds NP up to whole waves, and the
+// threads past NP own no pair (they load, store and write back only their
+// share of the plane and write-back rectangle).
+template <int RW_, int RH_>
+struct Geo {
+    static constexpr int RW = RW_, RH = RH_, RN = RW * RH;
+    static constexpr int TX = RW - 2 * H, TY = RH - 2 * H;  // core
+    static constexpr int PPR = RW / 2;                      // column pairs per row
+    static constexpr int PR = RH / 2;                       // rows between a thread's pairs
+    static constexpr int NP = RN / 4;                       // threads owning pairs
+    static constexpr int NT = (NP + 63) / 64 * 64;          // threads
+    static constexpr int NW = NT / 64;                      // waves
+    static constexpr int PF = 4, PP = 2;                    // plane values, pairs per thread
+    static constexpr int WX = TX + 2, WY = TY + 2;          // written nodes: core plus faces
+    static constexpr int WB = (WX * WY + NT - 1) / NT;      // written back per thread
+    // LDS bytes: the correction ring and the face entries (table + slots)
+    static constexpr int LDS = RING * RN * 4 + 2 * (TAB + NW * SLOTS) * 16;
+    // both pairs of a thread share the red-black parity of their rows
+    static_assert(RW % 2 == 0 && PR % 2 == 0 && TAB <= NT, "region layout");
+};
+// 32 x 32 (20 x 20 core): 256 threads, 40 KB, 4 workgroups per CU.
+// 44 x 44 (32 x 32 core): 512 threads (484 own pairs), 72.5 KB, 2 per CU.
+// Both give 16 waves per CU.  (A 32 x 40 region with 320 threads measured 22%
+// slower than 32 x 32 on 161^3: its 5-wave workgroups lowered the waves per CU.)
+using Small = Geo<32, 32>;
+using Large = Geo<44, 44>;
+
+// Thread -> pair of the 44 x 44 region, row << 8 | pair column (0xffff: no
+// pair).  The LDS has 32 banks of 4 bytes; a half-sweep's reads by one wave
+// hit bank (44 r + 2 i + c_r) mod 32, where the active column c_r alternates
+// with the row.  A wave spanning whole 22-pair rows puts up to three lanes on
+// one bank (36% of LDS cycles were conflicts); 4 rows x 16 pairs cover every
+// bank exactly twice.  Waves 0-4 take such blocks of rows 0-19; the other 164
+// pairs go, one by one, to whichever of waves 5-7 has the lower count on the
+// two banks the pair's active column meets (one per step parity).
+struct PairMap {
+    uint16_t at[512];
+};
+constexpr PairMap large_pair_map()
+{
+    PairMap m{};
+    for (int t = 0; t < 512; ++t) m.at[t] = 0xffff;
+    for (int w = 0; w < 5; ++w)
+        for (int l = 0; l < 64; ++l)
+            m.at[64 * w + l] = (uint16_t)(((4 * w + l / 16) << 8) | (l % 16));
+    int cnt[3][2][32] = {};
+    int fill[3] = {0, 0, 0};
+    const int cap[3] = {64, 64, 36};
+    auto place = [&](int r, int i) {
+        int best = -1, bm = 0;
+        for (int k = 0; k < 3; ++k) {
+            if (fill[k] == cap[k]) continue;
+            int mk = 0;
+            for (int c0 = 0; c0 < 2; ++c0) {
+                const int b = (44 * r + 2 * i + ((r + c0) & 1)) % 32;
+                mk = cnt[k][c0][b] > mk ? cnt[k][c0][b] : mk;
+            }
+            if (best < 0 || mk < bm) {
+                best = k;
+                bm = mk;
+            }
+        }
+        for (int c0 = 0; c0 < 2; ++c0) ++cnt[best][c0][(44 * r + 2 * i + ((r + c0) & 1)) % 32];
+        m.at[64 * (5 + best) + fill[best]++] = (uint16_t)((r << 8) | i);
+    };
+    for (int r = 20; r < 22; ++r)
+        for (int i = 0; i < 16; ++i) place(r, i);
+    for (int r = 0; r < 22; ++r)
+        for (int i = 16; i < 22; ++i) place(r, i);
+    return m;
+}
+constexpr bool pair_map_is_bijection(const PairMap &m)
+{
+    bool seen[22][22] = {};
+    int n = 0;
+    for (int t = 0; t < 512; ++t) {
+        if (m.at[t] == 0xffff) continue;
+        const int r = m.at[t] >> 8, i = m.at[t] & 0xff;
+        if (r >= 22 || i >= 22 || seen[r][i]) return false;
+        seen[r][i] = true;
+        ++n;
+    }
+    return n == 484;
+}
+static_assert(pair_map_is_bijection(large_pair_map()), "every pair of the large region owned once");
+__device__ constexpr PairMap kLargePairs = large_pair_map();
+static_assert(Small::NT == 256 && Large::NT == 512, "thread counts");
+static_assert(4 * Small::LDS <= 160 * 1024 && 2 * Large::LDS <= 160 * 1024, "LDS per CU");
+template <class G>
+int tiles(int nx, int ny)
+{
+    return ceil_div(inner(nx), G::TX) * ceil_div(inner(ny), G::TY);
+}
 }  // namespace s3
 
-int smooth3_tiles(int nx, int ny)
+int smooth3_tiles(int nx, int ny, int rg)
 {
-    return ceil_div(s3::inner(nx), s3::TX) * ceil_div(s3::inner(ny), s3::TY);
+    return rg ? s3::tiles<s3::Large>(nx, ny) : s3::tiles<s3::Small>(nx, ny);
 }
 
-int smooth3_blocks(int nx, int ny, int nz, int zc)
+int smooth3_blocks(int nx, int ny, int nz, int zc, int rg)
 {
-    return smooth3_tiles(nx, ny) * ceil_div(s3::inner(nz), zc);
+    return smooth3_tiles(nx, ny, rg) * ceil_div(s3::inner(nz), zc);
+}
+
+// Region of a level: the 44 x 44 region when its lanes per plane (tiles x
+// threads, each thread updating two pairs per half-sweep) come to at most 90%
+// of the 32 x 32 region's.  On 161^3 that is 25 x 512 against 64 x 256 lanes;
+// 81^3 (9 x 512 against 16 x 256) and 41^3 keep the small region.
+int smooth3_region(int nx, int ny)
+{
+    const int64_t small = (int64_t)s3::tiles<s3::Small>(nx, ny) * s3::Small::NT;
+    const int64_t large = (int64_t)s3::tiles<s3::Large>(nx, ny) * s3::Large::NT;
+    return 10 * large <= 9 * small ? 1 : 0;
 }
 
 // Planes per workgroup: about 96 (a chunk also streams 2 K - 2 = 10 planes it
 // does not write), shorter when a launch of n_solves such levels would
-// otherwise have fewer than 2048 workgroups (8 per CU), but at least 8.
+// otherwise have fewer than 2048 workgroups (8 per CU) of the small region's
+// size (1024 of the large one's), but at least 8.
 // Measured on the 161^3 batch: chunks of ~96 planes (two per column) beat
 // 64, 40 and 32; whole columns were no faster (the halo reads of
 // neighbouring tiles then meet less often in L2).
-int smooth3_zc(int nx, int ny, int nz, int n_solves)
+int smooth3_zc(int nx, int ny, int nz, int n_solves, int rg)
 {
-    const int64_t tiles = (int64_t)smooth3_tiles(nx, ny) * std::max(n_solves, 1);
+    const int64_t tiles = (int64_t)smooth3_tiles(nx, ny, rg) * std::max(n_solves, 1);
+    const int64_t floor = rg ? 1024 : 2048;
     int nch = std::max(1, (nz + 48) / 96);
-    nch = std::max<int>(nch, (int)((2048 + tiles - 1) / tiles));
+    nch = std::max<int>(nch, (int)((floor + tiles - 1) / tiles));
     nch = std::min(nch, std::max(1, nz / 8));
     return ceil_div(nz, nch);
 }
@@ -641,13 +734,18 @@ __device__ __forceinline__ void bstore64(__amdgpu_buffer_rsrc_t r, double v, uin
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)vo, (int)so, 0);
 }
 
-template <bool FIRST, bool PROLONG>
-__global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__restrict__ Ls,
-                                                      const MgLevelDev *__restrict__ Cs, int dir,
-                                                      const MgStatus *__restrict__ st,
-                                                      const uint32_t *__restrict__ wl)
+// four waves per SIMD (128 VGPRs) for both geometries
+template <bool FIRST, bool PROLONG, class G>
+__global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__restrict__ Ls,
+                                                          const MgLevelDev *__restrict__ Cs,
+                                                          int dir,
+                                                          const MgStatus *__restrict__ st,
+                                                          const uint32_t *__restrict__ wl)
 {
     using namespace s3;
+    constexpr int RW = G::RW, RH = G::RH, RN = G::RN, NT = G::NT, NW = G::NW, TX = G::TX,
+                  TY = G::TY, PF = G::PF, PP = G::PP, PR = G::PR, PPR = G::PPR, NP = G::NP,
+                  WX = G::WX, WB = G::WB;
     __shared__ float ring[RING * RN];
     // face entries {a0..a3}, {a4, a5, 1/diag, -}: rows 0..127 the code table,
     // then SLOTS per wave for the step's fractional-face nodes
@@ -710,18 +808,33 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     }
 
     // column pairs m: region row ry = (tid >> 4) + PR m, columns rx0, rx0 + 1
-    const int ry0 = tid >> 4, rx0 = 2 * (tid & 15);
-    const int li0 = ry0 * RW + rx0;  // LDS index of column 0 of pair 0 (pair 1: + 16 RW)
+    // (threads without a pair: every offset out of range, ring distance 0)
+    bool own;
+    int ry0, rx0;
+    if constexpr (RW == 44) {
+        const uint32_t pm = kLargePairs.at[tid];
+        own = pm != 0xffffu;
+        ry0 = own ? (int)(pm >> 8) : 0;
+        rx0 = own ? 2 * (int)(pm & 0xffu) : 0;
+    } else {
+        own = tid < NP;
+        ry0 = own ? tid / PPR : 0;
+        rx0 = own ? 2 * (tid % PPR) : 0;
+    }
+    const int li0 = ry0 * RW + rx0;  // LDS index of column 0 of pair 0 (pair 1: + PR RW)
     // plane loads: a thread loads and stores the four nodes of its own column
     // pairs, value j = 2 m + c at region row ry0 + PR m, column rx0 + c, so
-    // that within a step it alone reads the plane it stores
+    // that within a step it alone reads the plane it stores.  One offset per
+    // pair, that of its second column (the first is 4 bytes below).  Region
+    // nodes outside the grid are never read by an update nor written back, so
+    // their values do not matter: only in-grid nodes need exact offsets, and
+    // theirs are never negative (x >= 0, y >= 0).  Any other offset either
+    // wraps past the array (the load returns zero) or reads another node.
     const int lx = xl + rx0, ly = yl + ry0;
-    uint32_t pvo[PF];
+    uint32_t pvo[PP];
 #pragma unroll
-    for (int j = 0; j < PF; ++j) {
-        const int x = lx + (j & 1), y = ly + PR * (j >> 1);
-        pvo[j] = (x >= 0 && x < nx && y >= 0 && y < ny) ? 4u * (uint32_t)(x + nx * y) : kOob;
-    }
+    for (int m = 0; m < PP; ++m)
+        pvo[m] = own ? 4u * (uint32_t)(lx + 1 + nx * (ly + PR * m)) : kOob;
     const int par = (xl + rx0 + yl + ry0) & 1;  // active column at step t: par ^ (t & 1)
     uint32_t cvo[PP];   // node index of column 0 (kOob when the pair leaves the grid)
     uint32_t rdpk = 0;  // byte 2m + c: ring distance of column c if it is updated, else 0
@@ -729,13 +842,13 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
 #pragma unroll
     for (int m = 0; m < PP; ++m) {
         const int ry = ry0 + PR * m, y = yl + ry, x = xl + rx0;
-        const bool pin = y >= 0 && y < ny && x >= 0 && x + 1 < nx;
+        const bool pin = own && y >= 0 && y < ny && x >= 0 && x + 1 < nx;
         cvo[m] = pin ? (uint32_t)(x + nx * y) : kOob;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const int rx = rx0 + c, xa = x + c;
             const int rd = min(min(rx, RW - 1 - rx), min(ry, RH - 1 - ry));
-            const bool in = xa >= 1 && xa <= nx - 2 && y >= 1 && y <= ny - 2;
+            const bool in = own && xa >= 1 && xa <= nx - 2 && y >= 1 && y <= ny - 2;
             rdpk |= (uint32_t)(in ? rd : 0) << (8 * (2 * m + c));
         }
     }
@@ -770,10 +883,12 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         const uint32_t zo = 4u * sxy * (uint32_t)z;
 #pragma unroll
         for (int j = 0; j < PF; ++j) {
-            float x = bload(src, pvo[j], zo);
+            const uint32_t o = pvo[j >> 1];
+            float x = bload(src, (j & 1) ? o : o - 4u, zo);
             if (PROLONG) {
-                const bool v = pvo[j] != kOob;
-                x += prolong_value(C, ceg, v ? lx + (j & 1) : 0, v ? ly + PR * (j >> 1) : 0, z);
+                const int gx = lx + (j & 1), gy = ly + PR * (j >> 1);
+                const bool v = own && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+                x += prolong_value(C, ceg, v ? gx : 0, v ? gy : 0, z);
                 x = v ? x : 0.0f;
             }
             pf[j] = x;
@@ -781,31 +896,29 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     };
     auto store_plane = [&](int z) {
         float *p = slot(z);
+        if (own)
 #pragma unroll
-        for (int j = 0; j < PF; ++j) p[li0 + PR * RW * (j >> 1) + (j & 1)] = pf[j];
+            for (int j = 0; j < PF; ++j) p[li0 + PR * RW * (j >> 1) + (j & 1)] = pf[j];
     };
     // finest level, post-smoothing: phi += e in FP64 instead of storing e (the
-    // outer step's own addition).  At the write-back point the final values of
-    // plane z are read from the ring and the loads of phi are issued; the
-    // addition and store follow the step's six half-sweeps (fold_finish), so
-    // the loads are not waited for (the ring slot itself is refilled at the
-    // next step's start, behind no barrier, so it is not read again).  Lanes
-    // outside the core are skipped by predicate (8 N may exceed kOob).
+    // outer step's own addition).  At the write-back point of plane z the loads
+    // of phi are issued; after the step's six half-sweeps, before its barrier,
+    // the final values of plane z (unchanged by the step) are read from the
+    // ring, added and stored (fold_finish), so the loads are not waited for and
+    // no ring value is held across the half-sweeps.  The slot is refilled only
+    // after the barrier.  Lanes outside the core are skipped by predicate (8 N
+    // may exceed kOob).
     const bool fold = !FIRST && L.phi != nullptr;
     const auto phr = buf_rsrc(fold ? L.phi : nullptr, fold ? 8u * N : 0u);
     double phv[WB];
-    float fev[WB];
     int fold_z = -1;
     auto write_back = [&](int z) {
         if (z < wz0 || z >= wz1) return;
         const uint32_t zo = 4u * sxy * (uint32_t)z;
         if (fold) {
-            const float *p = slot(z);
 #pragma unroll
-            for (int j = 0; j < WB; ++j) {
-                fev[j] = p[wli[j]];
+            for (int j = 0; j < WB; ++j)
                 phv[j] = wvo[j] != kOob ? bload64(phr, 2u * wvo[j], 2u * zo) : 0.0;
-            }
             fold_z = z;
             return;
         }
@@ -818,9 +931,12 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     auto fold_finish = [&]() {
         if (!fold || fold_z < 0) return;
         const uint32_t zo = 4u * sxy * (uint32_t)fold_z;
+        const float *p = slot(fold_z);
 #pragma unroll
-        for (int j = 0; j < WB; ++j)
-            if (wvo[j] != kOob) bstore64(phr, phv[j] + (double)fev[j], 2u * wvo[j], 2u * zo);
+        for (int j = 0; j < WB; ++j) {
+            const float ev = p[wli[j]];
+            if (wvo[j] != kOob) bstore64(phr, phv[j] + (double)ev, 2u * wvo[j], 2u * zo);
+        }
         fold_z = -1;
     };
     // coefficients of a plane: f of both columns of both pairs, code bytes packed
@@ -867,7 +983,9 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
     // the step loop, compiled separately for uniform and non-uniform levels
     auto run = [&](auto uni_tag) {
         constexpr bool UNI = decltype(uni_tag)::value;
-        constexpr bool ENT = FIRST;
+        // entry indices packed in bytes: only when every entry (table rows,
+        // then SLOTS per wave) is below 256, i.e. the 32 x 32 region
+        constexpr bool ENT = FIRST && TAB + NW * SLOTS <= 256;
         Coef Q[K];  // Q[i]: plane t - i
         Next nxt;
 
@@ -882,9 +1000,9 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
         bool nospec = false;  // no code word of the wave's six planes has the fractional bit
         float sa[6];
         uint32_t sreq = 0xffffffffu;
-        // face entries of the step's updates: byte j & 3 of ent[j >> 2] is the
-        // entry (table row, or 128 + the wave's slot) of update j = 2 (k - 1) +
-        // m, valid outside the SLOW mode
+        // face entries of the step's updates (ENT): byte j & 3 of ent[j >> 2]
+        // is the entry of update j = 2 (k - 1) + m, a table row or the wave's
+        // slot TAB + wave * SLOTS + slot, valid outside the SLOW mode
         uint32_t ent[3] = {0u, 0u, 0u};
         auto specials_issue = [&](int t, bool all) {
             // bytes c and 2 + c (the active column of both pairs) of the code
@@ -1118,8 +1236,8 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
             if (UNI) {
 #pragma unroll
                 for (int k = 1; k <= K; ++k) stage(k, t, Fast{}, all_tag);
-                __syncthreads();
                 fold_finish();
+                __syncthreads();
                 return;
             }
             specials_store();
@@ -1136,8 +1254,8 @@ __global__ __launch_bounds__(s3::NT, 4) void smooth3_kernel(const MgLevelDev *__
 #pragma unroll
                 for (int k = 1; k <= K; ++k) stage(k, t, Fast{}, all_tag);
             }
-            __syncthreads();
             fold_finish();
+            __syncthreads();
         };
         // steps ts .. te have all six half-sweeps in range: those run without
         // the per-half-sweep range tests
@@ -1184,18 +1302,25 @@ void mg_prolong_e2(const MgLevelDev *F, const MgLevelDev *C, int n_solves, int64
 }
 
 void mg_smooth3(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first, int dir,
-                int max_blocks, const MgStatus *st, hipStream_t s, const uint32_t *wl)
+                int max_blocks, int rg, const MgStatus *st, hipStream_t s, const uint32_t *wl)
 {
     if (n_solves <= 0 || max_blocks <= 0) return;
     const dim3 grid = batch_grid(max_blocks, n_solves, wl);
     // the prolongation runs as its own kernel (mg_prolong_e2) before this pass
     if (C) throw DeviceError(1, "mg_smooth3: no fused prolongation");  // invalid argument
-    if (first)
-        hipLaunchKernelGGL((smooth3_kernel<true, false>), grid, dim3(s3::NT), 0, s, L, C, dir, st,
-                           wl);
+    auto launch = [&](auto geo) {
+        using G = decltype(geo);
+        if (first)
+            hipLaunchKernelGGL((smooth3_kernel<true, false, G>), grid, dim3(G::NT), 0, s, L, C,
+                               dir, st, wl);
+        else
+            hipLaunchKernelGGL((smooth3_kernel<false, false, G>), grid, dim3(G::NT), 0, s, L, C,
+                               dir, st, wl);
+    };
+    if (rg)
+        launch(s3::Large{});
     else
-        hipLaunchKernelGGL((smooth3_kernel<false, false>), grid, dim3(s3::NT), 0, s, L, C, dir, st,
-                           wl);
+        launch(s3::Small{});
     LAUNCH_OK();
 }
 

@@ -92,17 +92,29 @@ static void observe_level(int64_t nodes, int n_solves)
     if (g_observe) observed()[{nodes, n_solves}] += 1;
 }
 
-int tuned_smooth3_zc(const Tuning &t, int nx, int ny, int nz, int n_solves)
+static bool entry_holds(const TuneEntry &e, int64_t nodes, int n_solves)
+{
+    return nodes >= e.nodes_min && nodes <= e.nodes_max && n_solves >= e.solves_min &&
+           n_solves <= e.solves_max;
+}
+
+int tuned_smooth3_zc(const Tuning &t, int nx, int ny, int nz, int n_solves, int rg)
 {
     const int64_t nodes = (int64_t)nx * ny * nz;
     for (const TuneEntry &e : t.entries) {
-        if (nodes < e.nodes_min || nodes > e.nodes_max || n_solves < e.solves_min ||
-            n_solves > e.solves_max || e.smoother_chunks <= 0)
-            continue;
+        if (!entry_holds(e, nodes, n_solves) || e.smoother_chunks <= 0) continue;
         const int nch = std::min(e.smoother_chunks, std::max(1, nz - 2));
         return ceil_div(std::max(1, nz - 2), nch);
     }
-    return smooth3_zc(nx, ny, nz, n_solves);
+    return smooth3_zc(nx, ny, nz, n_solves, rg);
+}
+
+int tuned_smooth3_region(const Tuning &t, int nx, int ny, int nz, int n_solves)
+{
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    for (const TuneEntry &e : t.entries)
+        if (entry_holds(e, nodes, n_solves) && e.smoother_region > 0) return e.smoother_region - 1;
+    return smooth3_region(nx, ny);
 }
 
 ProfileStats &profile_stats()
@@ -564,6 +576,29 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     // (one FP64 potential per solve, no ping-pong copy; per solve: TaskWork::fold)
     const bool fold = phase_pass;
 
+    // smoother region per level index: the solves of a launch share one, that
+    // of the group's largest grid at the level (levels as built below)
+    std::vector<int> level_rg;
+    {
+        std::vector<int64_t> largest;
+        for (MgTask *t : mgt) {
+            int d[3] = {t->g.nx, t->g.ny, t->g.nz};
+            for (size_t l = 0;; ++l) {
+                const int64_t N = (int64_t)d[0] * d[1] * d[2];
+                if (l == largest.size()) {
+                    largest.push_back(-1);
+                    level_rg.push_back(0);
+                }
+                if (N > largest[l]) {
+                    largest[l] = N;
+                    level_rg[l] = tuned_smooth3_region(tun, d[0], d[1], d[2], S);
+                }
+                if (std::min(d[0], std::min(d[1], d[2])) <= 5) break;
+                for (int k = 0; k < 3; ++k) d[k] = (d[k] + 1) / 2;
+            }
+        }
+    }
+
     std::vector<TaskWork> W(S);
     int maxT = 0;
     for (int s = 0; s < S; ++s) {
@@ -611,7 +646,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             const float ih = (float)(1.0 / (lv.g.h * lv.g.h));
             dv.ihx = dv.ihy = dv.ihz = ih;
             dv.omega = (float)estimate_omega(lv.g.nx, lv.g.ny, lv.g.nz);
-            dv.zc = phase_pass ? tuned_smooth3_zc(tun, lv.g.nx, lv.g.ny, lv.g.nz, S)
+            dv.rg = level_rg[l];
+            dv.zc = phase_pass ? tuned_smooth3_zc(tun, lv.g.nx, lv.g.ny, lv.g.nz, S, dv.rg)
                                : smooth_zc(lv.g.nz);
             if (l < tw.tail_start) observe_level(N, S);
             dv.e = lv.e.get();
@@ -740,7 +776,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 H[p].C[(size_t)l * S + s] = lev(l + 1);
                 const GridDesc &g = tw.L[l].g;
                 sblocks[l] = std::max(sblocks[l],
-                                      phase_pass ? smooth3_blocks(g.nx, g.ny, g.nz, tw.L[l].d.zc)
+                                      phase_pass ? smooth3_blocks(g.nx, g.ny, g.nz, tw.L[l].d.zc,
+                                                                  tw.L[l].d.rg)
                                                  : smooth_tiles(g.nx, g.ny) *
                                                        ceil_div(g.nz, smooth_zc(g.nz)));
                 if (phase_pass && 4 * lv_nodes(g) >= ((int64_t)1 << 31))
@@ -822,7 +859,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         const GridDesc &g = tw.L[l].g;
         switch (kind) {
         case 0:
-            return phase_pass ? smooth3_blocks(g.nx, g.ny, g.nz, d.zc)
+            return phase_pass ? smooth3_blocks(g.nx, g.ny, g.nz, d.zc, d.rg)
                               : (int64_t)smooth_tiles(g.nx, g.ny) * ceil_div(g.nz, d.zc);
         case 1:
             return stream ? mg_stream_blocks(g.nx, g.ny, g.nz, d.zc) : ceil_div(lv_nodes(g), 256);
@@ -906,8 +943,8 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                 HIP_OK(hipEventCreate(&ev.b));
                 HIP_OK(hipEventRecord(ev.a, w.stream));
             }
-            mg_smooth3(F, nullptr, S, pre ? 1 : 0, pre ? 1 : 0, (int)tot_of(l), dst, w.stream,
-                       wl_of(l));
+            mg_smooth3(F, nullptr, S, pre ? 1 : 0, pre ? 1 : 0, (int)tot_of(l), level_rg[l], dst,
+                       w.stream, wl_of(l));
             if (profiling && l == 0) {
                 HIP_OK(hipEventRecord(ev.b, w.stream));
                 evs.push_back(ev);

@@ -200,7 +200,8 @@ class RequestStatsC(C.Structure):
 
 class TuneEntryC(C.Structure):
     _fields_ = [("nodes_min", C.c_int64), ("nodes_max", C.c_int64), ("solves_min", C.c_int32),
-                ("solves_max", C.c_int32), ("smoother_chunks", C.c_int32)]
+                ("solves_max", C.c_int32), ("smoother_chunks", C.c_int32),
+                ("smoother_region", C.c_int32)]
 
 
 class TuningC(C.Structure):
@@ -298,6 +299,7 @@ SIGNATURES = {
     "mmpbsa_tuning_set": (C.c_int, [C.POINTER(TuningC)]),
     "mmpbsa_tuning_get": (C.c_int, [C.POINTER(TuningC), C.POINTER(TuneEntryC), C.c_int32]),
     "mmpbsa_tuning_smoother_zc": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "mmpbsa_tuning_smoother_region": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "mmpbsa_tuning_observe": (C.c_int, [C.c_int32]),
     "mmpbsa_tuning_observed": (C.c_int32, [c_int64_p, c_int32_p, c_int64_p, C.c_int32]),
     "mmpbsa_profile_enable": (C.c_int, [C.c_int32]),

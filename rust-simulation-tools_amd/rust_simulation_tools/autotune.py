@@ -3,18 +3,19 @@ reference, which fixes 8^3 SOR tiles and its thread chunking:
 pb_solver.rs:214-327, pb_grid.rs:233).
 
 A table maps (multigrid level size, solves sharing the launch) to the phase
-smoother's z-chunk count, together with the host check interval, the
-Born-radius kernel's j-chunking and the serving loop's round size.  None of
-the smoother choices changes a result bit (the smoother, the streaming
-restriction and prolongation are exact for any z-chunking), so a tuned run
-returns the energies of an untuned one.
+smoother's workgroup region (32 x 32 or 44 x 44 nodes) and z-chunk count,
+together with the host check interval, the Born-radius kernel's j-chunking
+and the serving loop's round size.  None of the smoother choices changes a
+result bit (the smoother, the streaming restriction and prolongation are
+exact for any tiling and z-chunking), so a tuned run returns the energies of
+an untuned one.
 
 Search: the workload is run once with observation on, which records the
 level shapes the solver lays out; shapes are grouped into size buckets
 (powers of two of the node count) weighted by node count x occurrences;
-then, heaviest bucket first, each candidate chunk count is timed on the
-workload and kept only when it beats the incumbent by more than the noise
-margin.  Tables persist as JSON keyed by device.
+then, heaviest bucket first, each region and then each chunk count is timed
+on the workload and kept only when it beats the incumbent by more than the
+noise margin.  Tables persist as JSON keyed by device.
 """
 from __future__ import annotations
 
@@ -36,7 +37,8 @@ def _lib():
 
 class TuneTable:
     """entries: [{"nodes_min", "nodes_max", "solves_min", "solves_max",
-    "smoother_chunks"}], check_cycles, gb_chunks (0 = built-in), round_frames
+    "smoother_chunks" (0 = built-in), "smoother_region" (0 = built-in, 1 = 32 x
+    32, 2 = 44 x 44)}], check_cycles, gb_chunks (0 = built-in), round_frames
     (the serving loop's round size; None = its default)."""
 
     def __init__(self, entries=None, check_cycles=DEFAULT_CHECK_CYCLES, gb_chunks=0,
@@ -68,7 +70,8 @@ def apply(table: TuneTable | None):
         arr = (TuneEntryC * max(1, len(table.entries)))()
         for k, e in enumerate(table.entries):
             arr[k] = TuneEntryC(int(e["nodes_min"]), int(e["nodes_max"]), int(e["solves_min"]),
-                                int(e["solves_max"]), int(e["smoother_chunks"]))
+                                int(e["solves_max"]), int(e.get("smoother_chunks", 0)),
+                                int(e.get("smoother_region", 0)))
         t = TuningC(table.check_cycles, table.gb_chunks, len(table.entries), arr)
         rc = lib.mmpbsa_tuning_set(C.byref(t))
     if rc != 0:
@@ -84,7 +87,8 @@ def current() -> TuneTable:
     lib.mmpbsa_tuning_get(C.byref(t), arr, n)
     entries = [{"nodes_min": arr[k].nodes_min, "nodes_max": arr[k].nodes_max,
                 "solves_min": arr[k].solves_min, "solves_max": arr[k].solves_max,
-                "smoother_chunks": arr[k].smoother_chunks} for k in range(n)]
+                "smoother_chunks": arr[k].smoother_chunks,
+                "smoother_region": arr[k].smoother_region} for k in range(n)]
     return TuneTable(entries, t.check_cycles, t.gb_chunks)
 
 
@@ -156,11 +160,11 @@ def _time(run, repeats):
     return best
 
 
-def tune(run, max_buckets=4, candidates=(1, 2, 3, 4, 6), repeats=2, margin=0.01,
-         check_cycles=(8, 4, 16), base: TuneTable | None = None, log=None):
-    """Tune the smoother chunk counts of the heaviest level buckets the
-    workload `run` (a callable) exercises, then the check interval.  Returns
-    (table, report).  The table is left applied."""
+def tune(run, max_buckets=4, candidates=(1, 2, 3, 4, 6), regions=(1, 2), repeats=2,
+         margin=0.01, check_cycles=(8, 4, 16), base: TuneTable | None = None, log=None):
+    """Tune the smoother region and chunk count of the heaviest level buckets
+    the workload `run` (a callable) exercises, then the check interval.
+    Returns (table, report).  The table is left applied."""
     table = (base or TuneTable()).copy()
     apply(table)
     run()  # first-use costs
@@ -171,24 +175,33 @@ def tune(run, max_buckets=4, candidates=(1, 2, 3, 4, 6), repeats=2, margin=0.01,
     report = {"default_s": t_default, "buckets": []}
     for bk in bks:
         entry = {k: bk[k] for k in ("nodes_min", "nodes_max", "solves_min", "solves_max")}
-        choice, tried = 0, {}
-        for c in candidates:
+
+        def with_entry(**kw):
             trial = table.copy()
+            old = [e for e in trial.entries if e["nodes_min"] == entry["nodes_min"]]
+            cur = dict(old[0]) if old else dict(entry, smoother_chunks=0, smoother_region=0)
+            cur.update(kw)
             trial.entries = [e for e in trial.entries if e["nodes_min"] != entry["nodes_min"]]
-            trial.entries.append(dict(entry, smoother_chunks=c))
-            apply(trial)
-            t = _time(run, repeats)
-            tried[c] = t
-            if log:
-                log(f"autotune: nodes [{entry['nodes_min']}, {entry['nodes_max']}] chunks {c}: "
-                    f"{t * 1e3:.1f} ms")
-            if t < best_t * (1.0 - margin):
-                best_t, choice, best_table = t, c, trial
-        if choice:
-            table = best_table
-        apply(table)
-        report["buckets"].append(dict(entry, weight=bk["weight"], chosen=choice or "built-in",
-                                      tried_ms={str(k): v * 1e3 for k, v in tried.items()}))
+            trial.entries.append(cur)
+            return trial
+
+        picked, tried = {}, {}
+        for key, values in (("smoother_region", regions), ("smoother_chunks", candidates)):
+            for v in values:
+                trial = with_entry(**{key: v})
+                apply(trial)
+                t = _time(run, repeats)
+                tried[f"{key.split('_')[1]} {v}"] = t
+                if log:
+                    log(f"autotune: nodes [{entry['nodes_min']}, {entry['nodes_max']}] "
+                        f"{key} {v}: {t * 1e3:.1f} ms")
+                if t < best_t * (1.0 - margin):
+                    best_t, table, picked[key] = t, trial, v
+            apply(table)
+        report["buckets"].append(dict(entry, weight=bk["weight"],
+                                      region=picked.get("smoother_region", "built-in"),
+                                      chunks=picked.get("smoother_chunks", "built-in"),
+                                      tried_ms={k: v * 1e3 for k, v in tried.items()}))
     tried = {}
     for cc in check_cycles:
         trial = table.copy()

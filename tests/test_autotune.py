@@ -16,23 +16,33 @@ def restore():
 def test_table_round_trip_and_lookup():
     lib = _abi.load_library()
     base = lib.mmpbsa_tuning_smoother_zc(161, 161, 161, 48)
+    base81 = lib.mmpbsa_tuning_smoother_zc(81, 81, 81, 48)
     assert base > 0
+    # built-in regions: 44 x 44 on 161^3 (25 tiles of 512 threads per plane
+    # against 64 of 256), 32 x 32 on 81^3 (9 x 512 against 16 x 256)
+    assert lib.mmpbsa_tuning_smoother_region(161, 161, 161, 48) == 2
+    assert lib.mmpbsa_tuning_smoother_region(81, 81, 81, 48) == 1
     t = autotune.TuneTable([{"nodes_min": 1 << 21, "nodes_max": (1 << 22) - 1, "solves_min": 1,
-                             "solves_max": 64, "smoother_chunks": 3}], check_cycles=4, gb_chunks=8)
+                             "solves_max": 64, "smoother_chunks": 3, "smoother_region": 1}],
+                           check_cycles=4, gb_chunks=8)
     autotune.apply(t)
     got = autotune.current()
     assert got.entries == t.entries and got.check_cycles == 4 and got.gb_chunks == 8
-    # 161^3 = 4.17e6 nodes: 3 chunks over the 159 interior planes
+    # 161^3 = 4.17e6 nodes: 3 chunks over the 159 interior planes, small region
     assert lib.mmpbsa_tuning_smoother_zc(161, 161, 161, 48) == 53
+    assert lib.mmpbsa_tuning_smoother_region(161, 161, 161, 48) == 1
     # outside the entry's solve range, or another size: the built-in choice
     assert lib.mmpbsa_tuning_smoother_zc(161, 161, 161, 96) == base
-    assert lib.mmpbsa_tuning_smoother_zc(81, 81, 81, 48) == \
-        lib.mmpbsa_tuning_smoother_zc(81, 81, 81, 48)
+    assert lib.mmpbsa_tuning_smoother_region(161, 161, 161, 96) == 2
+    assert lib.mmpbsa_tuning_smoother_zc(81, 81, 81, 48) == base81
     autotune.apply(None)
     assert autotune.current().entries == [] and autotune.current().check_cycles == 8
     assert lib.mmpbsa_tuning_smoother_zc(161, 161, 161, 48) == base
     with pytest.raises(ValueError):
         autotune.apply(autotune.TuneTable(check_cycles=-1))
+    with pytest.raises(ValueError):
+        autotune.apply(autotune.TuneTable([{"nodes_min": 0, "nodes_max": 9, "solves_min": 1,
+                                            "solves_max": 9, "smoother_region": 3}]))
 
 
 def test_save_load(tmp_path):

@@ -1,5 +1,5 @@
-"""Tuned launch shapes on the GPU: smoother z-chunking and the host check
-interval change no result bit, a Born-radius chunk count changes only the
+"""Tuned launch shapes on the GPU: smoother regions, z-chunking and the host
+check interval change no result bit, a Born-radius chunk count changes only the
 rounding of its partial sums, and the search loop runs end to end on a small
 workload (results unchanged, the table left applied and persistable)."""
 import numpy as np
@@ -31,14 +31,16 @@ def test_tuned_chunks_and_checks_bitwise(cfg0):
     top, fr, rec, lig = _frames(cfg0)
     pb = rst.PbParams(**PB)
     ref = rst.compute_binding_energy_frames(top, fr, rec, lig, pb_params=pb, batched=True)
-    for chunks, checks in ((1, 2), (5, 16), (2, 8)):
+    # regions: 1 = 32 x 32 nodes per workgroup, 2 = 44 x 44 (0: built-in)
+    for chunks, checks, region in ((1, 2, 1), (5, 16, 2), (2, 8, 0), (3, 8, 1), (1, 4, 2)):
         autotune.apply(autotune.TuneTable([{"nodes_min": 0, "nodes_max": 1 << 40, "solves_min": 0,
-                                            "solves_max": 1 << 20, "smoother_chunks": chunks}],
+                                            "solves_max": 1 << 20, "smoother_chunks": chunks,
+                                            "smoother_region": region}],
                                           check_cycles=checks))
         got = rst.compute_binding_energy_frames(top, fr, rec, lig, pb_params=pb, batched=True)
         for a, b in zip(got.frames(), ref.frames()):
             for k in KEYS:
-                assert getattr(a, k) == getattr(b, k), (chunks, checks, k)
+                assert getattr(a, k) == getattr(b, k), (chunks, checks, region, k)
 
 
 def test_gb_chunks_change_rounding_only(cfg0):
