@@ -1210,15 +1210,25 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
         load_plane(t0 + 1);
         // the code table and the first two planes are read by every wave
         __syncthreads();
-        auto step = [&](int t, bool all_tag) {
-            store_plane(t + 1);
+        // steps ts .. te have all six half-sweeps in range: those run without
+        // the per-half-sweep range tests
+        const int ts = max(t0, max(K, z0 + K - 1)), te = min(t1, min(nz - 2, z1 + K - 2));
+        auto steady = [&](int t) { return t >= ts && t <= te; };
+        // the coefficient window of step t (planes t .. t - 5), its active
+        // columns, and its fractional faces numbered with their loads issued
+        auto advance = [&](int t) {
 #pragma unroll
             for (int i = K - 1; i > 0; --i) Q[i] = Q[i - 1];
             Q[0] = pack(nxt);
             step_cols(t);
-            // order of issue: fractional faces, write-back, prefetches, so
-            // that waiting for the faces waits for nothing younger
-            if (!UNI) specials_issue(t, all_tag);
+            if (!UNI) specials_issue(t, steady(t));
+        };
+        advance(t0);
+        auto step = [&](int t, bool all_tag) {
+            store_plane(t + 1);
+            // order of issue: write-back, prefetches; the step's fractional
+            // faces were issued before them, at the end of the previous step,
+            // so waiting for the faces waits for nothing younger
             write_back(t - K);
             if (t < t1) {
                 load_coef(t + 1, nxt, UNI);
@@ -1232,10 +1242,14 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
             // barrier follows the last half-sweep: the next step's in-plane
             // neighbours and write-back read its values, and the next plane
             // store refills the slot of plane t - 6, which this step's
-            // write-back and last half-sweep read.
+            // write-back and last half-sweep read.  Before it, the next
+            // step's window moves in and its fractional faces are requested,
+            // so their loads overlap the potential update, the barrier and the
+            // next step's stores and prefetches.
             if (UNI) {
 #pragma unroll
                 for (int k = 1; k <= K; ++k) stage(k, t, Fast{}, all_tag);
+                if (t < t1) advance(t + 1);
                 fold_finish();
                 __syncthreads();
                 return;
@@ -1254,13 +1268,11 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
 #pragma unroll
                 for (int k = 1; k <= K; ++k) stage(k, t, Fast{}, all_tag);
             }
+            if (t < t1) advance(t + 1);
             fold_finish();
             __syncthreads();
         };
-        // steps ts .. te have all six half-sweeps in range: those run without
-        // the per-half-sweep range tests
-        const int ts = max(t0, max(K, z0 + K - 1)), te = min(t1, min(nz - 2, z1 + K - 2));
-        for (int t = t0; t <= t1; ++t) step(t, t >= ts && t <= te);
+        for (int t = t0; t <= t1; ++t) step(t, steady(t));
         // the grid's last plane (never updated, still in the ring) follows
         // the last step
         if (wz1 > z1) {
