@@ -1000,6 +1000,21 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
         bool nospec = false;  // no code word of the wave's six planes has the fractional bit
         float sa[6];
         uint32_t sreq = 0xffffffffu;
+        // Rolling masks of the window's code bits, column 0 in bits 0-11 and
+        // column 1 in bits 16-27: bit 2 (k - 1) + m is bit 7 (fmask:
+        // fractional faces) or bit 6 (kmask: kappa) of the code byte of pair
+        // m's column at plane t - k + 1.  Each step shifts them by one plane
+        // and inserts the new plane's bits (roll), so the numbering reads them
+        // instead of reassembling twelve bits from the six code words.
+        uint32_t fmask = 0u, kmask = 0u;
+        auto roll = [&](uint32_t w) {
+            // code bytes of w: pair 0 column 0, pair 0 column 1, pair 1 column 0,
+            // pair 1 column 1
+            fmask = ((fmask << 2) & 0x0ffc0ffcu) | ((w >> 7) & 1u) | ((w >> 22) & 2u) |
+                    ((w << 1) & 0x10000u) | ((w >> 14) & 0x20000u);
+            kmask = ((kmask << 2) & 0x0ffc0ffcu) | ((w >> 6) & 1u) | ((w >> 21) & 2u) |
+                    ((w << 2) & 0x10000u) | ((w >> 13) & 0x20000u);
+        };
         // face entries of the step's updates (ENT): byte j & 3 of ent[j >> 2]
         // is the entry of update j = 2 (k - 1) + m, a table row or the wave's
         // slot TAB + wave * SLOTS + slot, valid outside the SLOW mode
@@ -1012,10 +1027,7 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     ent[i] = __builtin_amdgcn_perm(Q[2 * i + 1].w, Q[2 * i].w, sel) & 0x7f7f7f7fu;
-            uint32_t anyw = 0;
-#pragma unroll
-            for (int k = 0; k < K; ++k) anyw |= Q[k].w;
-            nospec = __ballot((anyw & 0x80808080u) != 0u) == 0;
+            nospec = __ballot(fmask != 0u) == 0;
             if (nospec) {
                 spm = 0;
                 spx = 0;
@@ -1029,15 +1041,9 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
 #pragma unroll
                 for (int k = 1; k <= K; ++k)
                     zmask |= zok(k, t - k + 1) ? 3u << (2 * (k - 1)) : 0u;
-            uint32_t hb = 0, kb = 0;  // bit j: code bit 7 (fractional) / bit 6 (kappa)
-#pragma unroll
-            for (int k = 1; k <= K; ++k) {
-                const uint32_t w = Q[k - 1].w >> (8 * c);
-                hb |= ((w >> 7) & 1u) << (2 * (k - 1));
-                hb |= ((w >> 23) & 1u) << (2 * (k - 1) + 1);
-                kb |= ((w >> 6) & 1u) << (2 * (k - 1));
-                kb |= ((w >> 22) & 1u) << (2 * (k - 1) + 1);
-            }
+            // bit j: code bit 7 (fractional) / bit 6 (kappa) of update j
+            const uint32_t hb = (c ? fmask >> 16 : fmask) & 0xfffu;
+            const uint32_t kb = (c ? kmask >> 16 : kmask) & 0xfffu;
             spm = hb & zmask & (c ? spread[1] : spread[0]);
             const int cnt = __popc(spm);
             int inc = cnt;
@@ -1221,7 +1227,10 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
             for (int i = K - 1; i > 0; --i) Q[i] = Q[i - 1];
             Q[0] = pack(nxt);
             step_cols(t);
-            if (!UNI) specials_issue(t, steady(t));
+            if (!UNI) {
+                roll(Q[0].w);
+                specials_issue(t, steady(t));
+            }
         };
         advance(t0);
         auto step = [&](int t, bool all_tag) {
