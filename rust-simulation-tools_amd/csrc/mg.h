@@ -110,6 +110,7 @@ void mg_smooth(const MgLevelDev *L, const MgLevelDev *C, int n_solves, int first
 // rg selects the (x, y) region of a workgroup: 0 = 32 x 32 nodes (a 20 x 20
 // core), 1 = 44 x 44 (32 x 32 core); every solve of a launch uses the same.
 int smooth3_region(int nx, int ny);  // the built-in choice for a level
+int smooth3_threads(int rg);         // workgroup size of a region
 int smooth3_tiles(int nx, int ny, int rg);
 int smooth3_blocks(int nx, int ny, int nz, int zc, int rg);
 int smooth3_zc(int nx, int ny, int nz, int n_solves, int rg);

@@ -672,6 +672,8 @@ int tiles(int nx, int ny)
 }
 }  // namespace s3
 
+int smooth3_threads(int rg) { return rg ? s3::Large::NT : s3::Small::NT; }
+
 int smooth3_tiles(int nx, int ny, int rg)
 {
     return rg ? s3::tiles<s3::Large>(nx, ny) : s3::tiles<s3::Small>(nx, ny);

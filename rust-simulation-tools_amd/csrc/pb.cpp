@@ -1089,7 +1089,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         prof.total_ms += ms_tot;
         prof.launches += (int64_t)evs.size();
         prof.alg_bytes += bytes;
-        prof.grid_threads = tot_of(0) * 256;
+        prof.grid_threads = tot_of(0) * (phase_pass ? smooth3_threads(level_rg[0]) : 256);
     }
     for (int s = 0; s < S; ++s) {
         TaskWork &tw = W[s];
