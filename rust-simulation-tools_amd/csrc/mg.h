@@ -48,6 +48,12 @@ struct MgLevelDev {
     float *res;        // residual scratch
     int zc;            // smoother: planes per workgroup (z-chunk length)
     int rg;            // smoother region (host bookkeeping): 0 32 x 32, 1 44 x 44
+    // Fractional-face entries of a level the phase smoother sweeps (non-
+    // uniform maps): at every interior node whose code byte has kCodeSpecial,
+    // ent[2 idx] = {a0, a1, a2, a3} and ent[2 idx + 1] = {a4, a5, 1/diag, 0}
+    // (faces x+, x-, y+, y-, z+, z-); other entries are never read.  Null
+    // elsewhere.
+    const float4 *ent;
     // finest level only (else null): the FP64 potential the phase smoother's
     // post-smoothing pass adds its final correction to, phi += e, in place of
     // storing e (the outer step then reads phi alone)
@@ -147,6 +153,9 @@ void mg_inject(const MgLevelDev &F, const MgLevelDev &C, hipStream_t s);  // hos
 // max_nodes the largest C level.
 void mg_inject_pack_batch(const MgLevelDev *F, const MgLevelDev *C, int n, int64_t max_nodes,
                           hipStream_t s);
+// fill L[k].ent (where set) from the level's FP32 faces, code bytes, ihx and
+// c_k2, for n descriptors at once (device array; max_nodes the largest level)
+void mg_fill_entries(const MgLevelDev *L, int n, int64_t max_nodes, hipStream_t s);
 
 // Code byte of a node: bits 0-5 = face x+, x-, y+, y-, z+, z- has the solute
 // dielectric (else the solvent one), bit 6 = kappa-bar^2 is c_k2 (else 0),

@@ -784,9 +784,10 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
     const auto dst = buf_rsrc((FIRST || dir) ? L.e2 : L.e, 4u * N);
     const auto frs = buf_rsrc(L.f, 4u * N);
     const auto crs = buf_rsrc(uni ? nullptr : L.cw, uni ? 0u : N);
-    const auto exr = buf_rsrc(uni ? nullptr : L.ex, uni ? 0u : 4u * N);
-    const auto eyr = buf_rsrc(uni ? nullptr : L.ey, uni ? 0u : 4u * N);
-    const auto ezr = buf_rsrc(uni ? nullptr : L.ez, uni ? 0u : 4u * N);
+    // fractional-face entries (device memory, 64-bit addressing; loaded as
+    // native 4-vectors)
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const auto etab = to_global<const f4>(reinterpret_cast<const f4 *>(L.ent));
     MgLevelDev C{};
     if (PROLONG) C = Cs[s];
     const auto ceg = to_global<const float>(PROLONG ? C.e : L.f);
@@ -1000,22 +1001,20 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
         uint32_t spm = 0;
         int spx = 0, total = 0;
         bool nospec = false;  // no code word of the wave's six planes has the fractional bit
-        float sa[6];
+        f4 sa, sb;  // the entry a lane loads for its slot
         uint32_t sreq = 0xffffffffu;
-        // Rolling masks of the window's code bits, column 0 in bits 0-11 and
-        // column 1 in bits 16-27: bit 2 (k - 1) + m is bit 7 (fmask:
-        // fractional faces) or bit 6 (kmask: kappa) of the code byte of pair
-        // m's column at plane t - k + 1.  Each step shifts them by one plane
-        // and inserts the new plane's bits (roll), so the numbering reads them
-        // instead of reassembling twelve bits from the six code words.
-        uint32_t fmask = 0u, kmask = 0u;
+        // Rolling mask of the window's fractional-face bits, column 0 in bits
+        // 0-11 and column 1 in bits 16-27: bit 2 (k - 1) + m is bit 7 of the
+        // code byte of pair m's column at plane t - k + 1.  Each step shifts it
+        // by one plane and inserts the new plane's bits (roll), so the
+        // numbering reads it instead of reassembling twelve bits from the six
+        // code words.
+        uint32_t fmask = 0u;
         auto roll = [&](uint32_t w) {
             // code bytes of w: pair 0 column 0, pair 0 column 1, pair 1 column 0,
             // pair 1 column 1
             fmask = ((fmask << 2) & 0x0ffc0ffcu) | ((w >> 7) & 1u) | ((w >> 22) & 2u) |
                     ((w << 1) & 0x10000u) | ((w >> 14) & 0x20000u);
-            kmask = ((kmask << 2) & 0x0ffc0ffcu) | ((w >> 6) & 1u) | ((w >> 21) & 2u) |
-                    ((w << 2) & 0x10000u) | ((w >> 13) & 0x20000u);
         };
         // face entries of the step's updates (ENT): byte j & 3 of ent[j >> 2]
         // is the entry of update j = 2 (k - 1) + m, a table row or the wave's
@@ -1043,9 +1042,8 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
 #pragma unroll
                 for (int k = 1; k <= K; ++k)
                     zmask |= zok(k, t - k + 1) ? 3u << (2 * (k - 1)) : 0u;
-            // bit j: code bit 7 (fractional) / bit 6 (kappa) of update j
+            // bit j: code bit 7 (fractional faces) of update j
             const uint32_t hb = (c ? fmask >> 16 : fmask) & 0xfffu;
-            const uint32_t kb = (c ? kmask >> 16 : kmask) & 0xfffu;
             spm = hb & zmask & (c ? spread[1] : spread[0]);
             const int cnt = __popc(spm);
             int inc = cnt;
@@ -1057,7 +1055,7 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
             spx = inc - cnt;
             total = __builtin_amdgcn_readfirstlane(__shfl(inc, 63, 64));
             if (total == 0) return;
-            // requests: node index and kappa bit of each numbered update
+            // requests: node index of each numbered update
             const uint32_t a0 = cvo[0] + (uint32_t)c + sxy * (uint32_t)(t + 1);
             const uint32_t a1 = cvo[1] + (uint32_t)c + sxy * (uint32_t)(t + 1);
             uint32_t rem = spm;
@@ -1068,7 +1066,7 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
                     rem &= rem - 1u;
                     const uint32_t g = ((j & 1) ? a1 : a0) - sxy * (uint32_t)((j >> 1) + 1);
                     if (sl < SLOTS) {
-                        req[sl] = (g << 1) | ((kb >> j) & 1u);
+                        req[sl] = g;
                         if (ENT) {
                             const uint32_t sh = 8u * (uint32_t)(j & 3);
                             const uint32_t v = (uint32_t)(TAB + wv * SLOTS + sl) << sh;
@@ -1082,25 +1080,17 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
                     ++sl;
                 }
             }
+            // one lane per numbered update loads its precomputed entry
             sreq = lane < min(total, SLOTS) ? req[lane] : 0xffffffffu;
-            const uint32_t g = sreq >> 1;
-            const bool v = sreq != 0xffffffffu;
-            sa[0] = bload(exr, v ? 4u * g : kOob, 0u);
-            sa[1] = bload(exr, v ? 4u * (g - 1u) : kOob, 0u);
-            sa[2] = bload(eyr, v ? 4u * g : kOob, 0u);
-            sa[3] = bload(eyr, v ? 4u * (g - (uint32_t)nx) : kOob, 0u);
-            sa[4] = bload(ezr, v ? 4u * g : kOob, 0u);
-            sa[5] = bload(ezr, v ? 4u * (g - sxy) : kOob, 0u);
+            if (sreq != 0xffffffffu) {
+                sa = etab[2 * (size_t)sreq];
+                sb = etab[2 * (size_t)sreq + 1];
+            }
         };
         auto specials_store = [&]() {
             if (total == 0 || sreq == 0xffffffffu) return;
-            // face order of the update: x+, x-, y+, y-, z+, z-
-            const float a0 = sa[0], a1 = sa[1], a2 = sa[2], a3 = sa[3], a4 = sa[4], a5 = sa[5];
-            const float k2 = (sreq & 1u) ? ck2 : 0.0f;
-            const float dg = fmaf(((a0 + a1) + (a2 + a3)) + (a4 + a5), ih, k2);
-            spc[2 * (wv * SLOTS + lane)] = make_float4(a0, a1, a2, a3);
-            spc[2 * (wv * SLOTS + lane) + 1] =
-                make_float4(a4, a5, __builtin_amdgcn_rcpf(dg), 0.0f);
+            spc[2 * (wv * SLOTS + lane)] = make_float4(sa.x, sa.y, sa.z, sa.w);
+            spc[2 * (wv * SLOTS + lane) + 1] = make_float4(sb.x, sb.y, sb.z, sb.w);
         };
 
         // per step: the active column's ring distance and the byte offset,
@@ -1184,17 +1174,10 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
                     const float4 *e = fac + 2u * ei;
                     float4 A = e[0], B = e[1];
                     if (SLOW && sp && sl >= SLOTS) {
-                        const uint32_t g = cvo[m] + (uint32_t)c + sxy * (uint32_t)z;
-                        A.x = bload(exr, 4u * g, 0u);
-                        A.y = bload(exr, 4u * (g - 1u), 0u);
-                        A.z = bload(eyr, 4u * g, 0u);
-                        A.w = bload(eyr, 4u * (g - (uint32_t)nx), 0u);
-                        B.x = bload(ezr, 4u * g, 0u);
-                        B.y = bload(ezr, 4u * (g - sxy), 0u);
-                        const float k2 = (w & kCodeKappa) ? ck2 : 0.0f;
-                        const float dg =
-                            fmaf(((A.x + A.y) + (A.z + A.w)) + (B.x + B.y), ih, k2);
-                        B.z = __builtin_amdgcn_rcpf(dg);
+                        const size_t g = cvo[m] + (uint32_t)c + sxy * (uint32_t)z;
+                        const f4 a = etab[2 * g], b = etab[2 * g + 1];
+                        A = make_float4(a.x, a.y, a.z, a.w);
+                        B = make_float4(b.x, b.y, b.z, b.w);
                     }
                     const float sum = zs ? 0.0f : fmaf(A.x, exx, fmaf(A.y, emx, fmaf(A.z, eyy,
                                       fmaf(A.w, emy, fmaf(B.x, ezz, B.y * emz)))));
@@ -2172,6 +2155,39 @@ __global__ void pack_codes_batch_kernel(const MgLevelDev *__restrict__ Ls)
     face(L.ez, idx - sz, 5);
     if (L.k2[idx] != 0.0f) w |= kCodeKappa;
     const_cast<uint8_t *>(L.cw)[idx] = (uint8_t)w;
+}
+
+// The entry of a fractional-face node with the phase smoother's own
+// expressions (a sum of the six faces times ih plus kappa-bar^2, its
+// reciprocal), so that reading it is bitwise the same as computing it there.
+__global__ void fill_entries_kernel(const MgLevelDev *__restrict__ Ls)
+{
+    const MgLevelDev L = Ls[blockIdx.y];
+    if (L.nx == 0 || !L.ent) return;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nodes(L)) return;
+    const uint32_t w = L.cw[idx];
+    if (!(w & kCodeSpecial)) return;
+    int ix, iy, iz;
+    split3(L, idx, ix, iy, iz);
+    if (ix == 0 || iy == 0 || iz == 0 || ix == L.nx - 1 || iy == L.ny - 1 || iz == L.nz - 1)
+        return;
+    const int64_t sy = L.nx, sz = (int64_t)L.nx * L.ny;
+    const float a0 = L.ex[idx], a1 = L.ex[idx - 1], a2 = L.ey[idx], a3 = L.ey[idx - sy];
+    const float a4 = L.ez[idx], a5 = L.ez[idx - sz];
+    const float k2 = (w & kCodeKappa) ? L.c_k2 : 0.0f;
+    const float dg = fmaf(((a0 + a1) + (a2 + a3)) + (a4 + a5), L.ihx, k2);
+    float4 *e = const_cast<float4 *>(L.ent);
+    e[2 * idx] = make_float4(a0, a1, a2, a3);
+    e[2 * idx + 1] = make_float4(a4, a5, __builtin_amdgcn_rcpf(dg), 0.0f);
+}
+
+void mg_fill_entries(const MgLevelDev *L, int n, int64_t max_nodes, hipStream_t s)
+{
+    if (n <= 0 || max_nodes <= 0) return;
+    const dim3 grid((unsigned)std::max<int64_t>(1, ceil_div(max_nodes, 256)), (unsigned)n);
+    hipLaunchKernelGGL(fill_entries_kernel, grid, dim3(256), 0, s, L);
+    LAUNCH_OK();
 }
 
 void mg_inject_pack_batch(const MgLevelDev *F, const MgLevelDev *C, int n, int64_t max_nodes,

@@ -493,6 +493,7 @@ struct TaskLevel {
     GridDesc g;
     DBuf<float> ex, ey, ez, k2, e, e2, f, res;
     DBuf<uint8_t> cw;
+    DBuf<float4> ent;  // fractional-face entries (levels the phase pass sweeps)
     MgLevelDev d;
 };
 
@@ -682,6 +683,12 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
                     // injected faces and code bytes: one launch per level for
                     // every solve of the group, after this loop
                 }
+                // fractional-face entries of the levels the phase smoother
+                // sweeps, filled once the faces and code bytes are there
+                if (phase_pass && l < tw.tail_start) {
+                    lv.ent = DBuf<float4>(w.pool, 2 * (size_t)N);
+                    dv.ent = lv.ent.get();
+                }
             }
         }
         const int64_t N0 = lv_nodes(t->g);
@@ -739,7 +746,23 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             HIP_OK(hipMemcpyAsync(inj_dev.back().get(), h.data(), sizeof(MgLevelDev) * h.size(),
                                   hipMemcpyHostToDevice, w.stream));
             mg_inject_pack_batch(inj_dev.back().get(), inj_dev.back().get() + S, S, mx, w.stream);
+            mg_fill_entries(inj_dev.back().get() + S, S, mx, w.stream);
             inj_host.push_back(std::move(h));
+        }
+        // level 0: the maps' own FP32 faces and code bytes
+        std::vector<MgLevelDev> h0(S, null_level());
+        int64_t mx0 = 0;
+        for (int s = 0; s < S; ++s)
+            if (W[s].L[0].d.ent) {
+                h0[s] = W[s].L[0].d;
+                mx0 = std::max(mx0, lv_nodes(W[s].L[0].g));
+            }
+        if (mx0 > 0) {
+            inj_dev.emplace_back(w.pool, (size_t)S);
+            HIP_OK(hipMemcpyAsync(inj_dev.back().get(), h0.data(), sizeof(MgLevelDev) * h0.size(),
+                                  hipMemcpyHostToDevice, w.stream));
+            mg_fill_entries(inj_dev.back().get(), S, mx0, w.stream);
+            inj_host.push_back(std::move(h0));
         }
     }
 
