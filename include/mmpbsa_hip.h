@@ -446,6 +446,24 @@ int mmpbsa_server_poll(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *
 int mmpbsa_server_wait(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
                        mmpbsa_request_stats *stats);
 
+/* Statistics of a served request, computed on the device by the server thread
+ * when the request's last frame finishes (before its waiters wake): the
+ * BindingResult statistics (binding.rs:704-743) and interaction_entropy
+ * (entropy.rs:53-78) of its frames at the request's temperature (the PB
+ * parameters' for MMPBSA_METHOD_PB, else the GB parameters').  Fixed-order
+ * FP64 tree reductions: equal to mmpbsa_binding_stats_compute and
+ * mmpbsa_interaction_entropy over the returned frames to rounding. */
+typedef struct mmpbsa_request_summary {
+    mmpbsa_binding_stats stats;
+    double minus_tds;     /* NaN for a single frame */
+    double temperature;   /* K */
+} mmpbsa_request_summary;
+/* poll / wait that also fill summary (may be null). */
+int mmpbsa_server_poll_summary(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
+                               mmpbsa_request_stats *stats, mmpbsa_request_summary *summary);
+int mmpbsa_server_wait_summary(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
+                               mmpbsa_request_stats *stats, mmpbsa_request_summary *summary);
+
 /* ---- device memory --------------------------------------------------------
  * For callers that stage inputs in HBM without another GPU runtime (no
  * counterpart in the reference).  Allocation is on `device`; copies are

@@ -305,6 +305,7 @@ struct ServerRequest {
     bool started = false;
     int status = MMPBSA_OK;
     std::string err;
+    mmpbsa_request_summary summary{};  // device epilogue, set before the last frame counts
 };
 
 // Device bytes one frame holds while its PB solves run: per grid node of each
@@ -356,6 +357,39 @@ struct mmpbsa_server {
     void loop();
 };
 
+// The device epilogue of a finished request (frame_stats_kernel).
+void summarize(Worker &w, ServerRequest &r)
+{
+    const int n = r.n_frames;
+    std::vector<double> h(4 * (size_t)n);
+    for (int f = 0; f < n; ++f) {
+        h[4 * f] = r.out[f].delta_mm;
+        h[4 * f + 1] = r.out[f].delta_polar;
+        h[4 * f + 2] = r.out[f].delta_sa;
+        h[4 * f + 3] = r.out[f].delta_total;
+    }
+    const double temp = r.method == MMPBSA_METHOD_PB ? r.pb.temperature : r.gb.temperature;
+    DBuf<double> d(w.pool, h.size() + 10);
+    HIP_OK(hipMemcpyAsync(d.get(), h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                          w.stream));
+    launch_frame_stats(d.get(), n, 0.00198688 * temp, d.get() + h.size(), w.stream);
+    double o[10];
+    HIP_OK(hipMemcpyAsync(o, d.get() + h.size(), sizeof(o), hipMemcpyDeviceToHost, w.stream));
+    HIP_OK(hipStreamSynchronize(w.stream));
+    mmpbsa_binding_stats &s = r.summary.stats;
+    s.mean_delta_mm = o[0];
+    s.mean_delta_polar = o[1];
+    s.mean_delta_sa = o[2];
+    s.mean_delta_total = o[3];
+    s.std_delta_total = o[4];
+    s.std_delta_mm = o[5];
+    s.std_delta_polar = o[6];
+    s.std_delta_sa = o[7];
+    s.sem_delta_total = o[8];
+    r.summary.minus_tds = o[9];
+    r.summary.temperature = temp;
+}
+
 // Continuous batching: each round takes frames from the waiting requests,
 // oldest first (a request's remaining frames, then the next request's), while
 // the round holds at most max_round frames and their admission estimate fits
@@ -400,8 +434,24 @@ void mmpbsa_server::loop()
             it.out = &r->out[f];
         }
         std::string err;
-        const int st = guarded(nullptr, [&] { run_items(*w, items, true, false); });
+        int st = guarded(nullptr, [&] { run_items(*w, items, true, false); });
         if (st != MMPBSA_OK) err = g_err;
+        // requests whose last frames ran in this round: their statistics on
+        // the device before the waiters wake (done and status change only on
+        // this thread, so they are read here without the lock)
+        if (st == MMPBSA_OK) {
+            std::map<ServerRequest *, int> ran;
+            for (auto &pk : picks) ++ran[pk.first];
+            for (auto &kv : ran) {
+                ServerRequest *r = kv.first;
+                if (r->status != MMPBSA_OK || r->done + kv.second != r->n_frames) continue;
+                st = guarded(nullptr, [&] { summarize(*w, *r); });
+                if (st != MMPBSA_OK) {
+                    err = g_err;
+                    break;
+                }
+            }
+        }
         {
             std::lock_guard<std::mutex> lk(mu);
             in_flight = 0.0;
@@ -1262,9 +1312,10 @@ namespace {
 // Copy out a finished request and release its ticket (caller holds srv->mu).
 int finish_request(mmpbsa_server *srv, std::unordered_map<uint64_t,
                    std::unique_ptr<ServerRequest>>::iterator it, mmpbsa_frame_terms *out,
-                   mmpbsa_request_stats *stats)
+                   mmpbsa_request_stats *stats, mmpbsa_request_summary *summary = nullptr)
 {
     ServerRequest &r = *it->second;
+    if (summary && r.status == MMPBSA_OK) *summary = r.summary;
     auto ms = [](Clock::time_point a, Clock::time_point b) {
         return std::chrono::duration<double, std::milli>(b - a).count();
     };
@@ -1299,13 +1350,30 @@ int mmpbsa_server_poll(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *
 int mmpbsa_server_wait(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
                        mmpbsa_request_stats *stats)
 {
+    return mmpbsa_server_wait_summary(srv, ticket, out, stats, nullptr);
+}
+
+int mmpbsa_server_poll_summary(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
+                               mmpbsa_request_stats *stats, mmpbsa_request_summary *summary)
+{
+    if (!srv) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null server");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(ticket);
+    if (it == srv->reqs.end()) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "unknown ticket");
+    if (it->second->done < it->second->n_frames) return MMPBSA_ERR_NOT_READY;
+    return finish_request(srv, it, out, stats, summary);
+}
+
+int mmpbsa_server_wait_summary(mmpbsa_server *srv, uint64_t ticket, mmpbsa_frame_terms *out,
+                               mmpbsa_request_stats *stats, mmpbsa_request_summary *summary)
+{
     if (!srv) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "null server");
     std::unique_lock<std::mutex> lk(srv->mu);
     auto it = srv->reqs.find(ticket);
     if (it == srv->reqs.end()) return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "unknown ticket");
     ServerRequest *r = it->second.get();
     srv->cv_done.wait(lk, [&] { return r->done >= r->n_frames; });
-    return finish_request(srv, srv->reqs.find(ticket), out, stats);
+    return finish_request(srv, srv->reqs.find(ticket), out, stats, summary);
 }
 
 int mmpbsa_device_alloc(int32_t device, int64_t bytes, void **out)

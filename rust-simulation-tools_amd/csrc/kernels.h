@@ -86,6 +86,9 @@ struct CellList {
     const int *start;   // [ncell + 1]
     const int *atoms;   // [n]
 };
+// statistics of n frames' (dMM, dPolar, dSA, dTotal) at kT (10 doubles out:
+// four means, std of total, mm, polar, sa, SEM of total, -T dS)
+void launch_frame_stats(const double *terms, int n, double kt, double *out10, hipStream_t s);
 void launch_sasa(const double *xyz, const double *radii, int n, double probe,
                  const double *pts, int n_pts, const CellList &cl, double max_ext,
                  double *per_atom, int *overflow, hipStream_t s);

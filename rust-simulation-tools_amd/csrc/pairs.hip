@@ -855,4 +855,81 @@ void launch_sasa(const double *xyz, const double *radii, int n, double probe, co
     LAUNCH_OK();
 }
 
+// ---- statistics of a served request (binding.rs:704-743, entropy.rs:53-78) ----
+// One workgroup over n frames' (dMM, dPolar, dSA, dTotal): means, sample
+// standard deviations about them (two passes, as the reference), the SEM of
+// dTotal and the interaction entropy -T dS = kT (m + ln(<exp(x - m)>)) of x =
+// (dMM - <dMM>) / kT, m = max x (log-sum-exp).  Fixed-order tree reductions, so
+// the result is deterministic; it equals the sequential host sums to rounding.
+// out: mean mm, polar, sa, total; std total, mm, polar, sa; sem total;
+// -T dS (NaN below two frames).
+__device__ void block_sum4(double (&v)[4], double (*red)[4])
+{
+    const int t = threadIdx.x;
+    for (int k = 0; k < 4; ++k) red[t][k] = v[k];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (t < off)
+            for (int k = 0; k < 4; ++k) red[t][k] += red[t + off][k];
+        __syncthreads();
+    }
+    for (int k = 0; k < 4; ++k) v[k] = red[0][k];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void frame_stats_kernel(const double *__restrict__ d, int n,
+                                                          double kt, double *__restrict__ out)
+{
+    __shared__ double red[256][4];
+    const int t = threadIdx.x;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int f = t; f < n; f += blockDim.x)
+        for (int k = 0; k < 4; ++k) s[k] += d[4 * f + k];
+    block_sum4(s, red);
+    double mu[4];
+    for (int k = 0; k < 4; ++k) mu[k] = s[k] / (double)n;
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int f = t; f < n; f += blockDim.x)
+        for (int k = 0; k < 4; ++k) {
+            const double e = d[4 * f + k] - mu[k];
+            q[k] += e * e;
+        }
+    block_sum4(q, red);
+    // entropy: the largest x, then the sum of exp(x - m)
+    double m[4] = {-INFINITY, 0.0, 0.0, 0.0};
+    for (int f = t; f < n; f += blockDim.x) m[0] = fmax(m[0], (d[4 * f] - mu[0]) / kt);
+    red[t][0] = m[0];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (t < off) red[t][0] = fmax(red[t][0], red[t + off][0]);
+        __syncthreads();
+    }
+    const double mx = red[0][0];
+    __syncthreads();
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int f = t; f < n; f += blockDim.x) a[0] += exp((d[4 * f] - mu[0]) / kt - mx);
+    block_sum4(a, red);
+    if (t != 0) return;
+    const double sd[4] = {n > 1 ? sqrt(q[0] / (double)(n - 1)) : 0.0,
+                          n > 1 ? sqrt(q[1] / (double)(n - 1)) : 0.0,
+                          n > 1 ? sqrt(q[2] / (double)(n - 1)) : 0.0,
+                          n > 1 ? sqrt(q[3] / (double)(n - 1)) : 0.0};
+    out[0] = mu[0];
+    out[1] = mu[1];
+    out[2] = mu[2];
+    out[3] = mu[3];
+    out[4] = sd[3];
+    out[5] = sd[0];
+    out[6] = sd[1];
+    out[7] = sd[2];
+    out[8] = sd[3] / sqrt((double)n);
+    out[9] = n >= 2 ? kt * (mx + log(a[0] / (double)n)) : NAN;
+}
+
+void launch_frame_stats(const double *terms, int n, double kt, double *out10, hipStream_t s)
+{
+    hipLaunchKernelGGL(frame_stats_kernel, dim3(1), dim3(256), 0, s, terms, n, kt, out10);
+    LAUNCH_OK();
+}
+
 }  // namespace mmpbsa

@@ -583,6 +583,20 @@ class RequestStats:
         self.n_frames = st.n_frames
 
 
+class RequestSummary:
+    """Statistics of a served request computed on the device when its last
+    frame finishes: the BindingResult statistics (binding.rs:704-743) as
+    attributes of the same names, and interaction_entropy (entropy.rs:53-78)
+    at the request's temperature (None for a single frame)."""
+
+    def __init__(self, s: "_abi.RequestSummaryC"):
+        for k, _ in _abi.BindingStatsC._fields_:
+            setattr(self, k, getattr(s.stats, k))
+        self.temperature = s.temperature
+        self.entropy = (None if s.minus_tds != s.minus_tds
+                        else EntropyEstimate(s.minus_tds, "interaction_entropy"))
+
+
 class Server:
     """Continuous-batching MM-PBSA server (extension: the reference has no
     served path).  submit() copies the frames ([F, N, 3] nm) and returns a
@@ -619,16 +633,19 @@ class Server:
         self._keep[t.value] = (c, f.shape[0])
         return t.value
 
-    def _finish(self, fn, ticket):
+    def _finish(self, fn, ticket, summary=False):
         c, n = self._keep[ticket]
         out = (FrameTermsC * n)()
         st = RequestStatsC()
-        rc = fn(self.h, C.c_uint64(ticket), out, C.byref(st))
+        sm = _abi.RequestSummaryC()
+        args = (self.h, C.c_uint64(ticket), out, C.byref(st)) + ((C.byref(sm),) if summary else ())
+        rc = fn(*args)
         if rc == _abi.MMPBSA_ERR_NOT_READY:
             return None
         del self._keep[ticket]
         _check(rc, self.lib)
-        return BindingResult([FrameEnergy(t) for t in out]), RequestStats(st)
+        res = (BindingResult([FrameEnergy(t) for t in out]), RequestStats(st))
+        return res + (RequestSummary(sm),) if summary else res
 
     def poll(self, ticket: int):
         """(BindingResult, RequestStats), or None while frames are pending."""
@@ -636,6 +653,14 @@ class Server:
 
     def wait(self, ticket: int):
         return self._finish(self.lib.mmpbsa_server_wait, ticket)
+
+    def poll_summary(self, ticket: int):
+        """(BindingResult, RequestStats, RequestSummary): the request's statistics
+        and interaction entropy computed on the device; None while pending."""
+        return self._finish(self.lib.mmpbsa_server_poll_summary, ticket, True)
+
+    def wait_summary(self, ticket: int):
+        return self._finish(self.lib.mmpbsa_server_wait_summary, ticket, True)
 
     def close(self):
         if self.h:

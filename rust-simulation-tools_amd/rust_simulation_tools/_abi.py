@@ -198,6 +198,10 @@ class RequestStatsC(C.Structure):
                 ("status", C.c_int32)]
 
 
+class RequestSummaryC(C.Structure):
+    _fields_ = [("stats", BindingStatsC), ("minus_tds", C.c_double), ("temperature", C.c_double)]
+
+
 class TuneEntryC(C.Structure):
     _fields_ = [("nodes_min", C.c_int64), ("nodes_max", C.c_int64), ("solves_min", C.c_int32),
                 ("solves_max", C.c_int32), ("smoother_chunks", C.c_int32),
@@ -291,6 +295,12 @@ SIGNATURES = {
                                      C.POINTER(RequestStatsC)]),
     "mmpbsa_server_wait": (C.c_int, [P, C.c_uint64, C.POINTER(FrameTermsC),
                                      C.POINTER(RequestStatsC)]),
+    "mmpbsa_server_poll_summary": (C.c_int, [P, C.c_uint64, C.POINTER(FrameTermsC),
+                                             C.POINTER(RequestStatsC),
+                                             C.POINTER(RequestSummaryC)]),
+    "mmpbsa_server_wait_summary": (C.c_int, [P, C.c_uint64, C.POINTER(FrameTermsC),
+                                             C.POINTER(RequestStatsC),
+                                             C.POINTER(RequestSummaryC)]),
     "mmpbsa_device_alloc": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
     "mmpbsa_device_free": (C.c_int, [C.c_void_p]),
     "mmpbsa_device_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),

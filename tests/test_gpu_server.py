@@ -117,3 +117,46 @@ def test_budget_admission(systems):
             _same(res[0], ref)
     finally:
         one.close()
+
+
+def _close(a, b, rel=1e-12):
+    assert abs(a - b) <= rel * max(1.0, abs(a), abs(b)), (a, b)
+
+
+def test_request_summary_on_device(systems):
+    """Each finished request's statistics (binding.rs:704-743) and interaction
+    entropy (entropy.rs:53-78) come from the device epilogue: equal, to
+    rounding, to the host functions over the frames the ticket returns; a
+    single frame has std 0 and no entropy.  The temperature is the request's
+    own (PB parameters for PB, GB parameters otherwise)."""
+    (ta, fa, ra, la), (tb, fb, rb, lb), (tc, fc, rc, lc) = systems
+    srv = rst.Server(max_round_frames=4)
+    try:
+        gb = rst.GbParams(model=2, temperature=310.0)
+        many = np.concatenate([fc] * 4)  # 8 frames over two rounds
+        t1 = srv.submit(tc, many, rc, lc, gb_params=gb)
+        t2 = srv.submit(ta, fa[:1], ra, la)
+        t3 = srv.submit(tb, fb, rb, lb, pb_params=rst.PbParams(**PB_SMALL, temperature=290.0))
+        for t, temp, n in ((t1, 310.0, 8), (t2, 300.0, 1), (t3, 290.0, 2)):
+            res, st, sm = srv.wait_summary(t)
+            assert st.n_frames == n and sm.temperature == temp
+            for k in ("mean_delta_mm", "mean_delta_polar", "mean_delta_sa", "mean_delta_total",
+                      "std_delta_total", "std_delta_mm", "std_delta_polar", "std_delta_sa",
+                      "sem_delta_total"):
+                _close(getattr(sm, k), getattr(res, k))
+            if n == 1:
+                assert sm.std_delta_total == 0.0 and sm.entropy is None
+            else:
+                ie = rst.interaction_entropy(res.frames(), temperature=temp)
+                _close(sm.entropy.minus_tds, ie.minus_tds)
+        # poll_summary on a finished request, and NOT_READY handling
+        t4 = srv.submit(tc, fc, rc, lc)
+        got = None
+        for _ in range(2000):
+            got = srv.poll_summary(t4)
+            if got is not None:
+                break
+            time.sleep(0.01)
+        assert got is not None and got[2].temperature == 300.0
+    finally:
+        srv.close()
