@@ -12,6 +12,7 @@ from rust_simulation_tools import synthetic
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 from test_trajectory import _write_dcd, _write_mdcrd  # noqa: E402
+from test_prmtop import _write as _write_prmtop  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -100,3 +101,33 @@ def test_dcd_reimaging_in_a_real_box(system, oracle, tmp_path):
     last = np.zeros((top.n_atoms, 3))
     assert lib.mmpbsa_trajectory_frames(s, out, 3, last.ctypes.data_as(rst._abi.c_double_p)) == 0
     assert np.array_equal(last, fixed[-1])
+
+
+def test_prmtop_and_dcd_against_oracle(system, oracle, tmp_path):
+    """Serving from files end to end: the complex written as a prmtop and read
+    back by read_prmtop (the library's parser), its frames as a DCD, through
+    compute_binding_energy with GB and with PB; every frame term is within
+    0.01 kcal/mol of the FP64 oracle run on the read topology and the frames
+    the DCD reader returns."""
+    top, frames, rec, lig = system
+    pt = tmp_path / "complex.prmtop"
+    _write_prmtop(pt, top)
+    t = rst.read_prmtop(pt)
+    assert t.n_atoms == top.n_atoms and t.n_residues == top.n_residues
+    d = tmp_path / "t.dcd"
+    _write_dcd(d, frames[:2])
+    traj, _ = rst.DcdReader(d).read_all()  # nm
+    for method in ("gb", "pb"):
+        kw = dict(pb_params=rst.PbParams(**PB)) if method == "pb" else \
+            dict(gb_params=rst.GbParams(model=2))
+        res = rst.compute_binding_energy(t, d, rec, lig, trajectory_format="dcd", **kw)
+        assert len(res.frames()) == 2
+        for k, f in enumerate(res.frames()):
+            o = oracle.frame_energy(t, traj[k] * 10.0, rec, lig,
+                                    method=1 if method == "pb" else 0,
+                                    gb=oracle.gb_params(model=2), pb=oracle.pb_params(**PB))
+            for part in ("complex", "receptor", "ligand"):
+                for term in ("mm", "polar", "sa"):
+                    a, b = getattr(f, f"{part}_{term}"), o[part][term]
+                    assert abs(a - b) < 0.01, (method, k, part, term, a, b)
+            assert abs(f.delta_total - o["delta_total"]) < 0.01, (method, k)
