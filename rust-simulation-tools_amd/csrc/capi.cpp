@@ -313,6 +313,10 @@ struct ServerRequest {
 // potentials per medium, the FP32 multigrid arrays of both solves and the
 // geometry scratch (an upper estimate used for admission only).
 constexpr double kPbBytesPerNode = 320.0;
+// Fractional-face entry tables of the solvated solve (the reference solve's
+// map is uniform and has none): 32 bytes per node of every level the phase
+// smoother sweeps, at most 8/7 of the finest level's nodes.
+constexpr double kEntryBytesPerNode = 32.0 * 8.0 / 7.0;
 
 double frame_bytes_estimate(const mmpbsa_split *s, const double *xyz, int method,
                             const mmpbsa_pb_params &p)
@@ -336,7 +340,7 @@ double frame_bytes_estimate(const mmpbsa_split *s, const double *xyz, int method
     else if (p.fillratio > 0.0 && p.fscale > 1)
         nodes += (double)coarse_grid(mn, mx, p.grid_spacing, p.fillratio, p.fscale).size();
     // the receptor and ligand grids are no larger than the complex's
-    return 3.0 * kPbBytesPerNode * nodes;
+    return 3.0 * (kPbBytesPerNode + kEntryBytesPerNode) * nodes;
 }
 
 }  // namespace
