@@ -157,18 +157,21 @@ def uneven_leg(rst, synthetic, workers):
             "mixed_over_one_at_a_time": alone / mixed}
 
 
-def _serve_all(rst, cx, pb, round_frames, frames=None):
+def _serve_all(rst, cx, pb, round_frames, frames=None, progress=None):
     """Submit every complex's frames at once to one server; (wall s, request
-    latencies ms)."""
+    latencies ms).  With a progress label, a line per finished request."""
     srv = rst.Server(hbm_budget_bytes=200 << 30, max_round_frames=round_frames)
     try:
         t0 = time.perf_counter()
         tickets = [srv.submit(top, f if frames is None else f[:frames], rec, lig, pb_params=pb)
                    for top, f, rec, lig in cx]
         lat = []
-        for t in tickets:
+        for k, t in enumerate(tickets):
             _, st = srv.wait(t)
             lat.append(st.total_ms)
+            if progress:
+                _note(f"{progress}: request {k + 1}/{len(tickets)} done at "
+                      f"{time.perf_counter() - t0:.1f} s")
         return time.perf_counter() - t0, lat
     finally:
         srv.close()
@@ -210,7 +213,7 @@ def many_complex_leg(rst, synthetic, n_cx, n_frames, workers, tune):
     nfr = n_cx * n_frames
     autotune.apply(None)
     default_round = 16
-    span0, lat0 = _serve_all(rst, cx, pb, default_round)
+    span0, lat0 = _serve_all(rst, cx, pb, default_round, progress="many built-in")
     _note(f"many: built-in {nfr / span0:.2f} frames/s")
     out = {"workload": f"{n_cx} complexes, N log-uniform [3k, 60k] (min {int(sizes.min())}, "
                        f"median {int(np.median(sizes))}, max {int(sizes.max())}), ligand "
@@ -243,7 +246,7 @@ def many_complex_leg(rst, synthetic, n_cx, n_frames, workers, tune):
             path = autotune.save(table, workload="many-complex")
         except OSError:
             path = None
-        span1, lat1 = _serve_all(rst, cx, pb, best_round)
+        span1, lat1 = _serve_all(rst, cx, pb, best_round, progress="many tuned")
         _note(f"many: tuned {nfr / span1:.2f} frames/s")
         autotune.apply(None)
         out["tuned"] = {"frames_per_s": nfr / span1, "wall_s": span1, "round_frames": best_round,
@@ -291,10 +294,11 @@ def serving_leg(rst, synthetic, n_req, workers, oracle_max_atoms, check_oracle):
             if dt > 0:
                 time.sleep(dt)
             tickets.append(srv.submit(top, f, rec, lig, pb_params=pb))
-        for t in tickets:
+        for k, t in enumerate(tickets):
             r, st = srv.wait(t)
             lat.append(st.total_ms)
             results.append(r)
+            _note(f"serving: request {k + 1}/{n_req} done at {time.perf_counter() - t0:.1f} s")
         span = time.perf_counter() - t0
     finally:
         srv.close()
@@ -321,6 +325,8 @@ def serving_leg(rst, synthetic, n_req, workers, oracle_max_atoms, check_oracle):
             top, fr, rec, lig = cx[k]
             o = O.frame_energy(top, fr[f] * 10.0, rec, lig, method=1, pb=O.pb_params(**AUTO_PB))
             errs.append(max(_frame_errors(results[k].frames()[f], o).values()))
+            _note(f"serving: oracle check {len(errs)}/{len(checked)} ({int(sizes[k])} atoms, "
+                  f"frame {f}): {errs[-1]:.2e} kcal/mol")
         out["oracle_check"] = {"sample_frames": len(sample), "checked_frames": len(checked),
                                "checked": [[int(sizes[k]), int(f)] for k, f in checked],
                                "rule": f"seeded 5% sample of frames, oracle run on complexes of "
