@@ -88,6 +88,9 @@ struct MgOuterDev {
     double *partials;                   // per-block sums of r^2
     int nbk;                            // number of partials (blocks per solve)
     FastDiv fd_nx, fd_nxy;
+    // outer64: index of a solve with the same grid and charge density whose
+    // outer step runs in the same threads (one read of rho for both), or -1
+    int partner;
 };
 
 // Batched launchers.  L / C / F / O / T are device arrays with one entry per

@@ -1907,98 +1907,123 @@ __global__ __launch_bounds__(1024) void tail_kernel(const MgTailDev *__restrict_
 }
 
 // ---- FP64 outer step: phi_out = phi_in + e, f0 = b - A phi_out (stencil_at,
-// pb_solver.rs:159-194), block partial sums of r^2.
+// pb_solver.rs:159-194), block partial sums of r^2.  The charge density rho is
+// read by the caller (interior nodes only) and passed in.
+__device__ __forceinline__ double outer_node(const MgOuterDev &O, int64_t idx, bool interior,
+                                             double rho)
+{
+    const int nx = O.nx, ny = O.ny;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    // descriptor pointers are device memory: global (not flat) accesses
+    const auto e = to_global<const float>(O.e);
+    const auto p = to_global<const double>(O.phi_in);
+    const auto po = to_global<double>(O.phi_out);
+    const auto f0 = to_global<float>(O.f0);
+    const double pc = e ? p[idx] + (double)e[idx] : p[idx];
+    if (e) po[idx] = pc;
+    if (!interior) {
+        f0[idx] = 0.0f;
+        return 0.0;
+    }
+    double pxp, pxm, pyp, pym, pzp, pzm;
+    if (e) {
+        pxp = p[idx + 1] + (double)e[idx + 1];
+        pxm = p[idx - 1] + (double)e[idx - 1];
+        pyp = p[idx + sy] + (double)e[idx + sy];
+        pym = p[idx - sy] + (double)e[idx - sy];
+        pzp = p[idx + sz] + (double)e[idx + sz];
+        pzm = p[idx - sz] + (double)e[idx - sz];
+    } else {
+        pxp = p[idx + 1];
+        pxm = p[idx - 1];
+        pyp = p[idx + sy];
+        pym = p[idx - sy];
+        pzp = p[idx + sz];
+        pzm = p[idx - sz];
+    }
+    double exp_, exm, eyp, eym, ezp, ezm, k2;
+    if (O.uni) {
+        exp_ = exm = eyp = eym = ezp = ezm = O.eps_u;
+        k2 = 0.0;
+    } else {
+        // coeffs<double> with global accesses
+        const auto cw = to_global<const uint8_t>(O.cw);
+        bool arrays = true;
+        if (O.cw) {
+            const uint32_t w = cw[idx];
+            k2 = (w & kCodeKappa) ? O.c_k2 : 0.0;
+            if (!(w & kCodeSpecial)) {
+                exp_ = face_bit<double>(w, 0, O.c_in, O.c_out);
+                exm = face_bit<double>(w, 1, O.c_in, O.c_out);
+                eyp = face_bit<double>(w, 2, O.c_in, O.c_out);
+                eym = face_bit<double>(w, 3, O.c_in, O.c_out);
+                ezp = face_bit<double>(w, 4, O.c_in, O.c_out);
+                ezm = face_bit<double>(w, 5, O.c_in, O.c_out);
+                arrays = false;
+            }
+        } else {
+            k2 = to_global<const double>(O.k2)[idx];
+        }
+        if (arrays) {
+            const auto ex = to_global<const double>(O.ex);
+            const auto ey = to_global<const double>(O.ey);
+            const auto ez = to_global<const double>(O.ez);
+            exp_ = ex[idx];
+            exm = ex[idx - 1];
+            eyp = ey[idx];
+            eym = ey[idx - sy];
+            ezp = ez[idx];
+            ezm = ez[idx - sz];
+        }
+    }
+    const double nb = (exp_ * pxp + exm * pxm) * O.ihx + (eyp * pyp + eym * pym) * O.ihy +
+                      (ezp * pzp + ezm * pzm) * O.ihz;
+    const double dg = (exp_ + exm) * O.ihx + (eyp + eym) * O.ihy + (ezp + ezm) * O.ihz + k2;
+    const double r = O.scale * rho + nb - dg * pc;
+    f0[idx] = (float)r;
+    return r * r;
+}
+
+// One block: 256 nodes of solve s and, when it has a partner (same grid, same
+// charge density: the solvated and reference solves of one grid), the same
+// nodes of the partner, so rho is read once for both.  Either side may have
+// stopped; the work list holds the lead solve while either side runs.
 __global__ __launch_bounds__(256) void outer64_kernel(const MgOuterDev *__restrict__ Os,
                                                       const MgStatus *__restrict__ st,
                                                       const uint32_t *__restrict__ wl)
 {
-    __shared__ double red[4];
+    __shared__ double red[2][4];
     const BatchBlock bb = batch_block(wl);
     const int s = bb.s;
-    if (stopped(st + s)) return;
     const MgOuterDev O = Os[s];
+    const int s2 = O.partner;
+    const bool ra = !stopped(st + s), rb = s2 >= 0 && !stopped(st + s2);
+    if (!ra && !rb) return;
     if (bb.b >= O.nbk) return;
     const int nx = O.nx, ny = O.ny, nz = O.nz;
     const int64_t N = (int64_t)nx * ny * nz;
     const int64_t idx = (int64_t)bb.b * blockDim.x + threadIdx.x;
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
-    double r2acc = 0.0;
+    double a2 = 0.0, b2 = 0.0;
     if (idx < N) {
-        // descriptor pointers are device memory: global (not flat) accesses
-        const auto e = to_global<const float>(O.e);
-        const auto p = to_global<const double>(O.phi_in);
-        const auto po = to_global<double>(O.phi_out);
-        const auto f0 = to_global<float>(O.f0);
-        const double pc = e ? p[idx] + (double)e[idx] : p[idx];
-        if (e) po[idx] = pc;
         int ix, iy, iz;
         split3(O, idx, ix, iy, iz);
-        if (ix == 0 || iy == 0 || iz == 0 || ix == nx - 1 || iy == ny - 1 || iz == nz - 1) {
-            f0[idx] = 0.0f;
-        } else {
-            double pxp, pxm, pyp, pym, pzp, pzm;
-            if (e) {
-                pxp = p[idx + 1] + (double)e[idx + 1];
-                pxm = p[idx - 1] + (double)e[idx - 1];
-                pyp = p[idx + sy] + (double)e[idx + sy];
-                pym = p[idx - sy] + (double)e[idx - sy];
-                pzp = p[idx + sz] + (double)e[idx + sz];
-                pzm = p[idx - sz] + (double)e[idx - sz];
-            } else {
-                pxp = p[idx + 1];
-                pxm = p[idx - 1];
-                pyp = p[idx + sy];
-                pym = p[idx - sy];
-                pzp = p[idx + sz];
-                pzm = p[idx - sz];
-            }
-            double exp_, exm, eyp, eym, ezp, ezm, k2;
-            if (O.uni) {
-                exp_ = exm = eyp = eym = ezp = ezm = O.eps_u;
-                k2 = 0.0;
-            } else {
-                // coeffs<double> with global accesses
-                const auto cw = to_global<const uint8_t>(O.cw);
-                bool arrays = true;
-                if (O.cw) {
-                    const uint32_t w = cw[idx];
-                    k2 = (w & kCodeKappa) ? O.c_k2 : 0.0;
-                    if (!(w & kCodeSpecial)) {
-                        exp_ = face_bit<double>(w, 0, O.c_in, O.c_out);
-                        exm = face_bit<double>(w, 1, O.c_in, O.c_out);
-                        eyp = face_bit<double>(w, 2, O.c_in, O.c_out);
-                        eym = face_bit<double>(w, 3, O.c_in, O.c_out);
-                        ezp = face_bit<double>(w, 4, O.c_in, O.c_out);
-                        ezm = face_bit<double>(w, 5, O.c_in, O.c_out);
-                        arrays = false;
-                    }
-                } else {
-                    k2 = to_global<const double>(O.k2)[idx];
-                }
-                if (arrays) {
-                    const auto ex = to_global<const double>(O.ex);
-                    const auto ey = to_global<const double>(O.ey);
-                    const auto ez = to_global<const double>(O.ez);
-                    exp_ = ex[idx];
-                    exm = ex[idx - 1];
-                    eyp = ey[idx];
-                    eym = ey[idx - sy];
-                    ezp = ez[idx];
-                    ezm = ez[idx - sz];
-                }
-            }
-            const double nb = (exp_ * pxp + exm * pxm) * O.ihx + (eyp * pyp + eym * pym) * O.ihy +
-                              (ezp * pzp + ezm * pzm) * O.ihz;
-            const double dg = (exp_ + exm) * O.ihx + (eyp + eym) * O.ihy + (ezp + ezm) * O.ihz + k2;
-            const double r = O.scale * to_global<const double>(O.rho)[idx] + nb - dg * pc;
-            f0[idx] = (float)r;
-            r2acc = r * r;
-        }
+        const bool interior =
+            !(ix == 0 || iy == 0 || iz == 0 || ix == nx - 1 || iy == ny - 1 || iz == nz - 1);
+        const double rho = interior ? to_global<const double>(O.rho)[idx] : 0.0;
+        if (ra) a2 = outer_node(O, idx, interior, rho);
+        if (rb) b2 = outer_node(Os[s2], idx, interior, rho);
     }
-    r2acc = wave_sum(r2acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r2acc;
+    a2 = wave_sum(a2);
+    b2 = wave_sum(b2);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a2;
+        red[1][threadIdx.x >> 6] = b2;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) O.partials[bb.b] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) {
+        if (ra) O.partials[bb.b] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        if (rb) Os[s2].partials[bb.b] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
 }
 
 // Convergence / divergence test of solve_lpbe_multigrid (pb_solver.rs:1085-1147),

@@ -391,6 +391,7 @@ static MgOuterDev outer_view(const GridDesc &g, const MapSet &m, const double *r
     O.rho = rho;
     O.fd_nx = FastDiv((uint32_t)std::max(1, g.nx));
     O.fd_nxy = FastDiv((uint32_t)std::max(1, g.nx * g.ny));
+    O.partner = -1;
     return O;
 }
 
@@ -723,6 +724,24 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
         }
     }
 
+    // Solves with the same grid and charge density (the solvated and reference
+    // solves of one grid) pair up for the FP64 outer step: the lead's blocks
+    // also do the partner's nodes, so rho is read once for both.
+    std::vector<int> partner(S, -1), lead_of(S, -1);
+    for (int s = 0; s < S; ++s) {
+        if (lead_of[s] >= 0) continue;
+        for (int s2 = s + 1; s2 < S; ++s2) {
+            const GridDesc &a = W[s].t->g, &b = W[s2].t->g;
+            if (lead_of[s2] >= 0 || partner[s2] >= 0 || W[s2].t->rho != W[s].t->rho ||
+                a.nx != b.nx || a.ny != b.ny || a.nz != b.nz)
+                continue;
+            partner[s] = s2;
+            lead_of[s2] = s;
+            break;
+        }
+        W[s].O0.partner = W[s].O[0].partner = W[s].O[1].partner = partner[s];
+    }
+
     // coarsened dielectrics and kappa (coarsen_dielectrics / coarsen_kappa,
     // pb_solver.rs:684-748) and their code bytes, level by level for all
     // solves with non-uniform maps at once
@@ -911,7 +930,11 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             std::vector<uint32_t> ids, pre(1, 0);
             int64_t acc = 0;
             for (int s = 0; s < S; ++s) {
-                const int64_t b = active[s] ? kind_blocks(kind, l, s) : 0;
+                // the outer step lists a lead while either side of its pair runs
+                const bool on = kind >= 0 ? (bool)active[s]
+                                          : lead_of[s] < 0 &&
+                                                (active[s] || (partner[s] >= 0 && active[partner[s]]));
+                const int64_t b = on ? kind_blocks(kind, l, s) : 0;
                 if (b <= 0) continue;
                 ids.push_back((uint32_t)s);
                 acc += b;
@@ -1048,8 +1071,15 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
     } graph_guard{drop_graph};
     if (use_graph) capture();
     // blocks of the FP64 outer step of the running solves when the lists were
-    // last built (a proxy for the level-0 work of every launch)
-    int64_t listed = tot_of(WL_OUT);
+    // last built (a proxy for the level-0 work of every launch), counted per
+    // solve
+    auto active_work = [&]() {
+        int64_t a = 0;
+        for (int s = 0; s < S; ++s)
+            if (active[s]) a += kind_blocks(-1, 0, s);
+        return a;
+    };
+    int64_t listed = active_work();
     const int chunk_cycles = std::max(2, tun.check_cycles);
     int done_cycles = 0;
     while (done_cycles < maxv) {
@@ -1077,7 +1107,7 @@ static void mg_solve_group(Worker &w, std::vector<MgTask *> &tasks, bool use_gra
             if (4 * running <= 3 * listed) {
                 for (int s = 0; s < S; ++s) active[s] = active[s] && hs[s].state == 0;
                 build_lists();
-                listed = tot_of(WL_OUT);
+                listed = active_work();
                 if (use_graph) {
                     drop_graph();
                     capture();
