@@ -1045,15 +1045,21 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
             // bit j: code bit 7 (fractional faces) of update j
             const uint32_t hb = (c ? fmask >> 16 : fmask) & 0xfffu;
             spm = hb & zmask & (c ? spread[1] : spread[0]);
-            const int cnt = __popc(spm);
-            int inc = cnt;
+            // exclusive prefix sum of the lane counts (0..12) and the wave
+            // total from one ballot per count bit: mbcnt gives the set bits
+            // below the lane, so no cross-lane shuffles (LDS round trips)
+            const uint32_t cnt = (uint32_t)__popc(spm);
+            int pre = 0, tot = 0;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int y = __shfl_up(inc, d, 64);
-                inc += lane >= d ? y : 0;
+            for (int bit = 0; bit < 4; ++bit) {
+                const uint64_t bm = __ballot((cnt >> bit) & 1u);
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+                pre += (int)below << bit;
+                tot += __popcll(bm) << bit;
             }
-            spx = inc - cnt;
-            total = __builtin_amdgcn_readfirstlane(__shfl(inc, 63, 64));
+            spx = pre;
+            total = __builtin_amdgcn_readfirstlane(tot);
             if (total == 0) return;
             // requests: node index of each numbered update
             const uint32_t a0 = cvo[0] + (uint32_t)c + sxy * (uint32_t)(t + 1);
