@@ -565,7 +565,6 @@ namespace s3 {
 constexpr int K = 6;                   // half-sweeps per pass
 constexpr int H = K;                   // halo
 constexpr int RING = 8;                // correction planes t-6 .. t+1
-constexpr int SLOTS = 32;              // fractional-face slots per wave and step
 constexpr int TAB = 128;               // code-table entries
 constexpr uint32_t kOob = 0x80000000u; // offset outside every array
 constexpr int kRsrcWord3 = 0x00020000; // raw buffer, 32-bit data format
@@ -580,9 +579,10 @@ __host__ __device__ constexpr int inner(int n) { return n > 2 ? n - 2 : 1; }
 // columns 2 (t % PPR) and the next); NT rounds NP up to whole waves, and the
 // threads past NP own no pair (they load, store and write back only their
 // share of the plane and write-back rectangle).
-template <int RW_, int RH_>
+template <int RW_, int RH_, int SL_>
 struct Geo {
     static constexpr int RW = RW_, RH = RH_, RN = RW * RH;
+    static constexpr int SLOTS = SL_;                       // fractional-face slots per wave
     static constexpr int TX = RW - 2 * H, TY = RH - 2 * H;  // core
     static constexpr int PPR = RW / 2;                      // column pairs per row
     static constexpr int PR = RH / 2;                       // rows between a thread's pairs
@@ -598,11 +598,13 @@ struct Geo {
     static_assert(RW % 2 == 0 && PR % 2 == 0 && TAB <= NT, "region layout");
 };
 // 32 x 32 (20 x 20 core): 256 threads, 40 KB, 4 workgroups per CU.
-// 44 x 44 (32 x 32 core): 512 threads (484 own pairs), 72.5 KB, 2 per CU.
+// 44 x 44 (32 x 32 core): 512 threads (484 own pairs), 79.5 KB, 2 per CU.
 // Both give 16 waves per CU.  (A 32 x 40 region with 320 threads measured 22%
 // slower than 32 x 32 on 161^3: its 5-wave workgroups lowered the waves per CU.)
-using Small = Geo<32, 32>;
-using Large = Geo<44, 44>;
+// The slots per wave fill what the LDS budget leaves: 32 in the small region
+// (4 workgroups per CU, 40 KB each), 60 in the large one (2 per CU, 79.5 KB).
+using Small = Geo<32, 32, 32>;
+using Large = Geo<44, 44, 60>;
 
 // Thread -> pair of the 44 x 44 region, row << 8 | pair column (0xffff: no
 // pair).  The LDS has 32 banks of 4 bytes; a half-sweep's reads by one wave
@@ -747,7 +749,7 @@ __global__ __launch_bounds__(G::NT, 4) void smooth3_kernel(const MgLevelDev *__r
     using namespace s3;
     constexpr int RW = G::RW, RH = G::RH, RN = G::RN, NT = G::NT, NW = G::NW, TX = G::TX,
                   TY = G::TY, PF = G::PF, PP = G::PP, PR = G::PR, PPR = G::PPR, NP = G::NP,
-                  WX = G::WX, WB = G::WB;
+                  WX = G::WX, WB = G::WB, SLOTS = G::SLOTS;
     __shared__ float ring[RING * RN];
     // face entries {a0..a3}, {a4, a5, 1/diag, -}: rows 0..127 the code table,
     // then SLOTS per wave for the step's fractional-face nodes
