@@ -50,6 +50,9 @@ struct mmpbsa_split {
     // decomposition: complex residue -> positions in [rec atoms..., lig atoms...]
     DArr<int> d_seg_pos, d_seg_ptr;
     int cross_excl = 0;
+    // the complex is exactly the two partners and no excluded, 1-4 or bonded
+    // term spans them: its MM terms come from the partners' and the cross pairs
+    bool mm_parts = false;
     std::vector<std::unique_ptr<Worker>> workers;
     // results of the last trajectory call
     std::vector<mmpbsa_frame_terms> traj_out;
@@ -240,10 +243,14 @@ void run_items(Worker &w, const std::vector<FrameItem> &items, bool batched, boo
         const FrameItem &it = items[f];
         mmpbsa_frame_terms *o = it.out;
         mmpbsa_subsystem_energy *e[3] = {&o->complex_, &o->receptor, &o->ligand};
+        mmpbsa_mm_terms mm3[3];
+        const mmpbsa_split *sp = items[f].s;
+        if (sp->mm_parts)
+            run_mm_parts(w, sp->ctop, sp->rtop, sp->ltop, dptr(f, 0), dptr(f, 1), dptr(f, 2),
+                         sp->d_ra.p, (int)sp->ra.size(), sp->d_la.p, (int)sp->la.size(), mm3);
         for (int k = 0; k < 3; ++k) {
-            mmpbsa_mm_terms mm;
-            run_mm(w, *tops(f, k), dptr(f, k), &mm);
-            e[k]->mm = mm_total(mm);
+            if (!sp->mm_parts) run_mm(w, *tops(f, k), dptr(f, k), &mm3[k]);
+            e[k]->mm = mm_total(mm3[k]);
             e[k]->polar = it.method == MMPBSA_METHOD_PB
                               ? pr[3 * f + k].total
                               : run_gb(w, *tops(f, k), dptr(f, k), it.gb, nullptr, nullptr);
@@ -811,6 +818,19 @@ int mmpbsa_split_create(mmpbsa_ctx *ctx, const int32_t *receptor_res, int32_t n_
         };
         scan(s->ctop.h_ex_ptr, s->ctop.h_ex_col);
         scan(s->ctop.h_p14_ptr, s->ctop.h_p14_col);
+        bool parts = !s->cross_excl;
+        for (int a = 0; a < ctop.n && parts; ++a) parts = is_r[a] != is_l[a];
+        auto one_side = [&](const std::vector<int> &terms, int stride, int atoms) {
+            for (size_t b = 0; b + stride <= terms.size() && parts; b += stride) {
+                const bool r = is_r[terms[b]];
+                for (int k = 1; k < atoms; ++k)
+                    if ((bool)is_r[terms[b + k]] != r) parts = false;
+            }
+        };
+        one_side(ctop.bonds, 3, 2);
+        one_side(ctop.angles, 4, 3);
+        one_side(ctop.dihs, 6, 4);
+        s->mm_parts = parts;
     });
     if (rc == MMPBSA_OK) *out = s.release();
     return rc;

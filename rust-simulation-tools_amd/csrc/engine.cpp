@@ -49,6 +49,57 @@ void run_mm(Worker &w, const DevTopo &t, const double *d_xyz, mmpbsa_mm_terms *o
     out->elec_14 = w.pin[6];
 }
 
+// MM terms of the complex, receptor and ligand of one frame (out[0..2]) when
+// the complex is exactly the two partners and no excluded, 1-4 or bonded term
+// spans them: the complex's pair terms are the partners' own pair terms plus
+// the receptor x ligand pairs, so its receptor-receptor and ligand-ligand
+// pairs are evaluated once, not twice.  Every pair term is the one the
+// complex's own pass computes; the complex's sums run over the partners' and
+// the cross blocks (double-double, then rounded once), its bonded sums over
+// the partners' bonded blocks.
+void run_mm_parts(Worker &w, const DevTopo &ct, const DevTopo &rt, const DevTopo &lt,
+                  const double *dc, const double *dr, const double *dl, const int *d_ra, int nr,
+                  const int *d_la, int nl, mmpbsa_mm_terms out[3])
+{
+    const PairTopo vc = ct.view(), vr = rt.view(), vl = lt.view();
+    const int nbr = bonded_blocks(vr), nbl = bonded_blocks(vl);
+    const int nmr = vr.n >= 2 ? mm_blocks(vr.n) : 0, nml = vl.n >= 2 ? mm_blocks(vl.n) : 0;
+    const int ncx = mm_cross_blocks(nr, nl);
+    DBuf<double> pb(w.pool, (size_t)(nbr + nbl) * 3), pm(w.pool, (size_t)(nmr + nml + ncx) * 8 + 8),
+        res(w.pool, 21);
+    launch_bonded(dr, vr, pb.get(), w.stream);
+    launch_bonded(dl, vl, pb.get() + (size_t)nbr * 3, w.stream);
+    {
+        KernelTimer kt(w, "mm_pair_kernel",
+                       10.0 * ((double)vr.n * (vr.n - 1) + (double)vl.n * (vl.n - 1)));
+        if (nmr) launch_mm_pairs(dr, vr, pm.get(), w.stream);
+        if (nml) launch_mm_pairs(dl, vl, pm.get() + (size_t)nmr * 8, w.stream);
+    }
+    {
+        KernelTimer kt(w, "mm_cross_kernel", 20.0 * (double)nr * nl);
+        launch_mm_cross(dc, vc, d_ra, nr, d_la, nl, pm.get() + (size_t)(nmr + nml) * 8, w.stream);
+    }
+    // complex: every block; receptor: the first ones; ligand: the next ones
+    launch_sum_partials(pb.get(), nbr + nbl, 3, 3, res.get(), w.stream);
+    launch_sum_partials_dd(pm.get(), nmr + nml + ncx, 4, res.get() + 3, w.stream);
+    launch_sum_partials(pb.get(), nbr, 3, 3, res.get() + 7, w.stream);
+    launch_sum_partials_dd(pm.get(), nmr, 4, res.get() + 10, w.stream);
+    launch_sum_partials(pb.get() + (size_t)nbr * 3, nbl, 3, 3, res.get() + 14, w.stream);
+    launch_sum_partials_dd(pm.get() + (size_t)nmr * 8, nml, 4, res.get() + 17, w.stream);
+    HIP_OK(hipMemcpyAsync(w.pin, res.get(), sizeof(double) * 21, hipMemcpyDeviceToHost, w.stream));
+    w.sync();
+    for (int k = 0; k < 3; ++k) {
+        const double *v = w.pin + 7 * k;
+        out[k].bond = v[0];
+        out[k].angle = v[1];
+        out[k].dihedral = v[2];
+        out[k].vdw = v[3];
+        out[k].elec = v[4];
+        out[k].vdw_14 = v[5];
+        out[k].elec_14 = v[6];
+    }
+}
+
 // ---- GB ----------------------------------------------------------------------
 double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_params &p,
               double *d_born, std::vector<double> *born_out)

@@ -150,6 +150,12 @@ struct SolveInfo {
 
 // ---- drivers (device coordinates already resident on w.stream) ------------
 void run_mm(Worker &w, const DevTopo &t, const double *d_xyz, mmpbsa_mm_terms *out);
+// complex, receptor and ligand MM terms of one frame from the partners' pairs
+// and the receptor x ligand pairs (a split whose complex is exactly the two
+// partners, with no excluded, 1-4 or bonded term across them)
+void run_mm_parts(Worker &w, const DevTopo &ct, const DevTopo &rt, const DevTopo &lt,
+                  const double *dc, const double *dr, const double *dl, const int *d_ra, int nr,
+                  const int *d_la, int nl, mmpbsa_mm_terms out[3]);
 double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_params &p,
               double *d_born /* nullable: pooled scratch used */, std::vector<double> *born_out);
 double run_gb_from_radii(Worker &w, const DevTopo &t, const double *d_xyz,

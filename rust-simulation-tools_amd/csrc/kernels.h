@@ -49,6 +49,12 @@ void launch_gb_energy(const double *xyz, const PairTopo &t, const double *born, 
 // MM: 8 doubles (4 dd terms) per tile block, plus bonded partials.
 int mm_blocks(int n);
 void launch_mm_pairs(const double *xyz, const PairTopo &t, double *partials, hipStream_t s);
+// Receptor x ligand pairs of a complex (A, B: complex atom indices; no
+// excluded or 1-4 pair across them): mm_pairs' vdW and Coulomb terms of those
+// pairs, 8 doubles per block as launch_mm_pairs.
+int mm_cross_blocks(int nA, int nB);
+void launch_mm_cross(const double *xyz, const PairTopo &t, const int *A, int nA, const int *B,
+                     int nB, double *partials, hipStream_t s);
 int bonded_blocks(const PairTopo &t);
 void launch_bonded(const double *xyz, const PairTopo &t, double *partials, hipStream_t s);
 // Deterministic reductions of block partials.

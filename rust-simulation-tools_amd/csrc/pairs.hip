@@ -336,6 +336,75 @@ __global__ __launch_bounds__(kTile) void mm_pair_kernel(const double *__restrict
     block_store_dd<kTile>(e14, red, outp + 6);
 }
 
+// The receptor x ligand pairs of a complex whose partners share no excluded
+// or 1-4 pair: the vdW and Coulomb terms of mm_pair_kernel for exactly those
+// pairs (same expressions, the LJ table read in the complex's index order), in
+// its partial layout (vdw, elec, vdw_14 = 0, elec_14 = 0 as double-doubles).
+// Grid (ceil(nA / kTile), ceil(nB / kTile)); A and B hold complex atom indices.
+__global__ __launch_bounds__(kTile) void mm_cross_kernel(const double *__restrict__ xyz,
+                                                         PairTopo T, const int *__restrict__ A,
+                                                         int nA, const int *__restrict__ Bv,
+                                                         int nB, double *__restrict__ partials)
+{
+    __shared__ double sx[kTile], sy[kTile], sz[kTile], sq[kTile];
+    __shared__ int st[kTile], sid[kTile];
+    __shared__ dd red[kTile / 64];
+    const int t = threadIdx.x;
+    const int ia = blockIdx.x * kTile + t;
+    const int j0 = blockIdx.y * kTile;
+    double *outp = partials + ((size_t)blockIdx.x * gridDim.y + blockIdx.y) * 8;
+    if (j0 + t < nB) {
+        const int j = Bv[j0 + t];
+        sid[t] = j;
+        sx[t] = xyz[3 * j];
+        sy[t] = xyz[3 * j + 1];
+        sz[t] = xyz[3 * j + 2];
+        sq[t] = T.q_amber[j];
+        st[t] = T.type[j];
+    }
+    __syncthreads();
+    dd vdw = {0, 0}, elec = {0, 0};
+    if (ia < nA) {
+        const int i = A[ia];
+        const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+        const double qi = T.q_amber[i];
+        const int ti = T.type[i];
+        const int m = min(kTile, nB - j0);
+        for (int k = 0; k < m; ++k) {
+            const double dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double ir = 1.0 / sqrt(r2);
+            const double ir2 = ir * ir;
+            const double ir6 = ir2 * ir2 * ir2;
+            // row of the lower complex index, as mm_pair_kernel
+            const size_t cell = i < sid[k] ? (size_t)ti * T.n_types + st[k]
+                                           : (size_t)st[k] * T.n_types + ti;
+            const double LA = T.ljA[cell], LB = T.ljB[cell];
+            const double lj = LA * ir6 * ir6 - LB * ir6;
+            const double cq = qi * sq[k] * ir;
+            dd_add(vdw, lj);
+            dd_add(elec, cq);
+        }
+    }
+    block_store_dd<kTile>(vdw, red, outp + 0);
+    block_store_dd<kTile>(elec, red, outp + 2);
+    if (t < 4) outp[4 + t] = 0.0;
+}
+
+int mm_cross_blocks(int nA, int nB)
+{
+    return nA > 0 && nB > 0 ? ceil_div(nA, kTile) * ceil_div(nB, kTile) : 0;
+}
+
+void launch_mm_cross(const double *xyz, const PairTopo &t, const int *A, int nA, const int *B,
+                     int nB, double *partials, hipStream_t s)
+{
+    if (nA <= 0 || nB <= 0) return;
+    hipLaunchKernelGGL(mm_cross_kernel, dim3(ceil_div(nA, kTile), ceil_div(nB, kTile)),
+                       dim3(kTile), 0, s, xyz, t, A, nA, B, nB, partials);
+    LAUNCH_OK();
+}
+
 int mm_blocks(int n)
 {
     const int nb = std::max(1, ceil_div(n, kTile));
