@@ -9,6 +9,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -229,34 +230,75 @@ void run_items(Worker &w, const std::vector<FrameItem> &items, bool batched, boo
             jobs.push_back(j);
         }
     }
-    if (!jobs.empty()) {
-        if (batched) {
-            run_pb_jobs(w, jobs, true);
-        } else {
-            for (auto &j : jobs) {
-                std::vector<PbJob> one(1, j);
-                run_pb_jobs(w, one, false);
+    // MM, GB and SA of every frame: their own kernels on their own inputs,
+    // independent of the PB solves.
+    std::vector<double> mmv(3 * nf), gbv(3 * nf, 0.0), sav(3 * nf);
+    auto other_terms = [&](Worker &x) {
+        for (size_t f = 0; f < nf; ++f) {
+            const FrameItem &it = items[f];
+            const mmpbsa_split *sp = it.s;
+            mmpbsa_mm_terms mm3[3];
+            if (sp->mm_parts)
+                run_mm_parts(x, sp->ctop, sp->rtop, sp->ltop, dptr(f, 0), dptr(f, 1), dptr(f, 2),
+                             sp->d_ra.p, (int)sp->ra.size(), sp->d_la.p, (int)sp->la.size(), mm3);
+            for (int k = 0; k < 3; ++k) {
+                if (!sp->mm_parts) run_mm(x, *tops(f, k), dptr(f, k), &mm3[k]);
+                mmv[3 * f + k] = mm_total(mm3[k]);
+                if (it.method != MMPBSA_METHOD_PB)
+                    gbv[3 * f + k] = run_gb(x, *tops(f, k), dptr(f, k), it.gb, nullptr, nullptr);
+                double tsasa = 0.0, tsa = 0.0;
+                run_sa(x, *tops(f, k), dptr(f, k), hptr(f, k), it.sa, nullptr, nullptr, &tsasa,
+                       &tsa);
+                sav[3 * f + k] = tsa;
             }
         }
+    };
+    if (!jobs.empty() && batched && !profile_stats().enabled) {
+        // Beside the batched solver: on the companion worker (its own stream)
+        // in a second host thread, so these kernels fill the GPU while the
+        // solver's narrow launches (the one-workgroup tail, small levels) and
+        // host checks leave it idle.  The staged coordinates are complete
+        // before the other stream reads them; profiled runs keep them apart so
+        // that kernel timers see each kernel alone.
+        w.sync();
+        Worker &side = w.companion();
+        std::exception_ptr err;
+        std::thread th([&] {
+            try {
+                other_terms(side);
+            } catch (...) {
+                err = std::current_exception();
+            }
+        });
+        try {
+            run_pb_jobs(w, jobs, true);
+        } catch (...) {
+            th.join();
+            throw;
+        }
+        th.join();
+        if (err) std::rethrow_exception(err);
+    } else {
+        if (!jobs.empty()) {
+            if (batched) {
+                run_pb_jobs(w, jobs, true);
+            } else {
+                for (auto &j : jobs) {
+                    std::vector<PbJob> one(1, j);
+                    run_pb_jobs(w, one, false);
+                }
+            }
+        }
+        other_terms(w);
     }
     for (size_t f = 0; f < nf; ++f) {
         const FrameItem &it = items[f];
         mmpbsa_frame_terms *o = it.out;
         mmpbsa_subsystem_energy *e[3] = {&o->complex_, &o->receptor, &o->ligand};
-        mmpbsa_mm_terms mm3[3];
-        const mmpbsa_split *sp = items[f].s;
-        if (sp->mm_parts)
-            run_mm_parts(w, sp->ctop, sp->rtop, sp->ltop, dptr(f, 0), dptr(f, 1), dptr(f, 2),
-                         sp->d_ra.p, (int)sp->ra.size(), sp->d_la.p, (int)sp->la.size(), mm3);
         for (int k = 0; k < 3; ++k) {
-            if (!sp->mm_parts) run_mm(w, *tops(f, k), dptr(f, k), &mm3[k]);
-            e[k]->mm = mm_total(mm3[k]);
-            e[k]->polar = it.method == MMPBSA_METHOD_PB
-                              ? pr[3 * f + k].total
-                              : run_gb(w, *tops(f, k), dptr(f, k), it.gb, nullptr, nullptr);
-            double tsasa = 0.0, tsa = 0.0;
-            run_sa(w, *tops(f, k), dptr(f, k), hptr(f, k), it.sa, nullptr, nullptr, &tsasa, &tsa);
-            e[k]->sa = tsa;
+            e[k]->mm = mmv[3 * f + k];
+            e[k]->polar = it.method == MMPBSA_METHOD_PB ? pr[3 * f + k].total : gbv[3 * f + k];
+            e[k]->sa = sav[3 * f + k];
         }
         o->delta_mm = o->complex_.mm - o->receptor.mm - o->ligand.mm;
         o->delta_polar = o->complex_.polar - o->receptor.polar - o->ligand.polar;

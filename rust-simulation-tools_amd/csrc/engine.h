@@ -139,6 +139,12 @@ struct Worker {
     Worker();
     ~Worker();
     void sync();
+    // a second worker (own stream and pool) for work that runs beside this
+    // one's in another host thread; made on first use
+    Worker &companion();
+
+private:
+    std::unique_ptr<Worker> side_;
 };
 
 // Results of one PB solve on one grid (solve_lpbe_multigrid semantics).

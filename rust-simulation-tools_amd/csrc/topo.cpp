@@ -394,4 +394,10 @@ Worker::~Worker()
 
 void Worker::sync() { HIP_OK(hipStreamSynchronize(stream)); }
 
+Worker &Worker::companion()
+{
+    if (!side_) side_ = std::make_unique<Worker>();
+    return *side_;
+}
+
 }  // namespace mmpbsa
