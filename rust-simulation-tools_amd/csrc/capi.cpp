@@ -254,14 +254,14 @@ void run_items(Worker &w, const std::vector<FrameItem> &items, bool batched, boo
         }
     };
     if (!jobs.empty() && batched && !profile_stats().enabled) {
-        // Beside the batched solver: on the companion worker (its own stream)
+        // Beside the batched solver: on an auxiliary worker (its own stream)
         // in a second host thread, so these kernels fill the GPU while the
         // solver's narrow launches (the one-workgroup tail, small levels) and
         // host checks leave it idle.  The staged coordinates are complete
         // before the other stream reads them; profiled runs keep them apart so
         // that kernel timers see each kernel alone.
         w.sync();
-        Worker &side = w.companion();
+        Worker &side = w.aux(0);
         std::exception_ptr err;
         std::thread th([&] {
             try {

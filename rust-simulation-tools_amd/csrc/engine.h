@@ -139,12 +139,13 @@ struct Worker {
     Worker();
     ~Worker();
     void sync();
-    // a second worker (own stream and pool) for work that runs beside this
-    // one's in another host thread; made on first use
-    Worker &companion();
+    // further workers (own stream and pool) for work that runs beside this
+    // one's in other host threads, made on first use: 0 the frames' MM, GB
+    // and SA terms, 1 the fine dielectric maps during the coarse PB stage
+    Worker &aux(int i);
 
 private:
-    std::unique_ptr<Worker> side_;
+    std::unique_ptr<Worker> aux_[2];
 };
 
 // Results of one PB solve on one grid (solve_lpbe_multigrid semantics).

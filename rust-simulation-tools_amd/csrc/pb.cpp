@@ -6,9 +6,11 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <map>
 #include <string>
+#include <thread>
 
 #include "engine.h"
 
@@ -1327,6 +1329,39 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
         S.side[1].ionic = false;
         S.side[1].info = &j->out->ref;
     }
+    // The fine stage's dielectric maps depend only on the coordinates, the
+    // cell lists and the fine grids.  With batched solves (and no profiling)
+    // they are built on an auxiliary worker in a second host thread while the
+    // coarse stage runs, so the flood fills' short launches and host checks
+    // overlap the coarse solves.
+    std::vector<MapReq> rq_fine;
+    for (PbJob *j : act) {
+        PbJob::State &S = *j->st;
+        for (auto &sd : S.side)
+            rq_fine.push_back(MapReq{j->t, j->d_xyz, &S.cells, S.fine, &j->p, sd.ein, sd.eout,
+                                     sd.ionic, S.kappa, &sd.fm});
+    }
+    struct Joiner {
+        std::thread t;
+        ~Joiner()
+        {
+            if (t.joinable()) t.join();
+        }
+    } fine_maps;
+    std::exception_ptr maps_err;
+    const bool maps_beside = batched && !profile_stats().enabled && !rq_fine.empty();
+    if (maps_beside) {
+        w.sync();  // cell lists and coordinates complete before the other stream reads them
+        Worker &mw = w.aux(1);
+        fine_maps.t = std::thread([&rq_fine, &mw, &maps_err] {
+            try {
+                build_maps_batch(mw, rq_fine);
+                mw.sync();
+            } catch (...) {
+                maps_err = std::current_exception();
+            }
+        });
+    }
     // coarse focusing solves with Debye-Hueckel boundaries (pb_energy.rs:150-186)
     std::vector<MgTask *> tasks;
     {
@@ -1407,15 +1442,11 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
     }
     // fine solves
     tasks.clear();
-    {
-        std::vector<MapReq> rq;
-        for (PbJob *j : act) {
-            PbJob::State &S = *j->st;
-            for (auto &sd : S.side)
-                rq.push_back(MapReq{j->t, j->d_xyz, &S.cells, S.fine, &j->p, sd.ein, sd.eout,
-                                    sd.ionic, S.kappa, &sd.fm});
-        }
-        build_maps_batch(w, rq);
+    if (maps_beside) {
+        fine_maps.t.join();
+        if (maps_err) std::rethrow_exception(maps_err);
+    } else {
+        build_maps_batch(w, rq_fine);
     }
     for (PbJob *j : act) {
         PbJob::State &S = *j->st;

@@ -394,10 +394,10 @@ Worker::~Worker()
 
 void Worker::sync() { HIP_OK(hipStreamSynchronize(stream)); }
 
-Worker &Worker::companion()
+Worker &Worker::aux(int i)
 {
-    if (!side_) side_ = std::make_unique<Worker>();
-    return *side_;
+    if (!aux_[i]) aux_[i] = std::make_unique<Worker>();
+    return *aux_[i];
 }
 
 }  // namespace mmpbsa
