@@ -8,6 +8,7 @@
 #include <cstring>
 #include <exception>
 #include <functional>
+#include <future>
 #include <map>
 #include <string>
 #include <thread>
@@ -1329,17 +1330,22 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
         S.side[1].ionic = false;
         S.side[1].info = &j->out->ref;
     }
-    // The fine stage's dielectric maps depend only on the coordinates, the
-    // cell lists and the fine grids.  With batched solves (and no profiling)
-    // they are built on an auxiliary worker in a second host thread while the
-    // coarse stage runs, so the flood fills' short launches and host checks
-    // overlap the coarse solves.
-    std::vector<MapReq> rq_fine;
+    // The dielectric and kappa maps depend only on the coordinates, the cell
+    // lists and the grids.  With batched solves (and no profiling) they are
+    // built on an auxiliary worker in a second host thread: the coarse maps
+    // while the main thread computes the coarse Debye-Hueckel boundaries, then
+    // the fine maps while it runs the coarse stage, so the flood fills' short
+    // launches and host checks overlap that work.
+    std::vector<MapReq> rq_coarse, rq_fine;
     for (PbJob *j : act) {
         PbJob::State &S = *j->st;
-        for (auto &sd : S.side)
+        for (auto &sd : S.side) {
+            if (S.focus)
+                rq_coarse.push_back(MapReq{j->t, j->d_xyz, &S.cells, S.cg, &j->p, sd.ein, sd.eout,
+                                           sd.ionic, S.kappa, &sd.cm});
             rq_fine.push_back(MapReq{j->t, j->d_xyz, &S.cells, S.fine, &j->p, sd.ein, sd.eout,
                                      sd.ionic, S.kappa, &sd.fm});
+        }
     }
     struct Joiner {
         std::thread t;
@@ -1347,13 +1353,23 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
         {
             if (t.joinable()) t.join();
         }
-    } fine_maps;
+    } maps_thread;
+    std::promise<void> coarse_ready;
+    std::future<void> coarse_maps = coarse_ready.get_future();
     std::exception_ptr maps_err;
     const bool maps_beside = batched && !profile_stats().enabled && !rq_fine.empty();
     if (maps_beside) {
         w.sync();  // cell lists and coordinates complete before the other stream reads them
         Worker &mw = w.aux(1);
-        fine_maps.t = std::thread([&rq_fine, &mw, &maps_err] {
+        maps_thread.t = std::thread([&rq_coarse, &rq_fine, &mw, &coarse_ready, &maps_err] {
+            try {
+                build_maps_batch(mw, rq_coarse);
+                mw.sync();
+            } catch (...) {
+                coarse_ready.set_exception(std::current_exception());
+                return;
+            }
+            coarse_ready.set_value();
             try {
                 build_maps_batch(mw, rq_fine);
                 mw.sync();
@@ -1361,20 +1377,11 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
                 maps_err = std::current_exception();
             }
         });
+    } else {
+        build_maps_batch(w, rq_coarse);
     }
     // coarse focusing solves with Debye-Hueckel boundaries (pb_energy.rs:150-186)
     std::vector<MgTask *> tasks;
-    {
-        std::vector<MapReq> rq;
-        for (PbJob *j : act) {
-            PbJob::State &S = *j->st;
-            if (!S.focus) continue;
-            for (auto &sd : S.side)
-                rq.push_back(MapReq{j->t, j->d_xyz, &S.cells, S.cg, &j->p, sd.ein, sd.eout,
-                                    sd.ionic, S.kappa, &sd.cm});
-        }
-        build_maps_batch(w, rq);
-    }
     for (PbJob *j : act) {
         PbJob::State &S = *j->st;
         if (!S.focus) continue;
@@ -1393,6 +1400,7 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
             tasks.push_back(&sd.ct);
         }
     }
+    if (maps_beside) coarse_maps.get();  // rethrows a failure of the coarse maps
     if (!tasks.empty()) mg_solve_batch(w, tasks, batched, false);
     // coarse validity (pb_energy.rs:188-194), then fine boundary values
     {
@@ -1433,7 +1441,8 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
                 } else {
                     need[si] = true;
                 }
-                sd.cm = MapSet();
+                // (the coarse maps are released once the map thread is done:
+                // they live in its worker's pool)
                 sd.cphi.release();
                 ++k;
             }
@@ -1443,11 +1452,13 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
     // fine solves
     tasks.clear();
     if (maps_beside) {
-        fine_maps.t.join();
+        maps_thread.t.join();
         if (maps_err) std::rethrow_exception(maps_err);
     } else {
         build_maps_batch(w, rq_fine);
     }
+    for (PbJob *j : act)
+        for (auto &sd : j->st->side) sd.cm = MapSet();
     for (PbJob *j : act) {
         PbJob::State &S = *j->st;
         for (auto &sd : S.side) {
