@@ -1347,6 +1347,11 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
                                      sd.ionic, S.kappa, &sd.fm});
         }
     }
+    // (the thread's promise and error slot are declared before the thread, so
+    // that on unwinding its join runs before they are destroyed)
+    std::promise<void> coarse_ready;
+    std::future<void> coarse_maps = coarse_ready.get_future();
+    std::exception_ptr maps_err;
     struct Joiner {
         std::thread t;
         ~Joiner()
@@ -1354,9 +1359,6 @@ void run_pb_jobs(Worker &w, std::vector<PbJob> &jobs, bool batched)
             if (t.joinable()) t.join();
         }
     } maps_thread;
-    std::promise<void> coarse_ready;
-    std::future<void> coarse_maps = coarse_ready.get_future();
-    std::exception_ptr maps_err;
     const bool maps_beside = batched && !profile_stats().enabled && !rq_fine.empty();
     if (maps_beside) {
         w.sync();  // cell lists and coordinates complete before the other stream reads them
