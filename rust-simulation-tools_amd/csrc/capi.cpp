@@ -51,9 +51,10 @@ struct mmpbsa_split {
     // decomposition: complex residue -> positions in [rec atoms..., lig atoms...]
     DArr<int> d_seg_pos, d_seg_ptr;
     int cross_excl = 0;
-    // the complex is exactly the two partners and no excluded, 1-4 or bonded
-    // term spans them: its MM terms come from the partners' and the cross pairs
-    bool mm_parts = false;
+    // the complex is exactly the two partners: its GB radii come from the
+    // partners' psi sums plus the cross sums; when also no excluded, 1-4 or
+    // bonded term spans them, its MM terms from the partners' and the cross pairs
+    bool gb_parts = false, mm_parts = false;
     std::vector<std::unique_ptr<Worker>> workers;
     // results of the last trajectory call
     std::vector<mmpbsa_frame_terms> traj_out;
@@ -241,10 +242,15 @@ void run_items(Worker &w, const std::vector<FrameItem> &items, bool batched, boo
             if (sp->mm_parts)
                 run_mm_parts(x, sp->ctop, sp->rtop, sp->ltop, dptr(f, 0), dptr(f, 1), dptr(f, 2),
                              sp->d_ra.p, (int)sp->ra.size(), sp->d_la.p, (int)sp->la.size(), mm3);
+            const bool gb = it.method != MMPBSA_METHOD_PB;
+            if (gb && sp->gb_parts)
+                run_gb_parts(x, sp->ctop, sp->rtop, sp->ltop, dptr(f, 0), dptr(f, 1), dptr(f, 2),
+                             sp->d_ra.p, (int)sp->ra.size(), sp->d_la.p, (int)sp->la.size(),
+                             it.gb, &gbv[3 * f]);
             for (int k = 0; k < 3; ++k) {
                 if (!sp->mm_parts) run_mm(x, *tops(f, k), dptr(f, k), &mm3[k]);
                 mmv[3 * f + k] = mm_total(mm3[k]);
-                if (it.method != MMPBSA_METHOD_PB)
+                if (gb && !sp->gb_parts)
                     gbv[3 * f + k] = run_gb(x, *tops(f, k), dptr(f, k), it.gb, nullptr, nullptr);
                 double tsasa = 0.0, tsa = 0.0;
                 run_sa(x, *tops(f, k), dptr(f, k), hptr(f, k), it.sa, nullptr, nullptr, &tsasa,
@@ -860,8 +866,10 @@ int mmpbsa_split_create(mmpbsa_ctx *ctx, const int32_t *receptor_res, int32_t n_
         };
         scan(s->ctop.h_ex_ptr, s->ctop.h_ex_col);
         scan(s->ctop.h_p14_ptr, s->ctop.h_p14_col);
-        bool parts = !s->cross_excl;
-        for (int a = 0; a < ctop.n && parts; ++a) parts = is_r[a] != is_l[a];
+        bool cover = true;
+        for (int a = 0; a < ctop.n && cover; ++a) cover = is_r[a] != is_l[a];
+        s->gb_parts = cover;
+        bool parts = cover && !s->cross_excl;
         auto one_side = [&](const std::vector<int> &terms, int stride, int atoms) {
             for (size_t b = 0; b + stride <= terms.size() && parts; b += stride) {
                 const bool r = is_r[terms[b]];

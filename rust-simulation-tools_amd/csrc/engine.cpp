@@ -101,6 +101,69 @@ void run_mm_parts(Worker &w, const DevTopo &ct, const DevTopo &rt, const DevTopo
 }
 
 // ---- GB ----------------------------------------------------------------------
+// GB energies of the complex, receptor and ligand of one frame (out[0..2]) when
+// the complex is exactly the two partners: each partner's psi sums (over its
+// own atoms) serve both its own Born radii and, plus the sums over the other
+// partner's atoms, the complex's, so the complex's intra-partner pairs are
+// not evaluated a second time.  Every pair term is the one the complex's own
+// pass computes; a complex atom's psi is its partner sum plus its cross sum
+// (the order of the additions differs from the complex's own pass, so radii
+// agree to rounding).
+void run_gb_parts(Worker &w, const DevTopo &ct, const DevTopo &rt, const DevTopo &lt,
+                  const double *dc, const double *dr, const double *dl, const int *d_ra, int nr,
+                  const int *d_la, int nl, const mmpbsa_gb_params &p, double out[3])
+{
+    const PairTopo vc = ct.view(), vr = rt.view(), vl = lt.view();
+    const int cr = gb_psi_chunks(nr), cl = gb_psi_chunks(nl);
+    const int xr = gb_psi_cross_chunks(nr, nl), xl = gb_psi_cross_chunks(nl, nr);
+    DBuf<double> psi_r(w.pool, (size_t)cr * nr + 1), psi_l(w.pool, (size_t)cl * nl + 1);
+    DBuf<double> x_r(w.pool, (size_t)xr * nr + 1), x_l(w.pool, (size_t)xl * nl + 1);
+    DBuf<double> born(w.pool, (size_t)vc.n + nr + nl + 1);
+    double *born_c = born.get(), *born_r = born_c + vc.n, *born_l = born_r + nr;
+    {
+        // 32 FLOP per ordered pair, as run_gb
+        KernelTimer kt(w, "gb_radii", 32.0 * ((double)nr * (nr - 1) + (double)nl * (nl - 1)));
+        launch_gb_radii(dr, vr, p.model, p.offset, p.rgbmax, p.cutoff, psi_r.get(), born_r,
+                        w.stream);
+        launch_gb_radii(dl, vl, p.model, p.offset, p.rgbmax, p.cutoff, psi_l.get(), born_l,
+                        w.stream);
+    }
+    {
+        KernelTimer kt(w, "gb_radii_cross", 64.0 * (double)nr * nl);
+        launch_gb_psi_cross(dc, vc, d_ra, nr, d_la, nl, p.offset, p.cutoff, x_r.get(), w.stream);
+        launch_gb_psi_cross(dc, vc, d_la, nl, d_ra, nr, p.offset, p.cutoff, x_l.get(), w.stream);
+    }
+    launch_gb_finalize_parts(psi_r.get(), cr, x_r.get(), xr, d_ra, nr, vc, p.model, p.offset,
+                             p.rgbmax, born_c, w.stream);
+    launch_gb_finalize_parts(psi_l.get(), cl, x_l.get(), xl, d_la, nl, vc, p.model, p.offset,
+                             p.rgbmax, born_c, w.stream);
+    const double kappa = compute_kappa(p.salt_concentration, p.solvent_dielectric, p.temperature);
+    const PairTopo *views[3] = {&vc, &vr, &vl};
+    const double *xyz[3] = {dc, dr, dl};
+    const double *borns[3] = {born_c, born_r, born_l};
+    int nbk[3];
+    for (int k = 0; k < 3; ++k) nbk[k] = views[k]->n > 0 ? gb_energy_blocks(views[k]->n) : 0;
+    DBuf<double> part(w.pool, (size_t)nbk[0] + nbk[1] + nbk[2] + 1), res(w.pool, 3);
+    double *pk = part.get();
+    for (int k = 0; k < 3; ++k) {
+        const PairTopo &v = *views[k];
+        if (v.n == 0) {
+            launch_fill_f64(res.get() + k, 0.0, 1, w.stream);
+            continue;
+        }
+        {
+            KernelTimer kt(w, "gb_energy_kernel", 13.0 * (double)v.n * (v.n - 1));
+            launch_gb_energy(xyz[k], v, borns[k], kappa, p.solute_dielectric,
+                             p.solvent_dielectric, p.cutoff, pk, w.stream);
+        }
+        launch_sum_partials(pk, nbk[k], 1, 1, res.get() + k, w.stream);
+        pk += nbk[k];
+    }
+    HIP_OK(hipMemcpyAsync(w.pin, res.get(), sizeof(double) * 3, hipMemcpyDeviceToHost, w.stream));
+    w.sync();
+    for (int k = 0; k < 3; ++k) out[k] = w.pin[k];
+}
+
 double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_params &p,
               double *d_born, std::vector<double> *born_out)
 {

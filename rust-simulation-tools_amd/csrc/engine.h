@@ -163,6 +163,12 @@ void run_mm(Worker &w, const DevTopo &t, const double *d_xyz, mmpbsa_mm_terms *o
 void run_mm_parts(Worker &w, const DevTopo &ct, const DevTopo &rt, const DevTopo &lt,
                   const double *dc, const double *dr, const double *dl, const int *d_ra, int nr,
                   const int *d_la, int nl, mmpbsa_mm_terms out[3]);
+// complex, receptor and ligand GB energies of one frame, the complex's Born
+// radii from the partners' psi sums plus the cross sums (a split whose complex
+// is exactly the two partners)
+void run_gb_parts(Worker &w, const DevTopo &ct, const DevTopo &rt, const DevTopo &lt,
+                  const double *dc, const double *dr, const double *dl, const int *d_ra, int nr,
+                  const int *d_la, int nl, const mmpbsa_gb_params &p, double out[3]);
 double run_gb(Worker &w, const DevTopo &t, const double *d_xyz, const mmpbsa_gb_params &p,
               double *d_born /* nullable: pooled scratch used */, std::vector<double> *born_out);
 double run_gb_from_radii(Worker &w, const DevTopo &t, const double *d_xyz,

@@ -41,6 +41,19 @@ void set_gb_chunks_override(int c);  // 0 restores the heuristic
 void launch_gb_radii(const double *xyz, const PairTopo &t, int model, double offset,
                      double rgbmax, double cutoff, double *psi_work, double *born_out,
                      hipStream_t s);
+// The chunks of psi partials launch_gb_radii writes for n atoms.
+int gb_psi_chunks(int n);
+// Complex Born radii from its partners: psi partials of the atoms of A (complex
+// indices) from the atoms of B, [gb_psi_cross_chunks(nA, nB)][nA]; then the
+// radii of one partner's atoms (map: their complex indices) from the partner's
+// own psi partials plus those, written at the complex indices.
+int gb_psi_cross_chunks(int nA, int nB);
+void launch_gb_psi_cross(const double *xyz, const PairTopo &t, const int *A, int nA, const int *B,
+                         int nB, double offset, double cutoff, double *part, hipStream_t s);
+void launch_gb_finalize_parts(const double *own, int own_chunks, const double *cross,
+                              int cross_chunks, const int *map, int n, const PairTopo &t,
+                              int model, double offset, double rgbmax, double *born,
+                              hipStream_t s);
 // GB energy: writes partials; final sum by launch_sum_partials.
 int gb_energy_blocks(int n);
 void launch_gb_energy(const double *xyz, const PairTopo &t, const double *born, double kappa,

@@ -94,15 +94,10 @@ __global__ __launch_bounds__(kTile) void gb_psi_kernel(const double *__restrict_
 }
 
 // OBC / HCT radius map, gb_energy.rs:219-255.
-__global__ void gb_finalize_kernel(const double *__restrict__ part, int chunks, int n,
-                                   const double *__restrict__ radii, double offset, int model,
-                                   double rgbmax, double *__restrict__ born)
+__device__ __forceinline__ double born_radius(double psi, double radius, double offset, int model,
+                                              double rgbmax)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double psi = 0.0;
-    for (int c = 0; c < chunks; ++c) psi += part[(size_t)c * n + i];
-    const double rho = radii[i] - offset;
+    const double rho = radius - offset;
     double inv;
     if (model == 0) {
         inv = 1.0 / rho - psi;
@@ -114,11 +109,88 @@ __global__ void gb_finalize_kernel(const double *__restrict__ part, int chunks, 
         const double p2 = pr * pr;
         const double p3 = p2 * pr;
         const double th = tanh(a * pr - b * p2 + g * p3);
-        inv = 1.0 / rho - th / radii[i];
+        inv = 1.0 / rho - th / radius;
     }
     double R = inv > 0.0 ? 1.0 / inv : rgbmax;
     if (R > rgbmax) R = rgbmax;
-    born[i] = R;
+    return R;
+}
+
+__global__ void gb_finalize_kernel(const double *__restrict__ part, int chunks, int n,
+                                   const double *__restrict__ radii, double offset, int model,
+                                   double rgbmax, double *__restrict__ born)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double psi = 0.0;
+    for (int c = 0; c < chunks; ++c) psi += part[(size_t)c * n + i];
+    born[i] = born_radius(psi, radii[i], offset, model, rgbmax);
+}
+
+// psi of the atoms of A from the atoms of B (complex indices; the partners of
+// a split are disjoint, so no pair is an atom with itself), j-chunked as
+// gb_psi_kernel: part[chunk][ia]
+__global__ __launch_bounds__(kTile) void gb_psi_cross_kernel(
+    const double *__restrict__ xyz, const double *__restrict__ radii,
+    const double *__restrict__ screen, const int *__restrict__ A, int nA,
+    const int *__restrict__ B, int nB, double offset, double cut2, int chunk,
+    double *__restrict__ part)
+{
+    __shared__ double sx[kTile], sy[kTile], sz[kTile], ss[kTile];
+    const int t = threadIdx.x;
+    const int ia = blockIdx.x * kTile + t;
+    const int j0 = blockIdx.y * chunk;
+    const int j1 = min(nB, j0 + chunk);
+    const bool act = ia < nA;
+    double xi = 0, yi = 0, zi = 0, rho_i = 1;
+    if (act) {
+        const int i = A[ia];
+        xi = xyz[3 * i];
+        yi = xyz[3 * i + 1];
+        zi = xyz[3 * i + 2];
+        rho_i = radii[i] - offset;
+    }
+    double acc = 0.0;
+    for (int jt = j0; jt < j1; jt += kTile) {
+        const int jj = jt + t;
+        if (jj < j1) {
+            const int j = B[jj];
+            sx[t] = xyz[3 * j];
+            sy[t] = xyz[3 * j + 1];
+            sz[t] = xyz[3 * j + 2];
+            ss[t] = screen[j] * (radii[j] - offset);
+        }
+        __syncthreads();
+        const int m = min(kTile, j1 - jt);
+        if (act) {
+            for (int k = 0; k < m; ++k) {
+                const double dx = xi - sx[k], dy = yi - sy[k], dz = zi - sz[k];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > cut2) continue;
+                acc += hct_term(r2, rho_i, ss[k]);
+            }
+        }
+        __syncthreads();
+    }
+    if (act) part[(size_t)blockIdx.y * nA + ia] = acc;
+}
+
+// Born radii of one partner's atoms in the complex: psi from the partner's
+// own partials plus the cross partials, atom map[p] of the complex
+__global__ void gb_finalize_parts_kernel(const double *__restrict__ own, int own_chunks,
+                                         const double *__restrict__ cross, int cross_chunks,
+                                         const int *__restrict__ map, int n,
+                                         const double *__restrict__ radii, double offset,
+                                         int model, double rgbmax, double *__restrict__ born)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double psi = 0.0;
+    for (int c = 0; c < own_chunks; ++c) psi += own[(size_t)c * n + p];
+    double x = 0.0;
+    for (int c = 0; c < cross_chunks; ++c) x += cross[(size_t)c * n + p];
+    const int a = map[p];
+    born[a] = born_radius(psi + x, radii[a], offset, model, rgbmax);
 }
 
 // j-chunks of the Born-radius kernel: enough workgroups to fill the GPU
@@ -133,6 +205,13 @@ int gb_chunks(int n)
     int c = o > 0 ? std::min(o, 64) : std::max(1, std::min(64, ceil_div(4096, nb)));
     c = std::min(c, std::max(1, ceil_div(n, kTile)));
     return c;
+}
+
+int gb_psi_chunks(int n)
+{
+    if (n <= 0) return 0;
+    const int chunk = ceil_div(ceil_div(n, gb_chunks(n)), kTile) * kTile;
+    return ceil_div(n, chunk);
 }
 
 void launch_gb_radii(const double *xyz, const PairTopo &t, int model, double offset,
@@ -241,6 +320,42 @@ __global__ __launch_bounds__(kTile) void gb_energy_kernel(
         }
     }
     block_store<kTile>(acc, red, outp);
+}
+
+// j-chunks of the cross kernel: about 4096 workgroups, at most one per tile of B
+static int cross_chunk_len(int nA, int nB)
+{
+    const int nb = std::max(1, ceil_div(nA, kTile));
+    const int c = std::max(1, std::min(ceil_div(4096, nb), ceil_div(nB, kTile)));
+    return ceil_div(ceil_div(nB, c), kTile) * kTile;
+}
+
+int gb_psi_cross_chunks(int nA, int nB)
+{
+    return nA > 0 && nB > 0 ? ceil_div(nB, cross_chunk_len(nA, nB)) : 0;
+}
+
+void launch_gb_psi_cross(const double *xyz, const PairTopo &t, const int *A, int nA, const int *B,
+                         int nB, double offset, double cutoff, double *part, hipStream_t s)
+{
+    if (nA <= 0 || nB <= 0) return;
+    const int chunk = cross_chunk_len(nA, nB);
+    hipLaunchKernelGGL(gb_psi_cross_kernel, dim3(ceil_div(nA, kTile), ceil_div(nB, chunk)),
+                       dim3(kTile), 0, s, xyz, t.radii, t.screen, A, nA, B, nB, offset,
+                       cutoff * cutoff, chunk, part);
+    LAUNCH_OK();
+}
+
+void launch_gb_finalize_parts(const double *own, int own_chunks, const double *cross,
+                              int cross_chunks, const int *map, int n, const PairTopo &t,
+                              int model, double offset, double rgbmax, double *born,
+                              hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(gb_finalize_parts_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, own,
+                       own_chunks, cross, cross_chunks, map, n, t.radii, offset, model, rgbmax,
+                       born);
+    LAUNCH_OK();
 }
 
 int gb_energy_blocks(int n)

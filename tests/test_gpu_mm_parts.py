@@ -1,11 +1,14 @@
-"""Frame MM terms when the complex is exactly the two partners: the complex's
-pair terms come from the receptor's and ligand's own passes plus the
-receptor x ligand pairs, so its intra-partner pairs are evaluated once.  Each
-subsystem's frame term must equal that subsystem's own full pass
+"""Frame MM and GB terms when the complex is exactly the two partners: the
+complex's pair terms come from the receptor's and ligand's own passes plus the
+receptor x ligand pairs, and its Born radii from the partners' psi sums plus
+the cross sums, so its intra-partner pairs are evaluated once.  Each
+subsystem's frame MM term must equal that subsystem's own full pass
 (compute_mm_energy on the extracted topology, every pair of it summed
 directly) to a few FP64 ulps, also on a frame whose MM total is ~1e14
-kcal/mol.  A split with a bond between the partners must take the full pass,
-which a sum of the partners would miss the bond of."""
+kcal/mol; each GB term must equal compute_gb_energy on the extracted topology
+to rounding (a complex atom's psi adds the same pair terms in another order).
+A split with a bond between the partners must take the full MM pass, which a
+sum of the partners would miss the bond of."""
 import numpy as np
 import pytest
 
@@ -63,3 +66,28 @@ def test_bond_across_partners_takes_full_pass(cfg0):
     b = np.asarray(top.bonds)
     assert any(res_of[i] == 9 and res_of[j] == 10 for i, j, _ in b)
     _check(top, np.asarray(x) / 10.0, list(range(10)), [10], ulps=4)
+
+
+@pytest.mark.parametrize("model", [0, 1, 2])
+def test_gb_from_partner_psi(cfg0, model):
+    top, x, rec, lig, _ = cfg0
+    x_nm = np.asarray(x) / 10.0
+    gb = rst.GbParams(model=model)
+    f = rst.compute_binding_energy_single_frame(top, x_nm, rec, lig, gb_params=gb)
+    ra, la = top.build_selection(rec), top.build_selection(lig)
+    ca = np.sort(np.concatenate([ra, la]))
+    for name, got, sel in (("complex", f.complex_polar, ca), ("receptor", f.receptor_polar, ra),
+                           ("ligand", f.ligand_polar, la)):
+        want = rst.compute_gb_energy(extract_subtopology(top, sel), x_nm[sel], gb).total
+        print(f"model {model} {name}: frame {got!r} direct {want!r}")
+        assert abs(got - want) <= 1e-10 * abs(want), name
+
+
+def test_gb_from_partner_psi_with_cutoff(cfg0):
+    top, x, rec, lig, _ = cfg0
+    x_nm = np.asarray(x) / 10.0
+    gb = rst.GbParams(model=2, cutoff=12.0, salt_concentration=0.0)
+    f = rst.compute_binding_energy_single_frame(top, x_nm, rec, lig, gb_params=gb)
+    ca = np.sort(np.concatenate([top.build_selection(rec), top.build_selection(lig)]))
+    want = rst.compute_gb_energy(extract_subtopology(top, ca), x_nm[ca], gb).total
+    assert abs(f.complex_polar - want) <= 1e-10 * abs(want)
