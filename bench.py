@@ -619,7 +619,7 @@ def main():
         step(args.warmup + args.steps)
         ms, n, by, gt = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
         lib.mmpbsa_profile_read(C.byref(ms), C.byref(n), C.byref(by), C.byref(gt))
-        kr = kernel_rooflines(lib, {"mm_pair_kernel": "fp64", "faces_kernel": "hbm",
+        kr = kernel_rooflines(lib, {"mm_pair_kernel": "fp64", "face_dielectrics": "hbm",
                                     "dh_boundary_kernel": "fp32"})
         lib.mmpbsa_profile_enable(0)
         # the same frames by MM-GBSA (OBC-II): batched throughput, then one

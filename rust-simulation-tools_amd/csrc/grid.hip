@@ -691,6 +691,278 @@ __global__ void faces_kernel(GridDesc g, const uint8_t *__restrict__ inside,
     }
 }
 
+// The same faces in two launches.  faces_split_kernel writes the faces whose
+// two nodes lie on one side and lists the mixed edges ((node << 2) | axis; one
+// atomic per workgroup, so the list's order varies, but each entry's result
+// does not).  faces_mixed_kernel gives each listed edge a wavefront: lanes test
+// the candidate atoms side by side (a row of cells at a time: the atoms of
+// consecutive x cells are contiguous in the cell list) and append the clipped
+// intervals to a wave-local LDS list in the atom scan order of edge_fraction;
+// a stable rank by start puts them in edge_fraction's insertion order, and the
+// merge runs in that order on wave-uniform values, so every face is bitwise
+// the one-thread-per-node kernel's.
+constexpr int kSplitBlock = 1024;
+
+// LDS written by some lanes of a wavefront, then read by others
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// lane l's value (l wave-uniform)
+__device__ inline double readlane_f64(double v, int l)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__global__ __launch_bounds__(kSplitBlock) void faces_split_kernel(
+    GridDesc g, const uint8_t *__restrict__ inside, double ein, double eout,
+    double *__restrict__ ex, double *__restrict__ ey, double *__restrict__ ez,
+    float *__restrict__ fx, float *__restrict__ fy, float *__restrict__ fz,
+    uint64_t *__restrict__ list, int cap, int *__restrict__ count)
+{
+    __shared__ int wcnt[kSplitBlock / 64];
+    __shared__ int bbase;
+    const int64_t N = g.size();
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = idx < N;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int pos[3] = {0, 0, 0};
+    if (in) node_of(g, idx, pos[0], pos[1], pos[2]);
+    const uint8_t a = in ? inside[idx] : 0;
+    const int lim[3] = {g.nx, g.ny, g.nz};
+    const int64_t step[3] = {1, (int64_t)g.nx, (int64_t)g.nx * g.ny};
+    double *dst[3] = {ex, ey, ez};
+    float *dstf[3] = {fx, fy, fz};
+    uint64_t bm[3];
+    int wtot = 0;
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        bool mixed = false;
+        double e = eout;
+        if (in && pos[axis] + 1 < lim[axis]) {
+            const uint8_t b = inside[idx + step[axis]];
+            if (a == b)
+                e = a ? ein : eout;
+            else
+                mixed = true;
+        }
+        if (in && !mixed) {
+            dst[axis][idx] = e;
+            dstf[axis][idx] = (float)e;
+        }
+        bm[axis] = __ballot(mixed);
+        wtot += __popcll(bm[axis]);
+    }
+    if (lane == 0) wcnt[wave] = wtot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < kSplitBlock / 64; ++k) t += wcnt[k];
+        bbase = t ? atomicAdd(count, t) : 0;
+    }
+    __syncthreads();
+    int base = bbase;
+    for (int k = 0; k < wave; ++k) base += wcnt[k];
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        const uint64_t m = bm[axis];
+        if ((m >> lane) & 1u) {
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            const int slot = base + (int)below;
+            if (slot < cap) list[slot] = ((uint64_t)idx << 2) | (uint64_t)axis;
+        }
+        base += __popcll(m);
+    }
+}
+
+__global__ __launch_bounds__(256) void faces_mixed_kernel(
+    GridDesc g, const double *__restrict__ xyz, const double *__restrict__ radii, double max_r,
+    CellList cl, double ein, double eout, const uint64_t *__restrict__ list,
+    const int *__restrict__ count, int cap, double *__restrict__ ex, double *__restrict__ ey,
+    double *__restrict__ ez, float *__restrict__ fx, float *__restrict__ fy,
+    float *__restrict__ fz, int *overflow)
+{
+    __shared__ double ia_s[4][kMaxIv], ib_s[4][kMaxIv];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    double *ia = ia_s[wave], *ib = ib_s[wave];
+    const int n = min(*count, cap);
+    double *dst[3] = {ex, ey, ez};
+    float *dstf[3] = {fx, fy, fz};
+    for (int q = blockIdx.x * 4 + wave; q < n; q += gridDim.x * 4) {
+        const uint64_t ent = list[q];
+        const int64_t idx = (int64_t)(ent >> 2);
+        const int axis = (int)(ent & 3u);
+        int x, y, z;
+        node_of(g, idx, x, y, z);
+        const double pt[3] = {g.ox + (double)x * g.h, g.oy + (double)y * g.h,
+                              g.oz + (double)z * g.h};
+        // edge_fraction (lo, hi, p1, p2) of this edge, atoms tested by lanes
+        const double lo = pt[axis], hi = pt[axis] + g.h;
+        const double p1 = axis == 0 ? pt[1] : pt[0];
+        const double p2 = axis == 2 ? pt[1] : pt[2];
+        const double len = hi - lo;
+        double f = 0.0;
+        if (len > 0.0) {
+            const int a1 = axis == 0 ? 1 : 0;
+            const int a2 = axis == 2 ? 1 : 2;
+            double mid[3];
+            mid[axis] = 0.5 * (lo + hi);
+            mid[a1] = p1;
+            mid[a2] = p2;
+            const double w = max_r + 0.5 * len + 1e-9;
+            const int cx0 = max(0, (int)floor((mid[0] - w - cl.ox) * cl.inv));
+            const int cx1 = min(cl.dx - 1, (int)floor((mid[0] + w - cl.ox) * cl.inv));
+            const int cy0 = max(0, (int)floor((mid[1] - w - cl.oy) * cl.inv));
+            const int cy1 = min(cl.dy - 1, (int)floor((mid[1] + w - cl.oy) * cl.inv));
+            const int cz0 = max(0, (int)floor((mid[2] - w - cl.oz) * cl.inv));
+            const int cz1 = min(cl.dz - 1, (int)floor((mid[2] + w - cl.oz) * cl.inv));
+            // the candidate cells form rows of consecutive x cells, and a
+            // row's atoms are one contiguous range of the cell list: lanes
+            // take the rows' ranges (cz outer, cy inner, as edge_fraction
+            // scans them), then the atoms of the ranges laid end to end
+            const int ncy = cy1 - cy0 + 1, ncz = cz1 - cz0 + 1;
+            const int nr = (cx0 <= cx1 && ncy > 0 && ncz > 0) ? ncy * ncz : 0;
+            int m = 0;  // intervals found (wave-uniform)
+            for (int r0 = 0; r0 < nr; r0 += 64) {
+                const int r = r0 + lane;
+                int kb = 0, kn = 0;
+                if (r < nr) {
+                    const int cy = cy0 + r % ncy, cz = cz0 + r / ncy;
+                    const int crow = cl.dx * (cy + cl.dy * cz);
+                    kb = cl.start[crow + cx0];
+                    kn = cl.start[crow + cx1 + 1] - kb;
+                }
+                int incl = kn;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int t = __shfl_up(incl, d, 64);
+                    if (lane >= d) incl += t;
+                }
+                const int excl = incl - kn;
+                const int total = __builtin_amdgcn_readlane(incl, 63);
+                const int rows = min(64, nr - r0);
+                for (int j0 = 0; j0 < total; j0 += 64) {
+                    const int j = j0 + lane;
+                    // the last row starting at or before j holds it
+                    int k = 0;
+                    for (int q = 0; q < rows; ++q) {
+                        const int eq = __builtin_amdgcn_readlane(excl, q);
+                        const int bq = __builtin_amdgcn_readlane(kb, q);
+                        if (j >= eq) k = bq + (j - eq);
+                    }
+                    bool hit = false;
+                    double ilo = 0.0, ihi = 0.0;
+                    if (j < total) {
+                        const int at = cl.atoms[k];
+                        const double R = radii[at] + 0.0;
+                        const double d1 = p1 - xyz[3 * at + a1];
+                        const double d2 = p2 - xyz[3 * at + a2];
+                        const double rs = R * R - d1 * d1 - d2 * d2;
+                        if (rs > 0.0) {
+                            const double rho = sqrt(rs);
+                            const double ca = xyz[3 * at + axis];
+                            ilo = fmax(ca - rho, lo);
+                            ihi = fmin(ca + rho, hi);
+                            hit = ilo < ihi;
+                        }
+                    }
+                    const uint64_t hm = __ballot(hit);
+                    if (hit) {
+                        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u));
+                        const int slot = m + (int)below;
+                        if (slot < kMaxIv) {
+                            ia[slot] = ilo;
+                            ib[slot] = ihi;
+                        }
+                    }
+                    m += __popcll(hm);
+                }
+            }
+            if (m > kMaxIv) {
+                if (lane == 0) atomicExch(overflow, 1);
+                m = kMaxIv;
+            }
+            wave_lds_sync();
+            // lane p holds interval p; its stable rank by start is its place
+            // in edge_fraction's insertion order (the first kMaxIv of the scan)
+            double sa = 0.0, sb = 0.0;
+            if (lane < m) {
+                sa = ia[lane];
+                sb = ib[lane];
+            }
+            int rank = 0;
+            for (int q = 0; q < m; ++q) {
+                const double aq = readlane_f64(sa, q);
+                rank += (aq < sa || (aq == sa && q < lane)) ? 1 : 0;
+            }
+            wave_lds_sync();
+            if (lane < m) {
+                ia[rank] = sa;
+                ib[rank] = sb;
+            }
+            wave_lds_sync();
+            if (lane < m) {
+                sa = ia[lane];
+                sb = ib[lane];
+            }
+            if (m > 0) {
+                // the merge of edge_fraction, in the same order, on uniform values
+                double tot = 0.0, c_lo = readlane_f64(sa, 0), c_hi = readlane_f64(sb, 0);
+                for (int k = 1; k < m; ++k) {
+                    const double ak = readlane_f64(sa, k), bk = readlane_f64(sb, k);
+                    if (ak <= c_hi) {
+                        c_hi = fmax(c_hi, bk);
+                    } else {
+                        tot += c_hi - c_lo;
+                        c_lo = ak;
+                        c_hi = bk;
+                    }
+                }
+                tot += c_hi - c_lo;
+                f = tot / len;
+                f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+            }
+        }
+        if (lane == 0) {
+            double e;
+            if (f <= 0.0)
+                e = eout;
+            else if (f >= 1.0)
+                e = ein;
+            else
+                e = 1.0 / (f / ein + (1.0 - f) / eout);
+            dst[axis][idx] = e;
+            dstf[axis][idx] = (float)e;
+        }
+        wave_lds_sync();
+    }
+}
+
+int faces_list_cap(int64_t n) { return (int)std::min<int64_t>(n / 4 + 1024, (int64_t)1 << 30); }
+
+void launch_faces_split(const GridDesc &g, const uint8_t *inside, const double *xyz,
+                        const double *radii, double max_r, const CellList &cl, double eps_in,
+                        double eps_out, double *ex, double *ey, double *ez, float *fx, float *fy,
+                        float *fz, uint64_t *list, int cap, int *count, int *overflow,
+                        hipStream_t s)
+{
+    const int64_t N = g.size();
+    hipLaunchKernelGGL(faces_split_kernel, dim3(ceil_div(N, kSplitBlock)), dim3(kSplitBlock), 0, s, g, inside,
+                       eps_in, eps_out, ex, ey, ez, fx, fy, fz, list, cap, count);
+    LAUNCH_OK();
+    hipLaunchKernelGGL(faces_mixed_kernel, dim3(2048), dim3(256), 0, s, g, xyz, radii, max_r, cl,
+                       eps_in, eps_out, list, count, cap, ex, ey, ez, fx, fy, fz, overflow);
+    LAUNCH_OK();
+}
+
 void launch_faces(const GridDesc &g, const uint8_t *inside, const double *xyz,
                   const double *radii, double max_r, const CellList &cl, double eps_in,
                   double eps_out, double *ex, double *ey, double *ez, float *fx, float *fy,

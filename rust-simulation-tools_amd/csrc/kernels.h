@@ -144,6 +144,16 @@ struct FloodJob {
 void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d_flags,
                       hipStream_t s);
 // Face dielectrics (FP64 + FP32 copies) from the SES map.
+// The same faces in two launches (trivial faces per node, then a wavefront per
+// mixed edge): list: cap entries (faces_list_cap(N) is ample); count: one int,
+// zeroed by the caller, the mixed edges found (above cap: the faces are
+// incomplete, run launch_faces).
+int faces_list_cap(int64_t n);
+void launch_faces_split(const GridDesc &g, const uint8_t *inside, const double *xyz,
+                        const double *radii, double max_r, const CellList &cl, double eps_in,
+                        double eps_out, double *ex, double *ey, double *ez, float *fx, float *fy,
+                        float *fz, uint64_t *list, int cap, int *count, int *overflow,
+                        hipStream_t s);
 void launch_faces(const GridDesc &g, const uint8_t *inside, const double *xyz,
                   const double *radii, double max_r, const CellList &cl, double eps_in,
                   double eps_out, double *ex, double *ey, double *ez, float *fx, float *fy,

@@ -324,24 +324,60 @@ static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
         launch_ses_batch(ses, table.get(), flags.get(), w.stream);
     }
     if (!faces.empty()) {
-        DBuf<int> ovf(w.pool, faces.size());
-        launch_fill_i32(ovf.get(), 0, (int64_t)faces.size(), w.stream);
-        for (size_t f = 0; f < faces.size(); ++f) {
+        // per map: the interval-overflow flag, then the mixed-edge count; one
+        // mixed-edge list serves the maps in turn (stream order).
+        // MMPBSA_FACES_LIST_CAP lowers the list's capacity (a test of the
+        // fallback below).
+        const size_t nf = faces.size();
+        int64_t nmax = 0;
+        for (size_t f = 0; f < nf; ++f) nmax = std::max(nmax, rq[faces[f]].g.size());
+        int cap_env = -1;
+        if (const char *e = std::getenv("MMPBSA_FACES_LIST_CAP")) cap_env = std::max(0, std::atoi(e));
+        auto cap_of = [&](int64_t n) {
+            const int c = faces_list_cap(n);
+            return cap_env >= 0 ? std::min(c, cap_env) : c;
+        };
+        DBuf<int> ovf(w.pool, 2 * nf);
+        DBuf<uint64_t> list(w.pool, (size_t)std::max(1, cap_of(nmax)));
+        launch_fill_i32(ovf.get(), 0, 2 * (int64_t)nf, w.stream);
+        auto issue = [&](size_t f, bool split) {
             MapReq &r = rq[faces[f]];
             MapSet &m = *r.m;
             // algorithmic bytes: the SES byte read, three FP64 and three FP32
             // face values written per node
-            KernelTimer kt(w, "faces_kernel", 37.0 * (double)r.g.size());
-            launch_faces(r.g, tmp[faces[f]].inside.get(), r.d_xyz, r.t->radii.p, r.t->max_radius,
-                         r.c->cl, r.ein, r.eout, m.ex.get(), m.ey.get(), m.ez.get(), m.fx.get(),
-                         m.fy.get(), m.fz.get(), ovf.get() + f, w.stream);
-        }
-        std::vector<int> ho(faces.size());
-        HIP_OK(hipMemcpyAsync(ho.data(), ovf.get(), sizeof(int) * ho.size(),
-                              hipMemcpyDeviceToHost, w.stream));
-        w.sync();
-        for (int v : ho)
-            if (v) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "face interval list overflow");
+            KernelTimer kt(w, "face_dielectrics", 37.0 * (double)r.g.size());
+            if (split)
+                launch_faces_split(r.g, tmp[faces[f]].inside.get(), r.d_xyz, r.t->radii.p,
+                                   r.t->max_radius, r.c->cl, r.ein, r.eout, m.ex.get(),
+                                   m.ey.get(), m.ez.get(), m.fx.get(), m.fy.get(), m.fz.get(),
+                                   list.get(), cap_of(r.g.size()), ovf.get() + nf + f,
+                                   ovf.get() + f, w.stream);
+            else
+                launch_faces(r.g, tmp[faces[f]].inside.get(), r.d_xyz, r.t->radii.p,
+                             r.t->max_radius, r.c->cl, r.ein, r.eout, m.ex.get(), m.ey.get(),
+                             m.ez.get(), m.fx.get(), m.fy.get(), m.fz.get(), ovf.get() + f,
+                             w.stream);
+        };
+        for (size_t f = 0; f < nf; ++f) issue(f, true);
+        std::vector<int> ho(2 * nf);
+        auto fetch = [&]() {
+            HIP_OK(hipMemcpyAsync(ho.data(), ovf.get(), sizeof(int) * ho.size(),
+                                  hipMemcpyDeviceToHost, w.stream));
+            w.sync();
+        };
+        fetch();
+        // a map with more mixed edges than its list holds takes the
+        // one-thread-per-node pass over every face
+        bool again = false;
+        for (size_t f = 0; f < nf; ++f)
+            if (ho[nf + f] > cap_of(rq[faces[f]].g.size())) {
+                launch_fill_i32(ovf.get() + f, 0, 1, w.stream);
+                issue(f, false);
+                again = true;
+            }
+        if (again) fetch();
+        for (size_t f = 0; f < nf; ++f)
+            if (ho[f]) throw DeviceError(MMPBSA_ERR_TOO_LARGE, "face interval list overflow");
     }
     for (size_t k = 0; k < rq.size(); ++k) {
         MapReq &r = rq[k];
