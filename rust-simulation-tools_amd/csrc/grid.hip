@@ -501,23 +501,268 @@ __global__ void flood_round_end_kernel(const FloodJob *__restrict__ job)
     }
 }
 
+// ---- the same flood on bits ------------------------------------------------
+// Rows of W = ceil(nx / 32) words along x (padding bits closed).  A sweep along
+// x runs fill_up / fill_down word by word with the carry between words; a
+// sweep along y or z gives a wavefront to each column of words: lane l holds
+// K = ceil(len / 64) consecutive words, and the line's forward fill
+// F_j = e_j | (o_j & F_(j-1)) is a scan of the maps c -> A | (P & c) (A, P bit
+// vectors: a word's map is (e_j, o_j)), composed within the lane and then
+// across lanes in log2(64) shuffle steps; the backward fill is the same scan
+// from the line's end.  32 lines of the grid go through each word at once.
+// The fixpoint (open nodes connected to an open boundary node) is the byte
+// flood's.
+constexpr int kFloodMaxK = 8;  // y and z lines of up to 512 nodes
+
+int64_t ses_bits_words(const std::vector<SesJob> &jobs)
+{
+    int64_t t = 0;
+    for (const SesJob &J : jobs) {
+        if (J.g.ny > 64 * kFloodMaxK || J.g.nz > 64 * kFloodMaxK) return 0;
+        t += 2 * (int64_t)((J.g.nx + 31) / 32) * J.g.ny * J.g.nz;
+    }
+    return t;
+}
+
+// open and exterior-seed bits of map blockIdx.y: one thread per bit of the
+// padded rows, a wavefront's 64 bits are two whole words
+__global__ __launch_bounds__(256) void flood_pack_kernel(const FloodJob *__restrict__ jobs)
+{
+    const FloodJob J = jobs[blockIdx.y];
+    const GridDesc g = J.g;
+    const int64_t rowbits = (int64_t)J.W * 32;
+    const int64_t total = rowbits * g.ny * g.nz;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)blockIdx.x * blockDim.x >= total) return;  // whole workgroup past the map
+    bool o = false, e = false;
+    if (v < total) {
+        const int64_t row = v / rowbits;
+        const int x = (int)(v - row * rowbits);
+        if (x < g.nx) {
+            const int y = (int)(row % g.ny), z = (int)(row / g.ny);
+            o = J.sas[row * g.nx + x] == 0;
+            e = o && (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 ||
+                      z == g.nz - 1);
+        }
+    }
+    const uint64_t bo = __ballot(o), be = __ballot(e);
+    const int lane = threadIdx.x & 63;
+    if (v < total && (lane & 31) == 0) {
+        J.open[v >> 5] = (uint32_t)(bo >> lane);
+        J.extb[v >> 5] = (uint32_t)(be >> lane);
+    }
+}
+
+// x sweeps: one thread per row of map blockIdx.y
+__global__ __launch_bounds__(256) void flood_bits_x_kernel(const FloodJob *__restrict__ jobs)
+{
+    const FloodJob J = jobs[blockIdx.y];
+    if (*J.active == 0) return;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (int64_t)J.g.ny * J.g.nz) return;
+    const int W = J.W;
+    const uint32_t *__restrict__ op = J.open + r * W;
+    uint32_t *__restrict__ ex = J.extb + r * W;
+    bool ch = false;
+    uint32_t c = 0u;
+    for (int w = 0; w < W; ++w) {
+        const uint32_t o = op[w], e = ex[w];
+        const uint32_t F = fill_up(e | (c & o), o);
+        if (F != e) {
+            ex[w] = F;
+            ch = true;
+        }
+        c = F >> 31;
+    }
+    c = 0u;
+    for (int w = W - 1; w >= 0; --w) {
+        const uint32_t o = op[w], e = ex[w];
+        const uint32_t B = fill_down(e | ((c << 31) & o), o);
+        if (B != e) {
+            ex[w] = B;
+            ch = true;
+        }
+        c = B & 1u;
+    }
+    if (ch) *J.changed = 1;
+}
+
+// y (axis 1) or z (axis 2) sweeps: one wavefront per column of words of map
+// blockIdx.y
+__global__ __launch_bounds__(256) void flood_bits_yz_kernel(const FloodJob *__restrict__ jobs,
+                                                            int axis)
+{
+    const FloodJob J = jobs[blockIdx.y];
+    if (*J.active == 0) return;
+    const GridDesc g = J.g;
+    const int W = J.W;
+    const int lane = threadIdx.x & 63;
+    const int64_t col = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    int64_t base, stride;
+    int len;
+    if (axis == 1) {  // column (w, z)
+        if (col >= (int64_t)W * g.nz) return;
+        const int64_t z = col / W, w = col - z * W;
+        base = z * g.ny * W + w;
+        stride = W;
+        len = g.ny;
+    } else {  // column (w, y)
+        if (col >= (int64_t)W * g.ny) return;
+        base = col;
+        stride = (int64_t)W * g.ny;
+        len = g.nz;
+    }
+    const int K = (len + 63) / 64;
+    uint32_t o[kFloodMaxK], e[kFloodMaxK];
+#pragma unroll
+    for (int k = 0; k < kFloodMaxK; ++k) {
+        const int j = lane * K + k;
+        const bool ok = k < K && j < len;
+        o[k] = ok ? J.open[base + j * stride] : 0u;
+        e[k] = ok ? J.extb[base + j * stride] : 0u;
+    }
+    // forward: the lane's map, then the scan across lanes
+    uint32_t A = 0u, P = ~0u;
+#pragma unroll
+    for (int k = 0; k < kFloodMaxK; ++k) {
+        if (k < K) {
+            A = e[k] | (o[k] & A);
+            P &= o[k];
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t Ap = __shfl_up(A, d, 64), Pp = __shfl_up(P, d, 64);
+        if (lane >= d) {
+            A |= P & Ap;
+            P &= Pp;
+        }
+    }
+    uint32_t c = __shfl_up(A, 1, 64);
+    if (lane == 0) c = 0u;
+    uint32_t F[kFloodMaxK];
+#pragma unroll
+    for (int k = 0; k < kFloodMaxK; ++k) {
+        F[k] = e[k] | (o[k] & c);
+        if (k < K) c = F[k];
+    }
+    // backward on the forward result, from the line's end
+    A = 0u;
+    P = ~0u;
+#pragma unroll
+    for (int k = kFloodMaxK - 1; k >= 0; --k) {
+        if (k < K) {
+            A = F[k] | (o[k] & A);
+            P &= o[k];
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t An = __shfl_down(A, d, 64), Pn = __shfl_down(P, d, 64);
+        if (lane + d < 64) {
+            A |= P & An;
+            P &= Pn;
+        }
+    }
+    c = __shfl_down(A, 1, 64);
+    if (lane == 63) c = 0u;
+    bool ch = false;
+#pragma unroll
+    for (int k = kFloodMaxK - 1; k >= 0; --k) {
+        if (k < K) {
+            const uint32_t B = F[k] | (o[k] & c);
+            c = B;
+            const int j = lane * K + k;
+            if (j < len && B != e[k]) {
+                J.extb[base + j * stride] = B;
+                ch = true;
+            }
+        }
+    }
+    if (__ballot(ch) != 0 && lane == 0) *J.changed = 1;
+}
+
+// End of a flood round of every map: active = changed, changed = 0.
+__global__ void flood_rounds_end_kernel(const FloodJob *__restrict__ jobs, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    *jobs[k].active = *jobs[k].changed;
+    *jobs[k].changed = 0;
+}
+
+// exterior bytes of map blockIdx.y from its bits
+__global__ __launch_bounds__(256) void flood_unpack_kernel(const FloodJob *__restrict__ jobs)
+{
+    const FloodJob J = jobs[blockIdx.y];
+    const GridDesc g = J.g;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.size()) return;
+    const int64_t row = idx / g.nx;
+    const int x = (int)(idx - row * g.nx);
+    J.ext[idx] = (uint8_t)((J.extb[row * J.W + (x >> 5)] >> (x & 31)) & 1u);
+}
+
 void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d_flags,
-                      hipStream_t s)
+                      uint32_t *d_bits, hipStream_t s)
 {
     const int n = (int)jobs.size();
     if (n == 0) return;
     std::vector<FloodJob> ht(n);
+    int64_t off = 0, max_words = 0, max_rows = 0, max_cols = 0, max_n = 0;
     for (int k = 0; k < n; ++k) {
         const SesJob &J = jobs[k];
         const int64_t N = J.g.size();
-        hipLaunchKernelGGL(ext_init_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, J.g, J.sas,
-                           J.ext);
-        LAUNCH_OK();
-        ht[k] = FloodJob{J.g, J.sas, J.ext, d_flags + k, d_flags + n + k};
+        const int W = (J.g.nx + 31) / 32;
+        ht[k] = FloodJob{J.g, J.sas, J.ext, d_flags + k, d_flags + n + k, W, nullptr, nullptr};
+        if (d_bits) {
+            const int64_t words = (int64_t)W * J.g.ny * J.g.nz;
+            ht[k].open = d_bits + off;
+            ht[k].extb = d_bits + off + words;
+            off += 2 * words;
+            max_words = std::max(max_words, words);
+            max_rows = std::max(max_rows, (int64_t)J.g.ny * J.g.nz);
+            max_cols = std::max(max_cols, (int64_t)W * std::max(J.g.ny, J.g.nz));
+            max_n = std::max(max_n, N);
+        } else {
+            hipLaunchKernelGGL(ext_init_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, J.g,
+                               J.sas, J.ext);
+            LAUNCH_OK();
+        }
     }
     HIP_OK(hipMemcpyAsync(d_table, ht.data(), sizeof(FloodJob) * n, hipMemcpyHostToDevice, s));
     launch_fill_i32(d_flags, 0, n, s);
     launch_fill_i32(d_flags + n, 1, n, s);
+    if (d_bits) {
+        // every map in the same launches; kRoundsBits rounds per host check
+        constexpr int kRoundsBits = 6;
+        hipLaunchKernelGGL(flood_pack_kernel, dim3(ceil_div(32 * max_words, 256), n), dim3(256), 0,
+                           s, d_table);
+        LAUNCH_OK();
+        std::vector<int> act(n, 1);
+        for (int round = 0; round < 100000; round += kRoundsBits) {
+            for (int r = 0; r < kRoundsBits; ++r) {
+                hipLaunchKernelGGL(flood_bits_x_kernel, dim3(ceil_div(max_rows, 256), n),
+                                   dim3(256), 0, s, d_table);
+                LAUNCH_OK();
+                for (int ax = 1; ax < 3; ++ax) {
+                    hipLaunchKernelGGL(flood_bits_yz_kernel, dim3(ceil_div(max_cols, 4), n),
+                                       dim3(256), 0, s, d_table, ax);
+                    LAUNCH_OK();
+                }
+                hipLaunchKernelGGL(flood_rounds_end_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, s,
+                                   d_table, n);
+                LAUNCH_OK();
+            }
+            HIP_OK(hipMemcpyAsync(act.data(), d_flags + n, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                  s));
+            HIP_OK(hipStreamSynchronize(s));
+            if (std::none_of(act.begin(), act.end(), [](int a) { return a != 0; })) break;
+        }
+        hipLaunchKernelGGL(flood_unpack_kernel, dim3(ceil_div(max_n, 256), n), dim3(256), 0, s,
+                           d_table);
+        LAUNCH_OK();
+    }
     // rounds of each map until a round changes nothing.  kRounds rounds of
     // every map still flooding are queued per host check (one check for all
     // of them); a round after a map's fixpoint finds active = 0 and its
@@ -527,6 +772,7 @@ void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d
     constexpr int kRounds = 4;
     std::vector<int> live(n), act(n, 1);
     for (int k = 0; k < n; ++k) live[k] = k;
+    if (d_bits) live.clear();
     for (int round = 0; !live.empty() && round < 100000; round += kRounds) {
         for (int k : live) {
             const GridDesc &g = jobs[k].g;

@@ -125,8 +125,8 @@ void launch_inside(const GridDesc &g, int n, const double *xyz, const double *ra
 void launch_inside2(const GridDesc &g, int n, const double *xyz, const double *radii, double pa,
                     double pb, uint8_t *out_a, uint8_t *out_b, hipStream_t s);
 // SES classification from SAS maps (flood fill of the exterior + bounded
-// distance transform) for many maps: each map's flood rounds (one host check
-// per four rounds), then each map's distance test.
+// distance transform) for many maps: flood rounds of every map (one host check
+// per few rounds), then each map's distance test.
 struct SesJob {
     GridDesc g;
     const uint8_t *sas;
@@ -139,10 +139,19 @@ struct FloodJob {
     uint8_t *ext;
     int *changed;
     int *active;  // 0 once a round changed nothing in this map
+    // bit-packed flood: rows of W words along x, word (w, y, z) at
+    // (z ny + y) W + w; open = outside-SAS nodes, extb = exterior nodes
+    int W;
+    uint32_t *open, *extb;
 };
-// d_table: device room for jobs.size() FloodJobs; d_flags: 2 jobs.size() ints
+// words the bit-packed flood of these jobs needs (open and exterior bits of
+// every map); 0 when a y or z line is longer than it takes (512 nodes)
+int64_t ses_bits_words(const std::vector<SesJob> &jobs);
+// d_table: device room for jobs.size() FloodJobs; d_flags: 2 jobs.size() ints;
+// d_bits: ses_bits_words(jobs) words for the bit-packed flood, or null for
+// the byte flood (one thread group per line of bytes, launches per map)
 void launch_ses_batch(const std::vector<SesJob> &jobs, FloodJob *d_table, int *d_flags,
-                      hipStream_t s);
+                      uint32_t *d_bits, hipStream_t s);
 // Face dielectrics (FP64 + FP32 copies) from the SES map.
 // The same faces in two launches (trivial faces per node, then a wavefront per
 // mixed edge): list: cap entries (faces_list_cap(N) is ample); count: one int,

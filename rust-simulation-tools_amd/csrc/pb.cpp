@@ -321,7 +321,13 @@ static void build_maps_batch(Worker &w, std::vector<MapReq> &rq)
     if (!ses.empty()) {
         DBuf<FloodJob> table(w.pool, ses.size());
         DBuf<int> flags(w.pool, 2 * ses.size());
-        launch_ses_batch(ses, table.get(), flags.get(), w.stream);
+        // MMPBSA_FLOOD=bytes: the byte flood (a test of one against the other)
+        const char *fe = std::getenv("MMPBSA_FLOOD");
+        const int64_t words =
+            (fe && std::strcmp(fe, "bytes") == 0) ? 0 : ses_bits_words(ses);
+        DBuf<uint32_t> bits;
+        if (words > 0) bits = DBuf<uint32_t>(w.pool, (size_t)words);
+        launch_ses_batch(ses, table.get(), flags.get(), bits.get(), w.stream);
     }
     if (!faces.empty()) {
         // per map: the interval-overflow flag, then the mixed-edge count; one
