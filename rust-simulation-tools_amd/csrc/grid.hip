@@ -1353,12 +1353,16 @@ __global__ __launch_bounds__(256) void maxabs_kernel(const double *__restrict__ 
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-__global__ void max_partials_kernel(const double *__restrict__ p, int nb, double *out)
+// one wavefront: lane maxima, then a shuffle tree (a maximum is exact, and
+// fmax from +0 drops NaNs in any order, so the result is order-independent)
+__global__ __launch_bounds__(64) void max_partials_kernel(const double *__restrict__ p, int nb,
+                                                          double *out)
 {
-    if (threadIdx.x != 0) return;
     double m = 0.0;
-    for (int k = 0; k < nb; ++k) m = fmax(m, p[k]);
-    *out = m;
+    for (int k = threadIdx.x; k < nb; k += 64) m = fmax(m, p[k]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if (threadIdx.x == 0) *out = m;
 }
 
 void launch_maxabs(const double *v, int64_t n, double *partials, int *n_blocks, hipStream_t s)
