@@ -404,6 +404,18 @@ int mmpbsa_binding_stats_compute(const mmpbsa_frame_terms *frames, int32_t n,
 int mmpbsa_interaction_entropy(const mmpbsa_frame_terms *frames, int32_t n, double temperature,
                                double *minus_tds);
 
+/* quasi_harmonic_entropy (entropy.rs:97-217; lib.rs:2226-2237): the
+ * mass-weighted covariance of the atomic fluctuations over n_frames frames
+ * xyz_frames[n_frames][n_atoms][3] (A, host memory), its eigenvalues, and the
+ * quantum harmonic oscillator entropy of the modes, as -T dS in kcal/mol, on
+ * GPU `device`.  Fewer than two frames or no atoms (the reference's None) is
+ * MMPBSA_ERR_INVALID_ARG; more than 20,000 atoms is MMPBSA_ERR_TOO_LARGE (the
+ * 3N x 3N covariance).  A negative or non-finite mass or a non-finite
+ * coordinate gives NaN, as the reference's does. */
+int mmpbsa_quasi_harmonic_entropy(int32_t device, const double *xyz_frames, int32_t n_frames,
+                                  int32_t n_atoms, const double *masses, double temperature,
+                                  double *minus_tds);
+
 /* ---- serving (no counterpart in the reference) ---------------------------
  * A server owns one worker thread and HIP stream.  Requests (frames of one
  * split, copied at submission) wait in a queue; each scheduling round takes

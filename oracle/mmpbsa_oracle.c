@@ -1745,3 +1745,114 @@ void orc_decomp_pairs(const mmpbsa_topology_view *t, const double *xyz, int32_t 
     pairset_free(&ex);
     pairset_free(&p14);
 }
+
+/* ---- quasi-harmonic entropy (entropy.rs:97-275) -------------------------- */
+static int desc_cmp(const void *a, const void *b)
+{
+    const double x = *(const double *)a, y = *(const double *)b;
+    /* NaN compares equal to everything, as partial_cmp().unwrap_or(Equal) */
+    if (x > y) return -1;
+    if (x < y) return 1;
+    return 0;
+}
+
+void orc_jacobi_eigenvalues(double *a, int32_t dim, double *eig)
+{
+    const double tol = 1e-10;
+    for (int32_t i = 0; i < dim; ++i) eig[i] = a[(int64_t)i * dim + i];
+    int changed = 1;
+    for (int sweep = 0; changed && sweep < 100; ++sweep) {
+        changed = 0;
+        for (int32_t p = 0; p < dim; ++p)
+            for (int32_t q = p + 1; q < dim; ++q) {
+                const double apq = a[(int64_t)p * dim + q];
+                if (fabs(apq) < tol) continue;
+                /* rotation angle from theta = (a_qq - a_pp) / (2 a_pq): the
+                 * smaller root t of t^2 + 2 theta t - 1 = 0 */
+                const double theta = (eig[q] - eig[p]) / (2.0 * apq);
+                double t;
+                if (fabs(theta) > 1e15) {
+                    t = 1.0 / (2.0 * theta);
+                } else {
+                    const double sg = theta >= 0.0 ? 1.0 : -1.0;
+                    t = sg / (fabs(theta) + sqrt(1.0 + theta * theta));
+                }
+                const double c = 1.0 / sqrt(1.0 + t * t);
+                const double sn = t * c;
+                const double h = sn / (1.0 + c);
+                eig[p] -= t * apq;
+                eig[q] += t * apq;
+                a[(int64_t)p * dim + q] = 0.0;
+                a[(int64_t)q * dim + p] = 0.0;
+                for (int32_t r = 0; r < dim; ++r) {
+                    if (r == p || r == q) continue;
+                    const double rp = a[(int64_t)r * dim + p], rq = a[(int64_t)r * dim + q];
+                    const double np_ = rp - sn * (rq + h * rp);
+                    const double nq = rq + sn * (rp - h * rq);
+                    a[(int64_t)r * dim + p] = np_;
+                    a[(int64_t)p * dim + r] = np_;
+                    a[(int64_t)r * dim + q] = nq;
+                    a[(int64_t)q * dim + r] = nq;
+                }
+                changed = 1;
+            }
+    }
+    qsort(eig, (size_t)dim, sizeof(double), desc_cmp);
+}
+
+int orc_quasi_harmonic_entropy(const double *xyz, int32_t nf, int32_t na, const double *masses,
+                               double temperature, double *minus_tds)
+{
+    if (nf < 2 || na <= 0) return 0;
+    const int32_t dim = 3 * na;
+    double *mean = calloc((size_t)dim, sizeof(double));
+    double *d = malloc(sizeof(double) * (size_t)dim);
+    double *cov = calloc((size_t)dim * (size_t)dim, sizeof(double));
+    double *eig = malloc(sizeof(double) * (size_t)dim);
+    /* mass-weighted means, frames in order */
+    for (int32_t f = 0; f < nf; ++f)
+        for (int32_t i = 0; i < na; ++i) {
+            const double w = sqrt(masses[i]);
+            for (int c = 0; c < 3; ++c)
+                mean[3 * i + c] += xyz[((int64_t)f * na + i) * 3 + c] * w;
+        }
+    for (int32_t k = 0; k < dim; ++k) mean[k] /= (double)nf;
+    /* covariance, upper triangle accumulated frame by frame */
+    for (int32_t f = 0; f < nf; ++f) {
+        for (int32_t i = 0; i < na; ++i) {
+            const double w = sqrt(masses[i]);
+            for (int c = 0; c < 3; ++c)
+                d[3 * i + c] = xyz[((int64_t)f * na + i) * 3 + c] * w - mean[3 * i + c];
+        }
+        for (int32_t i = 0; i < dim; ++i) {
+            const double di = d[i];
+            double *row = cov + (int64_t)i * dim;
+            for (int32_t j = i; j < dim; ++j) row[j] += di * d[j];
+        }
+    }
+    for (int32_t i = 0; i < dim; ++i)
+        for (int32_t j = i; j < dim; ++j) {
+            const double v = cov[(int64_t)i * dim + j] / (double)nf;
+            cov[(int64_t)i * dim + j] = v;
+            cov[(int64_t)j * dim + i] = v;
+        }
+    orc_jacobi_eigenvalues(cov, dim, eig);
+    /* quantum harmonic oscillator entropy per mode, SI units */
+    const double kbt = 1.380649e-23 * temperature;
+    const double hbar = 1.054571817e-34;
+    const double conv = 1e-20 * 1.66054e-27;
+    double s = 0.0;
+    for (int32_t k = 0; k < dim; ++k) {
+        const double l = eig[k];
+        if (l < 1e-10) continue;
+        const double alpha = hbar / sqrt(kbt * (l * conv));
+        if (alpha > 500.0) continue;
+        s += alpha / (exp(alpha) - 1.0) - log(1.0 - exp(-alpha));
+    }
+    *minus_tds = -temperature * 0.00198688 * s;
+    free(mean);
+    free(d);
+    free(cov);
+    free(eig);
+    return 1;
+}

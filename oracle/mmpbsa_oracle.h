@@ -21,6 +21,9 @@
  *   fractions: dense sampling.  LPBE: scipy sparse direct solve of the same
  *   discrete operator on small grids.  SASA: isolated atom = 4 pi (r+p)^2.
  *   Decomposition: sum over residues == receptor x ligand cross-pair sums.
+ *   Quasi-harmonic entropy: Jacobi eigenvalues on the reference's unit-test
+ *   matrices and against LAPACK; one oscillator's closed form and classical
+ *   limit; invariance under rigid motions; additivity over uncorrelated parts.
  */
 #ifndef MMPBSA_ORACLE_H
 #define MMPBSA_ORACLE_H
@@ -111,6 +114,15 @@ void orc_decomp_pairs(const mmpbsa_topology_view *t, const double *xyz, int32_t 
 /* Face-fraction strategy: 0 = scan all atoms per mixed edge (pb_grid.rs:742),
  * 1 = restrict to atoms in nearby cells (same interval set, same result). */
 void orc_set_face_scan(int mode);
+/* entropy.rs:227-275: eigenvalues of a symmetric dim x dim matrix (row-major,
+ * overwritten) by cyclic Jacobi rotations (off-diagonal threshold 1e-10, at
+ * most 100 sweeps), in descending order */
+void orc_jacobi_eigenvalues(double *a, int32_t dim, double *eig);
+/* entropy.rs:97-217: -T dS (kcal/mol) of frames xyz[nf][na][3] (A) with
+ * masses (amu); returns 0 (the reference's None) with fewer than two frames
+ * or no atoms, else 1 */
+int orc_quasi_harmonic_entropy(const double *xyz, int32_t nf, int32_t na, const double *masses,
+                               double temperature, double *minus_tds);
 void orc_set_threads(int n);
 void orc_set_force_divergence(int cycle);
 

@@ -88,6 +88,9 @@ def lib():
             "orc_pb_atom_terms": (None, [TV, dp, C.POINTER(PbParamsC), C.POINTER(PbResultC), dp]),
             "orc_decomp_pairs": (None, [TV, dp, C.c_int32, ip, C.c_int32, ip, ip, dp, dp]),
             "orc_set_face_scan": (None, [C.c_int]),
+            "orc_jacobi_eigenvalues": (None, [dp, C.c_int32, dp]),
+            "orc_quasi_harmonic_entropy": (C.c_int, [dp, C.c_int32, C.c_int32, dp, C.c_double,
+                                                     dp]),
             "orc_set_threads": (None, [C.c_int]),
             "orc_set_force_divergence": (None, [C.c_int]),
         }
@@ -407,3 +410,25 @@ def reimage(xyz, bonds, box):
             for a in comp:
                 x[a, d] += sh
     return x
+
+
+# ---- quasi-harmonic entropy (entropy.rs:97-275) ----------------------------
+def jacobi_eigenvalues(a):
+    """Eigenvalues (descending) of a symmetric matrix by the reference's cyclic
+    Jacobi rotations."""
+    m = np.array(a, dtype=np.float64, copy=True, order="C")
+    n = m.shape[0]
+    out = np.zeros(n)
+    lib().orc_jacobi_eigenvalues(m.ctypes.data_as(dp), n, out.ctypes.data_as(dp))
+    return out
+
+
+def quasi_harmonic_entropy(traj, masses, temperature=298.15):
+    """-T dS of frames traj [F, N, 3] (A) with masses [N] (amu), or None."""
+    t, tp = _d(traj)
+    m, mp = _d(masses)
+    out = C.c_double()
+    ok = lib().orc_quasi_harmonic_entropy(tp, t.shape[0] if t.ndim == 3 else 0,
+                                          t.shape[1] if t.ndim == 3 else 0, mp,
+                                          float(temperature), C.byref(out))
+    return out.value if ok else None

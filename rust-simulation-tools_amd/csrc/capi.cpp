@@ -1295,6 +1295,24 @@ int mmpbsa_interaction_entropy(const mmpbsa_frame_terms *frames, int32_t n, doub
     return MMPBSA_OK;
 }
 
+int mmpbsa_quasi_harmonic_entropy(int32_t device, const double *xyz_frames, int32_t n_frames,
+                                  int32_t n_atoms, const double *masses, double temperature,
+                                  double *minus_tds)
+{
+    if (!minus_tds || n_frames < 0 || n_atoms < 0 ||
+        (n_frames > 0 && n_atoms > 0 && (!xyz_frames || !masses)))
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG, "invalid argument");
+    if (n_frames < 2 || n_atoms == 0)
+        return fail(nullptr, MMPBSA_ERR_INVALID_ARG,
+                    "quasi-harmonic entropy needs >= 2 frames and >= 1 atom");
+    return guarded(nullptr, [&] {
+        require_device();
+        HIP_OK(hipSetDevice(device));
+        Worker w;
+        *minus_tds = qh_entropy(w, xyz_frames, n_frames, n_atoms, masses, temperature);
+    });
+}
+
 void mmpbsa_server_params_default(mmpbsa_server_params *p)
 {
     if (!p) return;

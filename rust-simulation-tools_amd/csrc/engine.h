@@ -155,6 +155,13 @@ struct SolveInfo {
     double final_residual = 0.0;
 };
 
+// quasi_harmonic_entropy (entropy.rs:97-217) of nf frames [nf][na][3] (A)
+// with masses (amu): -T dS in kcal/mol (NaN where the reference's is); at
+// most qh_max_dim() coordinates (3 na)
+int64_t qh_max_dim();
+double qh_entropy(Worker &w, const double *h_xyz, int nf, int na, const double *h_masses,
+                  double temperature);
+
 // ---- drivers (device coordinates already resident on w.stream) ------------
 void run_mm(Worker &w, const DevTopo &t, const double *d_xyz, mmpbsa_mm_terms *out);
 // complex, receptor and ligand MM terms of one frame from the partners' pairs

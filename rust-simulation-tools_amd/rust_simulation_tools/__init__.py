@@ -29,6 +29,7 @@ __all__ = [
     "compute_pb_energy", "compute_gb_energy_from_radii", "compute_gb_energy_cached", "GbCache",
     "compute_binding_energy_single_frame", "compute_binding_energy_frames",
     "decompose_binding_energy", "decompose_binding_energy_pb", "extract_subtopology",
+    "EntropyEstimate", "interaction_entropy", "quasi_harmonic_entropy",
 ]
 
 
@@ -235,6 +236,29 @@ def interaction_entropy(frames, temperature=298.15):
     _check(lib.mmpbsa_interaction_entropy(_terms_array(frames), len(frames), float(temperature),
                                           C.byref(out)), lib)
     return EntropyEstimate(out.value, "interaction_entropy")
+
+
+def quasi_harmonic_entropy(trajectory_3d, masses, temperature=298.15, device=0):
+    """quasi_harmonic_entropy (entropy.rs:97-217; lib.rs:2226-2237): -T dS
+    from the mass-weighted covariance of a subsystem's atomic fluctuations over
+    frames, trajectory_3d [F, N, 3] in Angstrom (passed through as the
+    reference passes it) with masses [N] in amu, computed on the GPU; None with
+    fewer than two frames or no atoms."""
+    t = np.ascontiguousarray(np.asarray(trajectory_3d, dtype=np.float64))
+    m = np.ascontiguousarray(np.asarray(masses, dtype=np.float64).reshape(-1))
+    if t.ndim != 3 or t.shape[2] != 3:
+        raise ValueError("trajectory_3d must have shape (frames, atoms, 3)")
+    if t.shape[1] != m.shape[0]:
+        raise ValueError("masses must have one value per atom")
+    if t.shape[0] < 2 or t.shape[1] == 0:
+        return None
+    lib = load_library()
+    out = C.c_double()
+    dp = C.POINTER(C.c_double)
+    _check(lib.mmpbsa_quasi_harmonic_entropy(int(device), t.ctypes.data_as(dp), t.shape[0],
+                                             t.shape[1], m.ctypes.data_as(dp),
+                                             float(temperature), C.byref(out)), lib)
+    return EntropyEstimate(out.value, "quasi_harmonic")
 
 
 class ResidueContribution:

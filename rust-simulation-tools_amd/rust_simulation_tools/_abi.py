@@ -285,6 +285,8 @@ SIGNATURES = {
                                                C.POINTER(BindingStatsC)]),
     "mmpbsa_interaction_entropy": (C.c_int, [C.POINTER(FrameTermsC), C.c_int32, C.c_double,
                                              c_double_p]),
+    "mmpbsa_quasi_harmonic_entropy": (C.c_int, [C.c_int32, c_double_p, C.c_int32, C.c_int32,
+                                                c_double_p, C.c_double, c_double_p]),
     "mmpbsa_server_params_default": (None, [C.POINTER(ServerParamsC)]),
     "mmpbsa_server_create": (C.c_int, [C.POINTER(ServerParamsC), C.c_int32, C.POINTER(P)]),
     "mmpbsa_server_destroy": (C.c_int, [P]),

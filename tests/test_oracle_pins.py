@@ -770,3 +770,74 @@ def test_decomposition_gb_weight_charge_power(oracle):
         r2 = shares[1][0] / shares[1][other]
         assert abs(shares[0][other]) > 1e-6
         assert abs(r2 / r1 - 2.5 ** 2) < 1e-9 * 6.25, (other, r2 / r1)
+
+
+# ---- quasi-harmonic entropy (entropy.rs:97-275) ---------------------------
+def test_jacobi_eigenvalues_known_answers(oracle):
+    # the reference's own unit tests: identity, and [[2, 1], [1, 2]] -> 3, 1
+    assert np.allclose(oracle.jacobi_eigenvalues(np.eye(4)), 1.0, rtol=0, atol=1e-10)
+    e = oracle.jacobi_eigenvalues([[2.0, 1.0], [1.0, 2.0]])
+    assert abs(e[0] - 3.0) < 1e-10 and abs(e[1] - 1.0) < 1e-10
+    # a dense symmetric matrix against LAPACK's (a wrong rotation fails here)
+    rng = np.random.default_rng(5)
+    b = rng.normal(size=(12, 12))
+    a = b @ b.T + np.diag(rng.uniform(0, 3, 12))
+    want = np.sort(np.linalg.eigvalsh(a))[::-1]
+    got = oracle.jacobi_eigenvalues(a)
+    assert np.allclose(got, want, rtol=1e-10, atol=1e-10)
+
+
+def _qho_entropy_over_k(x):
+    """Entropy of one quantum harmonic oscillator in units of k_B, x = hbar
+    omega / kT: S/k = x / (e^x - 1) - ln(1 - e^-x)."""
+    return x / math.expm1(x) - math.log(-math.expm1(-x))
+
+
+def test_quasi_harmonic_single_mode_closed_form(oracle):
+    # one 12 amu atom at +-1 A along x over two frames (the reference's test
+    # system): the mass-weighted variance is 12 A^2 amu (divided by F, not
+    # F - 1) along one axis and 0 along the others, so there is one mode with
+    # omega^2 = kT / lambda
+    T = 298.15
+    got = oracle.quasi_harmonic_entropy(np.array([[[1.0, 0, 0]], [[-1.0, 0, 0]]]), [12.0], T)
+    lam = 12.0 * 1e-20 * 1.66054e-27  # m^2 kg
+    x = 1.054571817e-34 / math.sqrt(1.380649e-23 * T * lam)
+    want = -T * 0.00198688 * _qho_entropy_over_k(x)
+    assert got == pytest.approx(want, rel=1e-12)
+    # the classical limit of the same expression (hbar omega << kT):
+    # S/k -> 1 - ln x; at this x (~0.3) the next term x^2 / 24 decides
+    assert _qho_entropy_over_k(x) == pytest.approx(1 - math.log(x) + x * x / 24, rel=2e-4)
+
+
+def test_quasi_harmonic_limits_and_none(oracle):
+    # fewer than two frames or no atoms: the reference's None
+    assert oracle.quasi_harmonic_entropy(np.zeros((1, 3, 3)), np.ones(3)) is None
+    assert oracle.quasi_harmonic_entropy(np.zeros((4, 0, 3)), np.ones(0)) is None
+    # no motion: every mode is zero and skipped
+    still = np.tile(np.arange(9.0).reshape(1, 3, 3), (5, 1, 1))
+    assert oracle.quasi_harmonic_entropy(still, [12.0, 14.0, 16.0]) == 0.0
+    # low temperature: x large, S/k ~ (x + 1) e^-x, so -T dS -> 0 from below
+    v = oracle.quasi_harmonic_entropy(np.array([[[0.01, 0, 0]], [[-0.01, 0, 0]]]), [1.0], 5.0)
+    assert -1e-30 < v <= 0.0
+
+
+def test_quasi_harmonic_invariances(oracle):
+    rng = np.random.default_rng(11)
+    na, nf = 6, 40
+    masses = rng.choice([1.008, 12.011, 14.007, 15.999], na)
+    base = rng.normal(scale=3.0, size=(na, 3))
+    traj = base + rng.normal(scale=0.2, size=(nf, na, 3))
+    s0 = oracle.quasi_harmonic_entropy(traj, masses)
+    # a rigid rotation and a translation of every frame leave the covariance's
+    # spectrum unchanged
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    moved = traj @ q.T + np.array([5.0, -2.0, 1.5])
+    assert oracle.quasi_harmonic_entropy(moved, masses) == pytest.approx(s0, rel=1e-9)
+    # independent partners: motions whose cross products sum to zero over the
+    # frames give a block-diagonal covariance, and additive entropies
+    a = np.array([[[0.3, 0.1, 0.0]], [[-0.3, -0.1, 0.0]], [[0.3, 0.1, 0.0]], [[-0.3, -0.1, 0.0]]])
+    b = np.array([[[0.0, 0.2, 0.1]], [[0.0, 0.2, 0.1]], [[0.0, -0.2, -0.1]], [[0.0, -0.2, -0.1]]])
+    sa = oracle.quasi_harmonic_entropy(a, [12.0])
+    sb = oracle.quasi_harmonic_entropy(b, [16.0])
+    sab = oracle.quasi_harmonic_entropy(np.concatenate([a, b], axis=1), [12.0, 16.0])
+    assert sab == pytest.approx(sa + sb, rel=1e-12)
